@@ -1,0 +1,139 @@
+#!/usr/bin/env python
+"""Benchmark: 28-qubit complex128 random-circuit throughput on one MI355X.
+
+Workload: the seeded 28-qubit, depth-40 random circuit (RX/RY/RZ/PHASE 50%,
+CNOT/CZ/ISWAP/SWAP 45%, CCNOT 5%, targets uniform over all qubits), state
+resident in HBM (4 GiB).  One step = the whole circuit applied once, in
+place, to the working state.  Prints one JSON line:
+
+  value      gates/s of the default executor (fused when available)
+  roofline   dominant kernel: algorithmic bytes per launch (32 * 2^28 B: one
+             read + one write of the state) / its average duration, timed
+             with HIP events on the launching stream inside the timed region
+  cpu_baseline  the numpy oracle (np.einsum per gate, 1 thread) on a
+             bounded prefix of the same circuit, on this host
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+N_QUBITS = 28
+DEPTH = 40
+SEED = 2024
+HBM_PEAK_GBS = 8000.0
+METRIC = "28q c128 gates/s as % HBM roofline; QAOA value+grad/s vs plain, same GPU"
+
+
+def cpu_baseline(desc, n, ngates):
+    """Time the oracle (numpy einsum per gate) on the first ``ngates`` gates
+    of the circuit at the full size, on this host."""
+    import numpy as np
+    from oracle import qf_oracle as O
+    ket = np.zeros(2 ** n, dtype=np.complex128)
+    ket[0] = 1.0
+    ket = ket.reshape([2] * n)
+    qubits = tuple(range(n))
+    t0 = time.perf_counter()
+    for g in desc[:ngates]:
+        ket = O.run_gate(ket, qubits, O.gate_matrix(g[0], *g[1]), g[2])
+    dt = time.perf_counter() - t0
+    del ket
+    return {"value": ngates / dt, "unit": "gates/s", "cores": 1, "kind": "port",
+            "sample": "first %d gates of the %d-qubit depth-%d circuit, numpy einsum "
+                      "per gate (single-threaded), host os.cpu_count()=%d, affinity=%d"
+                      % (ngates, n, DEPTH, os.cpu_count(), len(os.sched_getaffinity(0)))}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--qubits", type=int, default=N_QUBITS)
+    ap.add_argument("--executor", choices=["auto", "fused", "unfused"], default="auto")
+    ap.add_argument("--cpu-gates", type=int, default=6)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(sharding is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import quantumflow_amd as qf
+    from quantumflow_amd import fusion
+    from quantumflow_amd.workloads import random_mixed_circuit, build_gates
+
+    n = args.qubits
+    desc = random_mixed_circuit(n, DEPTH, seed=SEED)
+    gates = build_gates(desc)
+    qubits = tuple(range(n))
+    print("# workload: %d gates on %d qubits" % (len(gates), n), file=sys.stderr, flush=True)
+
+    ket = qf.zero_state(n)
+    work = qf.backend.as_device(ket.tensor).clone()
+    state = qf.State(work.view([2] * n), qubits)
+
+    plan = None
+    if args.executor in ("auto", "fused"):
+        plan = fusion.plan_for(gates, qubits)
+        if plan is None and args.executor == "fused":
+            raise SystemExit("fused executor unavailable")
+
+    def step(prof=None):
+        if plan is not None:
+            plan.run_inplace(work, prof=prof)
+        else:
+            fusion.apply_gates_inplace(gates, qubits, work, prof)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    prof = fusion.KernelTimer()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(prof)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+
+    gates_per_step = len(gates)
+    value = gates_per_step * args.steps / dt
+    ms = dt / args.steps * 1e3
+    bytes_per_pass = 32.0 * 2 ** n
+    kname, avg_ms, launches = prof.dominant()
+    achieved = bytes_per_pass / (avg_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": kname, "avg_ms": round(avg_ms, 4), "launches_timed": launches,
+            "bytes_per_launch": bytes_per_pass}
+
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": "gates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "complex128", "data": "synthetic",
+        "config": {"workload": "random_mixed_circuit", "qubits": n, "depth": DEPTH,
+                   "seed": SEED, "gates": gates_per_step,
+                   "executor": "fused" if plan is not None else "unfused",
+                   "passes_per_step": (plan.n_passes if plan is not None else gates_per_step)},
+        "gates_per_s_pct_of_hbm_roofline": round(100.0 * value * bytes_per_pass / (HBM_PEAK_GBS * 1e9), 2),
+        "roofline": roof,
+    }
+    if not args.no_cpu and args.cpu_gates > 0:
+        del state
+        out["cpu_baseline"] = cpu_baseline(desc, n, args.cpu_gates)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,146 @@
+/*
+ * qfx.h -- C-ABI of the MI355X (gfx950) state-vector engine behind
+ * quantumflow_amd.backend (the "rocm" QuantumFlow backend).
+ *
+ * Conventions
+ *   - A state of n qubits is 2^n complex128 amplitudes, interleaved
+ *     (re, im) doubles, contiguous, in device memory.  Flat index bit
+ *     (n-1-axis) belongs to tensor axis `axis` (big-endian in axis order,
+ *     quantumflow/qubits.py:63-68).
+ *   - `targets[j]` is the index BIT POSITION of gate qubit j, i.e.
+ *     n-1-axis_j.  Gate qubit 0 is the most significant bit of the gate's
+ *     row/column index (quantumflow/qubits.py:70-80).
+ *   - Matrices are row-major 2^k x 2^k complex128, row = output.
+ *   - Host pointers are marked (host); everything else is device memory
+ *     owned by the caller.  The library never frees caller memory.
+ *   - `stream` is a hipStream_t (NULL = default stream).  Calls only
+ *     enqueue work; none synchronises.
+ *   - Every function returns 0 on success or a negative QFX_E* code and
+ *     never throws; qfx_last_error() gives the message (thread-local).
+ *   - out == in is allowed for every apply call (each work item reads its
+ *     whole amplitude group before writing it).
+ */
+#ifndef QFX_H
+#define QFX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QFX_OK 0
+#define QFX_EINVAL (-1)   /* bad argument (maps to ValueError)       */
+#define QFX_ERANGE (-2)   /* size out of supported range             */
+#define QFX_EHIP (-3)     /* HIP runtime error                       */
+#define QFX_ENODEV (-4)   /* no usable gfx950 device                 */
+#define QFX_EKERNEL (-5)  /* a kernel reported failure via its flag  */
+
+#define QFX_MAX_QUBITS 34 /* 2^34 amplitudes = 256 GiB, fits 288 GB HBM */
+#define QFX_MAX_DENSE_K 12
+
+typedef struct {
+  double re, im;
+} qfx_c128;
+
+/* ---- library ---------------------------------------------------------- */
+
+/* Text of the last error on this thread (never NULL). */
+const char* qfx_last_error(void);
+
+/* Library version string. */
+const char* qfx_version(void);
+
+/* Initialise for the current device; returns QFX_ENODEV if it is not gfx950.
+ * Safe to call more than once.  Fills (host) props[8] when props != NULL:
+ * {CUs, LDS bytes per block, warp size, L2 bytes, clock kHz, total MiB,
+ *  max threads per block, 0}. */
+int qfx_init(int64_t* props);
+
+/* ---- gate application (replaces numpybk.tensormul,
+ *      quantumflow/backend/numpybk.py:159-214, on the Gate.run path
+ *      quantumflow/ops.py:161-166) ---------------------------------------- */
+
+/* Dense k-qubit gate, 1 <= k <= 3, matrix passed by value from the host. */
+int qfx_apply_matrix(void* stream, void* out, const void* in, int n, int k,
+                     const int* targets /*host,k*/,
+                     const qfx_c128* matrix /*host,4^k*/);
+
+/* Dense k-qubit gate, 1 <= k <= QFX_MAX_DENSE_K, matrix in device memory. */
+int qfx_apply_matrix_dev(void* stream, void* out, const void* in, int n,
+                         int k, const int* targets /*host,k*/,
+                         const void* matrix /*device,4^k complex128*/);
+
+/* Diagonal k-qubit gate (RZ, PHASE, CZ, CPHASE, ZZ, ...), 1 <= k <= 5:
+ * out[i] = diag[r(i)] * in[i]. */
+int qfx_apply_diagonal(void* stream, void* out, const void* in, int n, int k,
+                       const int* targets /*host,k*/,
+                       const qfx_c128* diag /*host,2^k*/);
+
+/* Monomial k-qubit gate (X, CNOT, SWAP, ISWAP, CCNOT, CSWAP, ...),
+ * 1 <= k <= 5: row r of the gate has its single non-zero phase[r] in
+ * column src[r].  phase == NULL means all ones: the result is then a pure
+ * permutation of the input, bit for bit. */
+int qfx_apply_monomial(void* stream, void* out, const void* in, int n, int k,
+                       const int* targets /*host,k*/,
+                       const int* src /*host,2^k*/,
+                       const qfx_c128* phase /*host,2^k or NULL*/);
+
+/* In-place variants that only touch the rows a diagonal or monomial gate
+ * changes (identity rows are skipped).  state is read and written. */
+int qfx_apply_diagonal_inplace(void* stream, void* state, int n, int k,
+                               const int* targets, const qfx_c128* diag);
+int qfx_apply_monomial_inplace(void* stream, void* state, int n, int k,
+                               const int* targets, const int* src,
+                               const qfx_c128* phase);
+
+/* ---- readouts (quantumflow/states.py:113-147, numpybk.py:125-128,
+ *      quantumflow/qubits.py:180-182) ------------------------------------- */
+
+/* Bytes of device workspace the reductions below need for `len` elements. */
+int64_t qfx_reduce_workspace_bytes(int64_t len);
+
+/* result[0] = sum_i conj(a_i) b_i   (np.vdot), complex128 in device memory */
+int qfx_inner(void* stream, void* result, const void* a, const void* b,
+              int64_t len, void* workspace);
+/* out[i] = |in_i| (hypot), float64 */
+int qfx_abs(void* stream, void* out, const void* in, int64_t len);
+/* out[i] = |in_i|^2, float64 */
+int qfx_abs2(void* stream, void* out, const void* in, int64_t len);
+/* result[0] = sum_i |psi_i|^2 * diag_i   (diag float64, device) */
+int qfx_expect_diag(void* stream, void* result, const void* psi,
+                    const void* diag, int64_t len, void* workspace);
+/* out[a] = <Z_a> = sum_i |psi_i|^2 (1 - 2 bit_{n-1-a}(i)) for every axis a */
+int qfx_expect_z(void* stream, void* out, const void* psi, int n,
+                 void* workspace);
+/* result[0] = sum_i |psi_i|^2 (float64) */
+int qfx_norm2(void* stream, void* result, const void* psi, int64_t len,
+              void* workspace);
+
+/* ---- states ------------------------------------------------------------ */
+
+/* Qubit relabelling (QubitVector.permute, quantumflow/qubits.py:145-161):
+ * out[i] = in[j] where bit src_bit[b] of j equals bit b of i.  out != in. */
+int qfx_permute_qubits(void* stream, void* out, const void* in, int n,
+                       const int* src_bit /*host,n*/);
+
+/* psi = |0...0> on n qubits. */
+int qfx_zero_state(void* stream, void* psi, int n);
+/* psi = |+...+> = 2^(-n/2) everywhere. */
+int qfx_plus_state(void* stream, void* psi, int n);
+
+/* ---- QAOA MaxCut (quantumflow/qaoa.py) --------------------------------- */
+
+/* Cut table C(z) = sum_e w_e [z_u != z_v] over all 2^n basis states, edges
+ * summed in the given order (bitwise equal to graph_cuts,
+ * quantumflow/qaoa.py:68-81).  edges: device int32 [m][2] with node labels
+ * 0..n-1 (node u <-> axis u); weights: device float64 [m] or NULL for 1.0;
+ * m <= 4096. */
+int qfx_cut_table(void* stream, void* out, int n, const void* edges,
+                  const void* weights, int m);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QFX_H */

@@ -1,0 +1,464 @@
+"""QuantumFlow backend for the AMD MI355X (gfx950): torch-ROCm tensors whose
+hot functions run hand-written HIP kernels through the C-ABI in
+include/qfx.h (loaded by quantumflow_amd._native).
+
+Exports every name of the reference's backend API
+(quantumflow/backend/__init__.py:25-35) with numpy-backend semantics
+(quantumflow/backend/numpybk.py, docs/source/backends.rst:71-92):
+
+* tensors are ``torch.complex128`` on the GPU, shaped [2]*N (a view of a
+  flat contiguous buffer);
+* ``tensormul`` (numpybk.py:159-214), ``inner`` (numpybk.py:125-128) and
+  ``absolute`` on complex tensors (used by State.probabilities,
+  states.py:113-119) call HIP kernels;
+* the extensions ``probabilities``, ``expectation``, ``expectation_z`` and
+  ``norm`` give the State readouts (states.py:113-147, qubits.py:180-182)
+  as single fused kernels;
+* everything else is shape plumbing done with torch on the device, or plain
+  numpy on host scalars (gate constructors build their small matrices on
+  the host with the same numpy expressions as the reference).
+
+There is no CPU fallback: without libqfx.so or a GPU every hot call raises.
+"""
+
+import math
+import string
+import typing
+
+import numpy as np
+import torch
+
+from .. import _native
+from ..config import DEVICE, SEED
+
+TL = torch
+name = 'rocm'
+version = torch.__version__
+__version__ = version
+
+CTYPE = torch.complex128
+"""Complex dtype of backend tensors"""
+FTYPE = torch.float64
+"""Real dtype of backend tensors"""
+TENSOR = torch.Tensor
+BKTensor = typing.Any
+TensorLike = typing.Any
+
+MAX_QUBITS = 33
+"""Out-of-place gate application holds two states: 2 x 128 GiB at 33 qubits
+fits the MI355X's 288 GB of HBM3E."""
+
+EINSUM_SUBSCRIPTS = string.ascii_lowercase + string.ascii_uppercase
+
+pi = np.pi
+
+_NP_CTYPE = np.complex128
+_inited = set()
+_workspace = {}
+
+
+# ---------------------------------------------------------------------------
+# device plumbing
+# ---------------------------------------------------------------------------
+
+def gpu_available() -> bool:
+    """True if a GPU is visible to torch and libqfx.so loads."""
+    try:
+        _native.load()
+    except Exception:
+        return False
+    return torch.cuda.is_available()
+
+
+def _ensure_device():
+    dev = torch.cuda.current_device() if torch.cuda.is_available() else None
+    if dev is None:
+        raise RuntimeError('quantumflow_amd rocm backend: no GPU visible to torch '
+                           '(there is no CPU fallback)')
+    if dev not in _inited:
+        props = (_native.ctypes.c_int64 * 8)()
+        _native.call('qfx_init', props)
+        _inited.add(dev)
+    return dev
+
+
+def stream_handle():
+    """hipStream_t of torch's current stream (0 = default stream)."""
+    _ensure_device()
+    return torch.cuda.current_stream().cuda_stream
+
+
+def workspace(nbytes):
+    """Device scratch of at least nbytes for reductions, reused in stream
+    order (all qfx calls go to torch's current stream)."""
+    dev = _ensure_device()
+    key = (dev, torch.cuda.current_stream().cuda_stream)
+    ws = _workspace.get(key)
+    if ws is None or ws.numel() < nbytes:
+        ws = torch.empty(max(int(nbytes), 1 << 20), dtype=torch.uint8,
+                         device='cuda')
+        _workspace[key] = ws
+    return ws
+
+
+def _ptr(t):
+    return _native.ctypes.c_void_p(t.data_ptr())
+
+
+def as_device(tensor):
+    """Flat contiguous complex128 device tensor (no copy if already one)."""
+    _ensure_device()
+    if isinstance(tensor, torch.Tensor):
+        t = tensor
+        if t.dtype != CTYPE or t.device.type != 'cuda':
+            t = t.to(device=DEVICE, dtype=CTYPE)
+    else:
+        t = torch.from_numpy(np.ascontiguousarray(
+            np.asarray(tensor, dtype=_NP_CTYPE))).to(DEVICE)
+    if not t.is_contiguous():
+        t = t.contiguous()
+    return t.reshape(-1)
+
+
+def _host(tensor):
+    if isinstance(tensor, torch.Tensor):
+        return tensor.detach().cpu().numpy()
+    return np.asarray(tensor)
+
+
+# ---------------------------------------------------------------------------
+# conversions (numpybk.py:79-112)
+# ---------------------------------------------------------------------------
+
+def ccast(value: complex) -> TensorLike:
+    """Cast value to complex tensor (if necessary)"""
+    return value
+
+
+def fcast(value: float) -> TensorLike:
+    """Cast value to float tensor (if necessary)"""
+    return value
+
+
+def astensor(array: TensorLike) -> BKTensor:
+    """Convert to a complex128 device tensor (shape preserved)."""
+    _ensure_device()
+    if isinstance(array, torch.Tensor):
+        return array.to(device=DEVICE, dtype=CTYPE)
+    arr = np.asarray(array, dtype=_NP_CTYPE)
+    return torch.from_numpy(np.ascontiguousarray(arr)).to(DEVICE)
+
+
+def astensorproduct(array: TensorLike) -> BKTensor:
+    """astensor, reshaped to [2]*N (numpybk.py:105-112)."""
+    tensor = astensor(array)
+    n = int(math.log2(tensor.numel()))
+    return tensor.reshape([2] * n)
+
+
+def evaluate(tensor: BKTensor) -> TensorLike:
+    """The value as a numpy object (device -> host copy; the only sync)."""
+    if isinstance(tensor, torch.Tensor):
+        arr = tensor.detach().cpu().numpy()
+        return arr[()] if arr.ndim == 0 else arr
+    return tensor
+
+
+def rank(tensor: BKTensor) -> int:
+    return len(tensor.shape)
+
+
+def size(tensor: BKTensor) -> int:
+    if isinstance(tensor, torch.Tensor):
+        return tensor.numel()
+    return np.size(tensor)
+
+
+def set_random_seed(seed: int) -> None:
+    np.random.seed(seed)
+    torch.manual_seed(seed)
+
+
+def getitem(tensor: BKTensor, key: typing.Any) -> BKTensor:
+    return tensor.__getitem__(key)
+
+
+# ---------------------------------------------------------------------------
+# elementwise / shape functions: torch on device tensors, numpy otherwise
+# ---------------------------------------------------------------------------
+
+def _dispatch(tfn, nfn):
+    def f(x, *args, **kwargs):
+        if isinstance(x, torch.Tensor):
+            return tfn(x, *args, **kwargs)
+        return nfn(x, *args, **kwargs)
+    f.__name__ = nfn.__name__
+    return f
+
+
+conj = _dispatch(torch.conj, np.conj)
+cos = _dispatch(torch.cos, np.cos)
+sin = _dispatch(torch.sin, np.sin)
+exp = _dispatch(torch.exp, np.exp)
+sqrt = _dispatch(torch.sqrt, np.sqrt)
+arccos = _dispatch(torch.arccos, np.arccos)
+imag = _dispatch(lambda t: torch.imag(t) if t.is_complex() else torch.zeros_like(t), np.imag)
+diag = _dispatch(torch.diag, np.diag)
+trace = _dispatch(torch.trace, np.trace)
+
+
+def real(tensor):
+    if isinstance(tensor, torch.Tensor):
+        return torch.real(tensor) if tensor.is_complex() else tensor
+    return np.real(tensor)
+
+
+def minimum(a, b):
+    if isinstance(a, torch.Tensor) or isinstance(b, torch.Tensor):
+        a = a if isinstance(a, torch.Tensor) else torch.tensor(a, device=DEVICE)
+        b = b if isinstance(b, torch.Tensor) else torch.tensor(b, device=a.device,
+                                                                dtype=a.dtype)
+        return torch.minimum(a, b)
+    return np.minimum(a, b)
+
+
+def sum(tensor, *args, **kwargs):  # noqa: A001 (numpy name)
+    if isinstance(tensor, torch.Tensor):
+        return torch.sum(tensor, *args, **kwargs)
+    return np.sum(tensor, *args, **kwargs)
+
+
+def reshape(tensor, shape):
+    if isinstance(tensor, torch.Tensor):
+        return torch.reshape(tensor, tuple(shape))
+    return np.reshape(tensor, shape)
+
+
+def transpose(tensor, perm=None):
+    if isinstance(tensor, torch.Tensor):
+        if perm is None:
+            perm = list(reversed(range(tensor.dim())))
+        return tensor.permute(*perm).contiguous()
+    return np.transpose(tensor, perm)
+
+
+def matmul(a, b):
+    if isinstance(a, torch.Tensor) or isinstance(b, torch.Tensor):
+        return torch.matmul(astensor(a), astensor(b))
+    return np.matmul(a, b)
+
+
+def outer(a, b):
+    if isinstance(a, torch.Tensor) or isinstance(b, torch.Tensor):
+        a = astensor(a).reshape(-1)
+        b = astensor(b).reshape(-1)
+        return torch.outer(a, b)
+    return np.outer(a, b)
+
+
+def einsum(subscripts, *operands):
+    if any(isinstance(o, torch.Tensor) for o in operands):
+        return torch.einsum(subscripts, *[astensor(o) for o in operands])
+    return np.einsum(subscripts, *operands)
+
+
+def cis(theta: float) -> BKTensor:
+    """cis(theta) = exp(i theta) (numpybk.py:131-137)."""
+    if isinstance(theta, torch.Tensor):
+        return torch.exp(1j * theta)
+    return np.exp(theta * 1.0j)
+
+
+def productdiag(tensor: BKTensor) -> BKTensor:
+    """Diagonal of the product tensor as [2]*(N/2) (numpybk.py:150-156)."""
+    n = rank(tensor)
+    t = reshape(tensor, [2 ** (n // 2), 2 ** (n // 2)])
+    t = diag(t)
+    return reshape(t, [2] * (n // 2))
+
+
+# ---------------------------------------------------------------------------
+# the hot path: gate application
+# ---------------------------------------------------------------------------
+
+class GateKind:
+    """Classification of a 2^k x 2^k gate matrix for kernel selection."""
+    DENSE, DIAGONAL, MONOMIAL = 'dense', 'diagonal', 'monomial'
+
+    __slots__ = ('kind', 'k', 'matrix', 'diag', 'src', 'phase')
+
+    def __init__(self, matrix):
+        m = np.asarray(matrix, dtype=_NP_CTYPE)
+        dim = int(round(math.sqrt(m.size)))
+        m = m.reshape(dim, dim)
+        self.k = int(math.log2(dim))
+        self.matrix = np.ascontiguousarray(m)
+        self.diag = self.src = self.phase = None
+        nz = m != 0
+        if self.k <= 5 and not np.any(nz & ~np.eye(dim, dtype=bool)):
+            self.kind = GateKind.DIAGONAL
+            self.diag = np.diag(m).copy()
+        elif self.k <= 5 and np.all(nz.sum(axis=1) == 1) and np.all(nz.sum(axis=0) == 1):
+            self.kind = GateKind.MONOMIAL
+            self.src = np.argmax(nz, axis=1).astype(np.int32)
+            ph = m[np.arange(dim), self.src]
+            self.phase = None if np.all(ph == 1.0) else ph
+        else:
+            self.kind = GateKind.DENSE
+
+
+def classify(matrix):
+    return GateKind(matrix)
+
+
+def apply_gate(out, state, n, bits, kind, inplace_sparse=False):
+    """Launch the kernel for `kind` on flat device tensors.  ``bits`` are
+    index bit positions (n-1-axis) in gate-qubit order."""
+    s = stream_handle()
+    k = kind.k
+    tb = _native.int_array(bits)
+    if kind.kind == GateKind.DIAGONAL:
+        d = _native.c128_array(kind.diag)
+        if inplace_sparse and out.data_ptr() == state.data_ptr():
+            _native.call('qfx_apply_diagonal_inplace', s, _ptr(out), n, k, tb, d)
+        else:
+            _native.call('qfx_apply_diagonal', s, _ptr(out), _ptr(state), n, k, tb, d)
+    elif kind.kind == GateKind.MONOMIAL:
+        src = _native.int_array(kind.src)
+        ph = None if kind.phase is None else _native.c128_array(kind.phase)
+        if inplace_sparse and out.data_ptr() == state.data_ptr():
+            _native.call('qfx_apply_monomial_inplace', s, _ptr(out), n, k, tb, src, ph)
+        else:
+            _native.call('qfx_apply_monomial', s, _ptr(out), _ptr(state), n, k, tb, src, ph)
+    elif k <= 3:
+        m = _native.c128_array(kind.matrix.reshape(-1))
+        _native.call('qfx_apply_matrix', s, _ptr(out), _ptr(state), n, k, tb, m)
+    else:
+        dm = torch.from_numpy(kind.matrix.reshape(-1)).to(out.device)
+        if out.data_ptr() == state.data_ptr():
+            tmp = torch.empty_like(state)
+            _native.call('qfx_apply_matrix_dev', s, _ptr(tmp), _ptr(state), n, k, tb, _ptr(dm))
+            out.copy_(tmp)
+        else:
+            _native.call('qfx_apply_matrix_dev', s, _ptr(out), _ptr(state), n, k, tb, _ptr(dm))
+    return out
+
+
+def tensormul(tensor0: BKTensor, tensor1: BKTensor,
+              indices: typing.List[int]) -> BKTensor:
+    """Contract gate ``tensor0`` (rank 2K, [ket_out..., bra_in...]) into
+    ``tensor1`` (rank N) on axes ``indices``; returns a new [2]*N device
+    tensor (numpybk.py:159-214).  ``tensor0`` may be a host array or a
+    GateKind (no device round trip) or a device tensor (copied to the host:
+    a synchronising call)."""
+    if isinstance(tensor0, GateKind):
+        kind = tensor0
+        k = kind.k
+    else:
+        k = rank(tensor0) // 2
+        kind = None
+    n = rank(tensor1)
+    assert k == len(indices)
+    if kind is None:
+        kind = classify(_host(tensor0).reshape(2 ** k, 2 ** k))
+    state = as_device(tensor1)
+    if n != int(math.log2(state.numel())):
+        raise ValueError('tensormul: state is not a product tensor')
+    out = torch.empty_like(state)
+    bits = [n - 1 - int(i) for i in indices]
+    apply_gate(out, state, n, bits, kind)
+    return out.view([2] * n)
+
+
+# ---------------------------------------------------------------------------
+# readouts
+# ---------------------------------------------------------------------------
+
+def inner(tensor0: BKTensor, tensor1: BKTensor) -> BKTensor:
+    """sum conj(t0) t1 over the flattened tensors (np.vdot), as a 0-dim
+    complex128 device tensor (numpybk.py:125-128)."""
+    a = as_device(tensor0)
+    b = as_device(tensor1)
+    if a.numel() != b.numel():
+        raise ValueError('inner: tensors differ in size')
+    res = torch.empty((), dtype=CTYPE, device=a.device)
+    ws = workspace(_native.load().qfx_reduce_workspace_bytes(a.numel()))
+    _native.call('qfx_inner', stream_handle(), _ptr(res), _ptr(a), _ptr(b), a.numel(), _ptr(ws))
+    return res
+
+
+def absolute(tensor: BKTensor) -> BKTensor:
+    """|x| elementwise; complex device tensors use the hypot kernel."""
+    if isinstance(tensor, torch.Tensor):
+        if tensor.is_complex() and tensor.dim() > 0:
+            a = as_device(tensor)
+            out = torch.empty(a.numel(), dtype=FTYPE, device=a.device)
+            _native.call('qfx_abs', stream_handle(), _ptr(out), _ptr(a), a.numel())
+            return out.view(tensor.shape)
+        return torch.abs(tensor)
+    return np.absolute(tensor)
+
+
+def probabilities(tensor: BKTensor) -> BKTensor:
+    """|psi|^2 as float64 [2]*N in one kernel (states.py:113-119)."""
+    a = as_device(tensor)
+    out = torch.empty(a.numel(), dtype=FTYPE, device=a.device)
+    _native.call('qfx_abs2', stream_handle(), _ptr(out), _ptr(a), a.numel())
+    return out.view(tuple(tensor.shape))
+
+
+def _real_diag(diag, numel):
+    if isinstance(diag, torch.Tensor):
+        d = diag.to(device=DEVICE)
+        d = torch.real(d) if d.is_complex() else d
+        d = d.to(FTYPE)
+    else:
+        d = torch.from_numpy(np.ascontiguousarray(
+            np.real(np.asarray(diag, dtype=_NP_CTYPE)).astype(np.float64))).to(DEVICE)
+    d = d.contiguous().reshape(-1)
+    if d.numel() != numel:
+        raise ValueError('expectation: diagonal does not match the state size')
+    return d
+
+
+def expectation(diag_hermitian: BKTensor, tensor: BKTensor) -> BKTensor:
+    """sum real(diag) |psi|^2 as a 0-dim float64 device tensor, one fused
+    reduction (states.py:131-147, exact mode)."""
+    a = as_device(tensor)
+    d = _real_diag(diag_hermitian, a.numel())
+    res = torch.empty((), dtype=FTYPE, device=a.device)
+    ws = workspace(_native.load().qfx_reduce_workspace_bytes(a.numel()))
+    _native.call('qfx_expect_diag', stream_handle(), _ptr(res), _ptr(a), _ptr(d), a.numel(),
+                 _ptr(ws))
+    return res
+
+
+def expectation_z(tensor: BKTensor) -> BKTensor:
+    """<Z_a> for every axis a in one pass; float64 device tensor [N]."""
+    a = as_device(tensor)
+    n = int(math.log2(a.numel()))
+    out = torch.empty(n, dtype=FTYPE, device=a.device)
+    ws = workspace(_native.load().qfx_reduce_workspace_bytes(a.numel()))
+    _native.call('qfx_expect_z', stream_handle(), _ptr(out), _ptr(a), n, _ptr(ws))
+    return out
+
+
+def norm(tensor: BKTensor) -> BKTensor:
+    """|<psi|psi>| (the squared 2-norm, qubits.py:180-182), 0-dim float64."""
+    a = as_device(tensor)
+    res = torch.empty((), dtype=FTYPE, device=a.device)
+    ws = workspace(_native.load().qfx_reduce_workspace_bytes(a.numel()))
+    _native.call('qfx_norm2', stream_handle(), _ptr(res), _ptr(a), a.numel(), _ptr(ws))
+    return res
+
+
+def zero_state_tensor(n: int) -> BKTensor:
+    """|0...0> on n qubits, filled on the device."""
+    _ensure_device()
+    t = torch.empty(2 ** n, dtype=CTYPE, device=DEVICE)
+    _native.call('qfx_zero_state', stream_handle(), _ptr(t), n)
+    return t.view([2] * n)
+
+
+if SEED is not None:
+    set_random_seed(SEED)

@@ -1,0 +1,143 @@
+"""GPU runs of the QuantumFlow API (Gate.run, Circuit.run, State, Gate @ Gate,
+Circuit.asgate, DAGCircuit) on the reference's known-answer circuits and on
+the seeded benchmark circuits, compared with the oracle."""
+import numpy as np
+import pytest
+from math import pi
+
+from oracle import qf_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def qf():
+    import quantumflow_amd as qf
+    assert qf.backend.gpu_available()
+    return qf
+
+
+def build(qf, desc):
+    from quantumflow_amd.workloads import build_gates
+    return build_gates(desc)
+
+
+def test_bits_kat(qf):
+    # tests/test_gates.py:59-77
+    for q in range(4):
+        ket = qf.X(q).run(qf.zero_state(4))
+        idx = [0, 0, 0, 0]
+        idx[q] = 1
+        assert ket.vec.asarray()[tuple(idx)] == 1.0
+    ket = qf.zero_state(8)
+    for q in (2, 4, 6):
+        ket = qf.X(q).run(ket)
+    assert ket.vec.asarray()[0, 0, 1, 0, 1, 0, 1, 0] == 1.0
+
+
+def test_state_labels_kat(qf):
+    # tests/test_states.py:54-66
+    ket = qf.zero_state(range(3, -1, -1))
+    ket = qf.X(0).run(ket)
+    ket = qf.X(1).run(ket)
+    assert ket.vec.asarray()[0, 0, 1, 1] == 1.0
+    ket = ket.relabel([0, 1, 3, 4])
+    assert ket.vec.asarray()[0, 0, 1, 1] == 1.0
+    ket = ket.permute([4, 3, 0, 1])
+    assert ket.vec.asarray()[1, 1, 0, 0] == 1.0
+
+
+def test_expectation_kats(qf):
+    ket = qf.H(0).run(qf.zero_state(1))
+    assert abs(qf.asarray(ket.expectation([0.4, 0.6])) - 0.5) < 1e-15
+    m = np.zeros([2] * 4)
+    m[0, 0, 0, 0], m[1, 0, 0, 0], m[0, 1, 0, 0], m[0, 0, 1, 0], m[0, 0, 0, 1] = 42, 1, 2, 3, 4
+    assert qf.asarray(qf.zero_state(4).expectation(m)) == 42
+    assert abs(qf.asarray(qf.w_state(4).expectation(m)) - 2.5) < 1e-15
+
+
+def test_qvm_wavefunction_kat(qf):
+    # tests/test_circuits.py:43-60
+    true = np.array([0.00167784 + 1.00210180e-05j, 0.50000000 - 4.99997185e-01j,
+                     0.50000000 - 4.99997185e-01j, 0.00167784 + 1.00210180e-05j])
+    c = qf.Circuit([qf.RY(pi / 2, 0), qf.RX(pi, 0), qf.RY(pi / 2, 1), qf.RX(pi, 1),
+                    qf.CNOT(0, 1), qf.RX(-pi / 2, 1), qf.RY(4.71572463191, 1),
+                    qf.RX(pi / 2, 1), qf.CNOT(0, 1), qf.RX(-2 * 2.74973750579, 0),
+                    qf.RX(-2 * 2.74973750579, 1)])
+    ket = c.run(qf.zero_state(2))
+    assert qf.states_close(ket, qf.State(true.reshape(2, 2)))
+    dag = qf.DAGCircuit(c)
+    ket2 = dag.H.run(dag.run(qf.zero_state(2)))
+    assert qf.states_close(ket2, qf.zero_state(2))
+
+
+def test_qft_kat(qf):
+    c = qf.Circuit([qf.X(2)])
+    c.extend(qf.qft_circuit([0, 1, 2]))
+    ket = c.run(qf.zero_state(3))
+    true = np.array([0.35355339 + 0.j, 0.25 + 0.25j, 0.35355339j, -0.25 + 0.25j,
+                     -0.35355339 + 0.j, -0.25 - 0.25j, -0.35355339j, 0.25 - 0.25j])
+    assert np.max(np.abs(ket.vec.asarray().reshape(-1) - true)) < 1e-8
+
+
+def test_adder_exact(qf):
+    # tests/test_circuits.py:298-350: exact basis-state arithmetic
+    circ = qf.addition_circuit([0, 1, 2], [3, 4, 5], [6, 7])
+    for c0 in range(2):
+        for a0 in range(8):
+            for a1 in range(8):
+                bits = [int(b) for b in format(a0, '03b')] + \
+                       [int(b) for b in format(a1, '03b')] + [c0, 0]
+                st = np.zeros([2] * 8)
+                st[tuple(bits)] = 1
+                out = circ.run(qf.State(st), fused=False).vec.asarray().reshape(-1)
+                nz = np.nonzero(out)[0]
+                assert len(nz) == 1 and out[nz[0]] == 1.0
+                ob = [(nz[0] >> (7 - q)) & 1 for q in range(8)]
+                assert int(''.join(str(ob[q]) for q in [7, 3, 4, 5]), 2) == a0 + a1 + c0
+
+
+def test_gate_composition_on_gpu(qf):
+    # ops.py:187-198 / circuits.py:111-118: asgate == product of matrices
+    g = qf.zyz_circuit(0.1, 2.2, 0.5, 0).asgate()
+    assert qf.gates_close(g, qf.ZYZ(0.1, 2.2, 0.5))
+    c = qf.Circuit([qf.H(0), qf.CNOT(0, 1), qf.RX(0.3, 2), qf.CCNOT(2, 0, 1), qf.ISWAP(1, 2)])
+    u = c.asgate()
+    ref = O.circuit_unitary([(g_.matrix, g_.qubits) for g_ in c], (0, 1, 2))
+    assert np.max(np.abs(u.matrix - ref)) < 1e-13
+
+
+def test_layered_rotation_12q(qf):
+    from quantumflow_amd.workloads import layered_rotation_circuit
+    desc = layered_rotation_circuit(12, 20, seed=1)
+    ref = O.run_circuit(desc, qubits=tuple(range(12)))
+    for fused in (False, True):
+        ket = qf.Circuit(build(qf, desc)).run(qf.zero_state(12), fused=fused)
+        got = ket.vec.asarray()
+        assert np.max(np.abs(got - ref)) <= 1e-12, fused
+        z = qf.asarray(ket.expectation_z())
+        assert np.max(np.abs(z - O.expectation_z(ref))) <= 1e-12
+
+
+@pytest.mark.parametrize("fused", [False, True])
+def test_mixed_circuit_20q(qf, fused):
+    from quantumflow_amd.workloads import random_mixed_circuit
+    n = 20
+    desc = random_mixed_circuit(n, 12, seed=3)
+    ref = O.run_circuit(desc, qubits=tuple(range(n)))
+    ket = qf.Circuit(build(qf, desc)).run(qf.zero_state(n), fused=fused)
+    got = ket.vec.asarray()
+    assert np.max(np.abs(got - ref)) <= 1e-12
+
+
+def test_w16_kat(qf):
+    # examples/state_prep_w16.py via tests/test_examples.py:29-32
+    from tests.test_oracle import w16_gates
+    gates = []
+    for g in w16_gates():
+        if len(g) == 2:
+            gates.append(qf.Gate(g[0], g[1]))
+        else:
+            gates.append(qf.STDGATES[g[0]](*g[1], *g[2]))
+    ket = qf.Circuit(gates).run(qf.zero_state(16))
+    assert qf.states_close(ket, qf.w_state(16))

@@ -1,0 +1,175 @@
+"""GPU parity of the per-gate kernels and readouts against the oracle.
+
+Tolerances: dense and diagonal gates max|d amplitude| <= 1e-12 (complex128);
+permutation gates (X, CNOT, SWAP, CCNOT, CSWAP) bitwise equal; cut tables
+bitwise equal.
+"""
+import numpy as np
+import pytest
+
+from oracle import qf_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+@pytest.fixture(scope="module")
+def bk():
+    import quantumflow_amd.backend as bk
+    assert bk.gpu_available(), "GPU visible but libqfx.so not loaded"
+    return bk
+
+
+def rand_state(n, rng):
+    v = rng.normal(size=2 ** n) + 1j * rng.normal(size=2 ** n)
+    return (v / np.linalg.norm(v)).reshape([2] * n)
+
+
+def rand_unitary(k, rng):
+    a = rng.normal(size=(2 ** k, 2 ** k)) + 1j * rng.normal(size=(2 ** k, 2 ** k))
+    q, r = np.linalg.qr(a)
+    return q * (np.diag(r) / np.abs(np.diag(r)))
+
+
+def gpu_tensormul(bk, mat, psi, axes):
+    k = len(axes)
+    out = bk.tensormul(mat.reshape([2] * (2 * k)), bk.astensorproduct(psi), axes)
+    return bk.evaluate(out)
+
+
+@pytest.mark.parametrize("n", [5, 11, 14])
+@pytest.mark.parametrize("k", [1, 2, 3])
+def test_dense_all_positions(bk, n, k):
+    print("dense n=%d k=%d" % (n, k), flush=True)
+    rng = np.random.default_rng(100 * n + k)
+    psi = rand_state(n, rng)
+    # every axis appears as a target, low and high bits included
+    for start in range(0, n, 1 if n <= 5 else 3):
+        axes = [(start + j * (1 + start % 2)) % n for j in range(k)]
+        if len(set(axes)) < k:
+            continue
+        u = rand_unitary(k, rng)
+        ref = O.tensormul(u.reshape([2] * (2 * k)), psi, axes)
+        got = gpu_tensormul(bk, u, psi, axes)
+        assert np.max(np.abs(got - ref)) <= TOL
+
+
+@pytest.mark.parametrize("k", [4, 5])
+def test_dense_device_matrix(bk, k):
+    rng = np.random.default_rng(7 + k)
+    n = 12
+    psi = rand_state(n, rng)
+    axes = list(rng.permutation(n)[:k])
+    u = rand_unitary(k, rng)
+    ref = O.tensormul(u.reshape([2] * (2 * k)), psi, axes)
+    got = gpu_tensormul(bk, u, psi, axes)
+    assert np.max(np.abs(got - ref)) <= TOL
+
+
+@pytest.mark.parametrize("name,params,k", [
+    ('RZ', (0.731,), 1), ('PHASE', (2.1,), 1), ('CZ', (), 2), ('ZZ', (0.37,), 2),
+    ('CPHASE', (1.3,), 2), ('S', (), 1), ('T', (), 1)])
+def test_diagonal_gates(bk, name, params, k):
+    rng = np.random.default_rng(11)
+    n = 13
+    psi = rand_state(n, rng)
+    m = O.gate_matrix(name, *params)
+    for axes in ([0, 1][:k], [n - 1, 0][:k], [5, 9][:k], [n - 2, n - 1][:k]):
+        ref = O.tensormul(m.reshape([2] * (2 * k)), psi, axes)
+        got = gpu_tensormul(bk, m, psi, axes)
+        assert np.max(np.abs(got - ref)) <= TOL
+
+
+@pytest.mark.parametrize("name,k", [('X', 1), ('CNOT', 2), ('SWAP', 2), ('CCNOT', 3),
+                                    ('CSWAP', 3), ('ISWAP', 2), ('Y', 1)])
+def test_permutation_gates_exact(bk, name, k):
+    rng = np.random.default_rng(12)
+    n = 12
+    psi = rand_state(n, rng)
+    m = O.gate_matrix(name)
+    for trial in range(6):
+        axes = [int(a) for a in rng.permutation(n)[:k]]
+        ref = O.tensormul(m.reshape([2] * (2 * k)), psi, axes)
+        got = gpu_tensormul(bk, m, psi, axes)
+        assert np.array_equal(got, ref), (name, axes)
+
+
+def test_inplace_sparse_matches_out_of_place(bk):
+    import torch
+    rng = np.random.default_rng(13)
+    n = 12
+    psi = rand_state(n, rng)
+    for name, params, k in [('CNOT', (), 2), ('CZ', (), 2), ('CCNOT', (), 3), ('PHASE', (0.4,), 1),
+                            ('SWAP', (), 2), ('ISWAP', (), 2), ('RZ', (1.1,), 1)]:
+        m = O.gate_matrix(name, *params)
+        kind = bk.classify(m)
+        axes = [int(a) for a in rng.permutation(n)[:k]]
+        bits = [n - 1 - a for a in axes]
+        src = bk.as_device(bk.astensorproduct(psi))
+        out = torch.empty_like(src)
+        bk.apply_gate(out, src, n, bits, kind)
+        work = src.clone()
+        bk.apply_gate(work, work, n, bits, kind, inplace_sparse=True)
+        assert torch.equal(out, work), name
+
+
+def test_readouts(bk):
+    rng = np.random.default_rng(14)
+    for n in (1, 3, 9, 17):
+        a = rand_state(n, rng)
+        b = rand_state(n, rng)
+        ta, tb = bk.astensorproduct(a), bk.astensorproduct(b)
+        assert abs(bk.evaluate(bk.inner(ta, tb)) - np.vdot(a, b)) <= TOL
+        assert np.max(np.abs(bk.evaluate(bk.absolute(ta)) - np.absolute(a))) <= 1e-15
+        assert np.max(np.abs(bk.evaluate(bk.probabilities(ta)) - O.probabilities(a))) <= 1e-15
+        d = rng.uniform(-3, 3, size=[2] * n)
+        assert abs(bk.evaluate(bk.expectation(d, ta)) - O.expectation(a, d)) <= TOL
+        assert np.max(np.abs(bk.evaluate(bk.expectation_z(ta)) - O.expectation_z(a))) <= TOL
+        assert abs(bk.evaluate(bk.norm(2 * ta)) - O.norm(2 * a)) <= 4 * TOL
+
+
+def test_reductions_deterministic(bk):
+    rng = np.random.default_rng(15)
+    a = bk.astensorproduct(rand_state(20, rng))
+    r1 = bk.evaluate(bk.inner(a, a))
+    r2 = bk.evaluate(bk.inner(a, a))
+    assert r1 == r2
+    z1 = bk.evaluate(bk.expectation_z(a))
+    z2 = bk.evaluate(bk.expectation_z(a))
+    assert np.array_equal(z1, z2)
+
+
+def test_permute_qubits(bk):
+    from quantumflow_amd.qubits import permute_axes
+    rng = np.random.default_rng(16)
+    n = 9
+    a = rand_state(n, rng)
+    perm = [int(x) for x in rng.permutation(n)]
+    got = bk.evaluate(permute_axes(bk.astensorproduct(a), perm))
+    assert np.array_equal(got, np.transpose(a, perm))
+
+
+def test_cut_table_bitwise(bk):
+    import networkx as nx
+    from quantumflow_amd.qaoa import graph_cuts
+    rng = np.random.default_rng(17)
+    for n in (3, 8, 14):
+        g = nx.gnp_random_graph(n, 0.5, seed=int(rng.integers(1000)))
+        edges = list(g.edges())
+        assert np.array_equal(bk.evaluate(graph_cuts(g)), O.graph_cuts(n, edges))
+        for (u, v) in edges:
+            g[u][v]['weight'] = float(rng.uniform(0.1, 2))
+        w = [g[u][v]['weight'] for u, v in edges]
+        assert np.array_equal(bk.evaluate(graph_cuts(g)), O.graph_cuts(n, edges, w))
+
+
+def test_errors_map_to_reference_types(bk):
+    import quantumflow_amd as qf
+    ket = qf.zero_state(3)
+    with pytest.raises(ValueError):
+        qf.X(7).run(ket)                       # unknown qubit (ops.py:164)
+    with pytest.raises(AssertionError):
+        bk.tensormul(np.eye(4).reshape([2] * 4), ket.tensor, [0])   # K mismatch
+    with pytest.raises(ValueError):
+        bk.tensormul(np.eye(4).reshape([2] * 4), ket.tensor, [1, 1])  # repeated target
