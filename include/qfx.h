@@ -129,6 +129,37 @@ int qfx_zero_state(void* stream, void* psi, int n);
 /* psi = |+...+> = 2^(-n/2) everywhere. */
 int qfx_plus_state(void* stream, void* psi, int n);
 
+/* ---- fused circuit execution: LDS-tiled passes --------------------------
+ * Replaces the per-gate loop of Circuit.run (quantumflow/circuits.py:87-99)
+ * for runs of gates that act inside one tile bit set.  A pass loads every
+ * tile of 2^tile_bits amplitudes (index bits tile_pos[0..T-1], ascending,
+ * which should include the lowest index bits for coalescing) into LDS,
+ * applies `nops` ops to it and stores it back, in place. */
+
+typedef struct {
+  int32_t kind;       /* 0 dense (coef: 4^k row-major), 1 sparse (coef: 2^k
+                         phases; srcs: 2^k source rows) */
+  int32_t k;          /* gate width, 1..5 */
+  int32_t coef;       /* first coefficient (complex128 index) */
+  int32_t src;        /* sparse: first source-row index into srcs */
+  int32_t t[5];       /* tile-local bit of gate qubit j (MSB first) */
+  int32_t sorted[5];  /* t[0..k-1] ascending */
+  uint32_t active;    /* sparse: rows written */
+  uint32_t phase;     /* sparse: rows multiplied by their phase */
+} qfx_tile_op;
+
+/* Host-side validation of an op list against tile width T, ncoef
+ * coefficients and nsrcs source rows (host arrays).  The device entry point
+ * trusts its op list; the Python planner checks every plan with this. */
+int qfx_tile_check_ops(int T, const qfx_tile_op* ops /*host*/, int nops,
+                       int ncoef, const int* srcs /*host*/, int nsrcs);
+
+/* One pass: 6 <= tile_bits <= 13, tile_bits <= n, block in {64, 256, 512,
+ * 1024} with 2^tile_bits >= block.  ops/coef/srcs in device memory. */
+int qfx_tile_pass(void* stream, void* state, int n, int tile_bits,
+                  const int* tile_pos /*host*/, const void* ops, int nops,
+                  const void* coef, const void* srcs, int block);
+
 /* ---- QAOA MaxCut (quantumflow/qaoa.py) --------------------------------- */
 
 /* Cut table C(z) = sum_e w_e [z_u != z_v] over all 2^n basis states, edges

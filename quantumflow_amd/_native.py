@@ -26,6 +26,13 @@ class QfxC128(ctypes.Structure):
     _fields_ = [('re', ctypes.c_double), ('im', ctypes.c_double)]
 
 
+class QfxTileOp(ctypes.Structure):
+    _fields_ = [('kind', ctypes.c_int32), ('k', ctypes.c_int32),
+                ('coef', ctypes.c_int32), ('src', ctypes.c_int32),
+                ('t', ctypes.c_int32 * 5), ('sorted', ctypes.c_int32 * 5),
+                ('active', ctypes.c_uint32), ('phase', ctypes.c_uint32)]
+
+
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
 _i64 = ctypes.c_int64
@@ -56,6 +63,8 @@ SIGNATURES = {
     'qfx_zero_state': (_i, [_vp, _vp, _i]),
     'qfx_plus_state': (_i, [_vp, _vp, _i]),
     'qfx_cut_table': (_i, [_vp, _vp, _i, _vp, _vp, _i]),
+    'qfx_tile_check_ops': (_i, [_i, _vp, _i, _i, _ip, _i]),
+    'qfx_tile_pass': (_i, [_vp, _vp, _i, _i, _ip, _vp, _i, _vp, _vp, _i]),
 }
 
 _lib = None

@@ -1,0 +1,133 @@
+"""Autotuner for the tiled executor and the batched QAOA kernels.
+
+Candidates are timed on the GPU with HIP events; the best configuration per
+problem key is stored in a JSON cache (QUANTUMFLOW_AUTOTUNE, default
+~/.cache/quantumflow_amd/autotune.json) and reused by ``choose``.  Without
+a cached entry ``choose`` returns the built-in defaults below, which are the
+configurations measured best on MI355X.
+"""
+
+import itertools
+import json
+import os
+import threading
+
+from . import config
+
+_lock = threading.Lock()
+_cache = None
+
+# (tile bits, low bits, block, fuse width)
+DEFAULTS = {'tile_bits': 13, 'low_bits': 3, 'block': 256, 'fuse_width': 4}
+
+
+def _load():
+    global _cache
+    if _cache is None:
+        try:
+            with open(config.AUTOTUNE_CACHE) as f:
+                _cache = json.load(f)
+        except (OSError, ValueError):
+            _cache = {}
+    return _cache
+
+
+def _save():
+    path = config.AUTOTUNE_CACHE
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        tmp = path + '.tmp'
+        with open(tmp, 'w') as f:
+            json.dump(_cache, f, indent=1, sort_keys=True)
+        os.replace(tmp, path)
+    except OSError:
+        pass
+
+
+def block_for(tile_bits, block):
+    """Largest allowed block not exceeding the tile."""
+    b = block
+    while b > 64 and (1 << tile_bits) < b:
+        b //= 2 if b > 256 else 4
+    return b
+
+
+def default_config(n):
+    cfg = dict(DEFAULTS)
+    if config.TILE_BITS:
+        cfg['tile_bits'] = config.TILE_BITS
+    cfg['tile_bits'] = min(cfg['tile_bits'], n)
+    cfg['low_bits'] = min(cfg['low_bits'], cfg['tile_bits'])
+    cfg['block'] = block_for(cfg['tile_bits'], cfg['block'])
+    return cfg
+
+
+def key_for(kind, n, extra=''):
+    return '%s:n=%d%s' % (kind, n, extra)
+
+
+def choose(n, ngates=0):
+    """Configuration for a circuit on n qubits: cached tuning result or the
+    defaults."""
+    with _lock:
+        cached = _load().get(key_for('circuit', n))
+    if cached:
+        cfg = dict(cached['config'])
+        cfg['tile_bits'] = min(cfg['tile_bits'], n)
+        cfg['block'] = block_for(cfg['tile_bits'], cfg['block'])
+        return cfg
+    return default_config(n)
+
+
+def tune_circuit(gates, qubits, reps=2, candidates=None, store=True, verbose=False):
+    """Time candidate plans for this gate list on the GPU and cache the best
+    for its qubit count.  Returns (best config, {config tuple: ms})."""
+    import torch
+    from . import backend as bk
+    from .fusion import TilePlan
+    n = len(qubits)
+    if candidates is None:
+        tb = sorted({min(t, n) for t in (11, 12, 13)})
+        candidates = [dict(tile_bits=t, low_bits=l, block=b, fuse_width=f)
+                      for t, l, b, f in itertools.product(tb, (3, 4), (256, 512), (1, 3, 4))]
+    work = bk.zero_state_tensor(n).reshape(-1).clone()
+    results = {}
+    best = None
+    for cfg in candidates:
+        cfg = dict(cfg)
+        cfg['block'] = block_for(cfg['tile_bits'], cfg['block'])
+        plan = TilePlan(gates, qubits, **cfg)
+        plan.run_inplace(work)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            plan.run_inplace(work)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        key = tuple(sorted(cfg.items()))
+        results[key] = ms
+        if verbose:
+            print('autotune', cfg, 'passes', plan.n_passes, 'ms %.3f' % ms, flush=True)
+        if best is None or ms < best[1]:
+            best = (cfg, ms)
+    if store:
+        with _lock:
+            _load()[key_for('circuit', n)] = {'config': best[0], 'ms': best[1]}
+            _save()
+    return best[0], results
+
+
+def choose_qaoa(default):
+    """Cached QAOA-engine configuration (or ``default``)."""
+    with _lock:
+        cached = _load().get('qaoa')
+    return dict(cached['config']) if cached else dict(default)
+
+
+def store_qaoa(cfg, ms):
+    with _lock:
+        _load()['qaoa'] = {'config': cfg, 'ms': ms}
+        _save()

@@ -1,10 +1,40 @@
-"""Gate fusion and LDS-tiled execution plans, plus launch timing helpers."""
+"""Gate fusion and LDS-tiled execution plans.
+
+The plain path of Circuit.run (quantumflow/circuits.py:87-99) streams the
+whole state through HBM once per gate.  A TilePlan instead groups the gate
+list into *passes*: each pass picks a set S of T index bits (the lowest L
+bits always, for coalescing, plus the qubits of the gates it absorbs) and
+the tile kernel (csrc/tile.hip) applies all of the pass's gates to every
+2^T-amplitude tile in LDS, so a pass costs one HBM read + write of the
+state however many gates it holds.
+
+Planning (host, once per circuit):
+  1. ``plan_passes``: greedy in circuit order.  A gate joins the current
+     pass if its bits fit in S (growing S up to T bits) and it does not
+     touch a qubit of a gate already deferred to a later pass (so it
+     commutes with every gate it overtakes); otherwise it is deferred and
+     its qubits are blocked for the rest of the pass.
+  2. ``fuse_ops``: within a pass, runs of consecutive gates whose union has
+     at most ``fuse_width`` (3..5) qubits are merged into one dense block
+     when a cost model (LDS sweeps + FP64 FMA work) says it is cheaper; the
+     block's matrix is the product of the gates' matrices embedded in the
+     block (Circuit.asgate semantics, quantumflow/circuits.py:111-118).
+  3. The op tables of all passes go to the device once; running the plan is
+     one kernel launch per pass.
+"""
 
 from collections import defaultdict
 
+import numpy as np
 import torch
 
+from . import _native
 from . import backend as bk
+from .states import State
+
+OP_DENSE, OP_SPARSE = 0, 1
+MAX_TILE_BITS = 13
+MIN_TILE_BITS = 6
 
 
 class KernelTimer:
@@ -64,11 +94,263 @@ def apply_gates_inplace(gates, qubits, work, prof=None):
     return work
 
 
-def run_unfused_profiled(gates, qubits, work, prof):
-    return apply_gates_inplace(gates, qubits, work, prof)
+# ---------------------------------------------------------------------------
+# planning
+# ---------------------------------------------------------------------------
+
+def plan_passes(bits_list, n, T, L, tileable=None):
+    """Group gates (given by their index-bit tuples, in circuit order) into
+    passes.  Returns a list of (sorted tile bits, [gate indices]); a gate
+    that is not tileable forms a pass of its own with tile bits None."""
+    pending = list(range(len(bits_list)))
+    passes = []
+    low = set(range(min(L, n)))
+    while pending:
+        first = pending[0]
+        if tileable is not None and not tileable[first]:
+            passes.append((None, [first]))
+            pending = pending[1:]
+            continue
+        S = set(low)
+        blocked = set()
+        taken, rest = [], []
+        for gi in pending:
+            bs = set(bits_list[gi])
+            if (bs & blocked) or (tileable is not None and not tileable[gi]):
+                blocked |= bs
+                rest.append(gi)
+                continue
+            u = S | bs
+            if len(u) <= T:
+                S = u
+                taken.append(gi)
+            else:
+                blocked |= bs
+                rest.append(gi)
+        # fill the tile with the lowest unused bits: longer contiguous runs
+        b = 0
+        while len(S) < T:
+            if b not in S:
+                S.add(b)
+            b += 1
+        passes.append((sorted(S), taken))
+        pending = rest
+    return passes
 
 
-def plan_for(gates, qubits):
+def embed(matrix, pos, nq):
+    """2^nq x 2^nq operator of a k-qubit gate whose qubit j sits at block
+    position pos[j] (block qubit 0 = most significant)."""
+    k = len(pos)
+    g = np.asarray(matrix, dtype=np.complex128).reshape([2] * (2 * k))
+    t = np.eye(2 ** nq, dtype=np.complex128).reshape([2] * nq + [2 ** nq])
+    out = np.tensordot(g, t, axes=(list(range(k, 2 * k)), list(pos)))
+    out = np.moveaxis(out, list(range(k)), list(pos))
+    return out.reshape(2 ** nq, 2 ** nq)
+
+
+class Op:
+    """A gate inside a pass: bits in gate-qubit order and its GateKind (or
+    a dense merged block)."""
+    __slots__ = ('bits', 'kind', 'matrix', 'ngates')
+
+    def __init__(self, bits, kind=None, matrix=None, ngates=1):
+        self.bits = list(bits)
+        self.kind = kind
+        self.matrix = matrix
+        self.ngates = ngates
+
+    def dense_matrix(self):
+        if self.matrix is not None:
+            return self.matrix
+        return self.kind.matrix
+
+    def is_dense(self):
+        return self.matrix is not None or self.kind.kind == bk.GateKind.DENSE
+
+    def active_rows(self):
+        kd = self.kind
+        d = 2 ** kd.k
+        if kd.kind == bk.GateKind.DIAGONAL:
+            return int(np.sum(kd.diag != 1.0))
+        src = kd.src
+        ph = kd.phase if kd.phase is not None else np.ones(d)
+        return int(np.sum((src != np.arange(d)) | (ph != 1.0)))
+
+
+# cost of one op per tile, in units of one full LDS read+write sweep
+FLOP_WEIGHT = 0.15
+
+
+def op_cost(op):
+    k = len(op.bits)
+    if op.is_dense():
+        return 1.0 + FLOP_WEIGHT * (2 ** k)
+    return op.active_rows() / float(2 ** k)
+
+
+def fuse_ops(ops, width):
+    """Merge runs of consecutive ops into dense blocks of <= width qubits
+    when the cost model prefers the block."""
+    if width < 2:
+        return ops
+    out = []
+    cur = None
+    for op in ops:
+        if cur is None:
+            cur = op
+            continue
+        q = list(cur.bits) + [b for b in op.bits if b not in cur.bits]
+        if len(q) <= width:
+            merged_cost = 1.0 + FLOP_WEIGHT * (2 ** len(q))
+            if merged_cost <= op_cost(cur) + op_cost(op):
+                m1 = embed(cur.dense_matrix(), [q.index(b) for b in cur.bits], len(q))
+                m2 = embed(op.dense_matrix(), [q.index(b) for b in op.bits], len(q))
+                cur = Op(q, None, m2 @ m1, cur.ngates + op.ngates)
+                continue
+        out.append(cur)
+        cur = op
+    if cur is not None:
+        out.append(cur)
+    return out
+
+
+def _encode_op(op, pos_index, coefs, srcs):
+    """Append op's coefficients/sources and return its qfx_tile_op."""
+    o = _native.QfxTileOp()
+    k = len(op.bits)
+    t = [pos_index[b] for b in op.bits]
+    o.k = k
+    for j in range(5):
+        o.t[j] = t[j] if j < k else 0
+    st = sorted(t)
+    for j in range(5):
+        o.sorted[j] = st[j] if j < k else 0
+    o.coef = len(coefs)
+    if op.is_dense():
+        o.kind = OP_DENSE
+        coefs.extend(np.asarray(op.dense_matrix(), dtype=np.complex128).reshape(-1))
+        o.src = 0
+        o.active = o.phase = 0
+    else:
+        kd = op.kind
+        d = 2 ** k
+        o.kind = OP_SPARSE
+        if kd.kind == bk.GateKind.DIAGONAL:
+            src = np.arange(d)
+            ph = kd.diag
+        else:
+            src = kd.src
+            ph = kd.phase if kd.phase is not None else np.ones(d, dtype=np.complex128)
+        coefs.extend(np.asarray(ph, dtype=np.complex128))
+        o.src = len(srcs)
+        srcs.extend(int(x) for x in src)
+        active = 0
+        phase = 0
+        for r in range(d):
+            if ph[r] != 1.0:
+                phase |= 1 << r
+            if ph[r] != 1.0 or src[r] != r:
+                active |= 1 << r
+        o.active, o.phase = active, phase
+    return o
+
+
+class TilePlan:
+    """A gate list compiled into LDS-tiled passes on the device."""
+
+    def __init__(self, gates, qubits, tile_bits=MAX_TILE_BITS, low_bits=3, block=256,
+                 fuse_width=4):
+        qubits = tuple(qubits)
+        n = len(qubits)
+        self.n = n
+        self.qubits = qubits
+        self.tile_bits = T = min(tile_bits, n)
+        self.low_bits = low_bits
+        from .autotune import block_for
+        self.block = block_for(T, block)
+        self.fuse_width = fuse_width
+        self.gates = list(gates)
+        bits_list = [tuple(n - 1 - qubits.index(q) for q in g.qubits) for g in gates]
+        tileable = [len(b) <= 5 and len(b) <= T for b in bits_list]
+        self.passes = []           # (kind, payload)
+        ops_host, coefs, srcs = [], [], []
+        for pos, members in plan_passes(bits_list, n, T, low_bits, tileable):
+            if pos is None:
+                g = gates[members[0]]
+                self.passes.append(('gate', (list(bits_list[members[0]]), g.kind)))
+                continue
+            pos_index = {b: j for j, b in enumerate(pos)}
+            ops = [Op(bits_list[i], gates[i].kind) for i in members]
+            ops = fuse_ops(ops, fuse_width)
+            start = len(ops_host)
+            for op in ops:
+                ops_host.append(_encode_op(op, pos_index, coefs, srcs))
+            self.passes.append(('tile', (pos, start, len(ops))))
+        self.n_passes = len(self.passes)
+        self.n_ops = len(ops_host)
+        # validate on the host, then upload once
+        nops = len(ops_host)
+        arr = (_native.QfxTileOp * max(1, nops))(*ops_host)
+        src_arr = _native.int_array(srcs)
+        _native.check(_native.load().qfx_tile_check_ops(T, arr, nops, len(coefs), src_arr,
+                                                        len(srcs)), 'qfx_tile_check_ops')
+        raw = bytes(arr)
+        dev = bk.DEVICE
+        self._ops = torch.frombuffer(bytearray(raw), dtype=torch.uint8).to(dev)
+        self._coef = torch.from_numpy(np.asarray(coefs if coefs else [0j],
+                                                 dtype=np.complex128)).to(dev)
+        self._srcs = torch.tensor(srcs if srcs else [0], dtype=torch.int32, device=dev)
+        self._opsize = _native.ctypes.sizeof(_native.QfxTileOp)
+        self._pos_arrays = [(_native.int_array(p[1][0]), len(p[1][0])) if p[0] == 'tile'
+                            else None for p in self.passes]
+
+    def run_inplace(self, work, prof=None):
+        """Run every pass in place on the flat complex128 device state."""
+        s = bk.stream_handle()
+        n = self.n
+        lib = _native.load()
+        wp = _native.ctypes.c_void_p(work.data_ptr())
+        base_ops = self._ops.data_ptr()
+        coef = _native.ctypes.c_void_p(self._coef.data_ptr())
+        srcs = _native.ctypes.c_void_p(self._srcs.data_ptr())
+        for (kind, payload), posarr in zip(self.passes, self._pos_arrays):
+            if kind == 'gate':
+                bits, gk = payload
+                tok = prof.start(kernel_class(gk)) if prof is not None else None
+                bk.apply_gate(work, work, n, bits, gk, inplace_sparse=True)
+            else:
+                pos, start, nops = payload
+                tok = prof.start('tile_pass') if prof is not None else None
+                rc = lib.qfx_tile_pass(s, wp, n, posarr[1], posarr[0],
+                                       _native.ctypes.c_void_p(base_ops + start * self._opsize),
+                                       nops, coef, srcs, self.block)
+                _native.check(rc, 'qfx_tile_pass')
+            if tok is not None:
+                prof.stop(tok)
+        return work
+
+    def run(self, ket: State) -> State:
+        if tuple(ket.qubits) != self.qubits:
+            raise ValueError('plan was built for qubits %s' % (self.qubits,))
+        work = bk.as_device(ket.tensor).clone()
+        self.run_inplace(work)
+        return State(work.view([2] * self.n), self.qubits, ket.memory)
+
+    def stats(self):
+        ng = len(self.gates)
+        return {'gates': ng, 'passes': self.n_passes, 'ops': self.n_ops,
+                'tile_bits': self.tile_bits, 'low_bits': self.low_bits,
+                'block': self.block, 'fuse_width': self.fuse_width}
+
+
+def plan_for(gates, qubits, **kwargs):
     """Fused execution plan for ``gates`` on a state with these qubits, or
-    None when the unfused path should run."""
-    return None
+    None when the state is too small to tile (fewer than 6 qubits)."""
+    n = len(qubits)
+    if n < MIN_TILE_BITS:
+        return None
+    from . import autotune
+    cfg = autotune.choose(n, len(gates))
+    cfg.update(kwargs)
+    return TilePlan(gates, qubits, **cfg)

@@ -1,0 +1,102 @@
+"""Host-side checks of the fusion planner (no GPU): the passes it builds,
+replayed op by op with the oracle, give the circuit's state; every op fits
+its tile; merged blocks equal the product of their gates; the encoded op
+tables pass the library's own validator."""
+import numpy as np
+import pytest
+
+from oracle import qf_oracle as O
+from quantumflow_amd import fusion, _native
+from quantumflow_amd import backend as bk
+from quantumflow_amd.workloads import random_mixed_circuit, layered_rotation_circuit, build_gates
+
+
+def replay(gates, n, T, L, width):
+    """Apply the planned (and fused) ops with the oracle, in plan order."""
+    qubits = tuple(range(n))
+    bits_list = [tuple(n - 1 - q for q in g.qubits) for g in gates]
+    passes = fusion.plan_passes(bits_list, n, T, L)
+    ket = O.zero_state(n)
+    covered = []
+    for pos, members in passes:
+        assert pos is not None and len(pos) == T and pos == sorted(pos)
+        assert set(range(min(L, n))) <= set(pos)
+        ops = [fusion.Op(bits_list[i], gates[i].kind) for i in members]
+        for op in ops:
+            assert set(op.bits) <= set(pos)
+        fused = fusion.fuse_ops(ops, width)
+        assert sum(o.ngates for o in fused) == len(members)
+        for op in fused:
+            assert len(op.bits) <= max(width, 3)
+            axes = [n - 1 - b for b in op.bits]
+            m = op.dense_matrix()
+            k = len(axes)
+            ket = O.tensormul(np.asarray(m).reshape([2] * (2 * k)), ket, axes)
+        covered += members
+    assert sorted(covered) == list(range(len(gates)))
+    return ket
+
+
+@pytest.mark.parametrize("T,L,width", [(6, 3, 1), (7, 2, 3), (8, 3, 4), (9, 4, 5)])
+def test_plan_replay_matches_circuit(T, L, width):
+    n = 10
+    desc = random_mixed_circuit(n, 12, seed=T)
+    gates = build_gates(desc)
+    ref = O.run_circuit(desc, qubits=tuple(range(n)))
+    got = replay(gates, n, T, L, width)
+    assert np.max(np.abs(got - ref)) < 1e-12
+
+
+def test_plan_replay_layered():
+    n = 12
+    desc = layered_rotation_circuit(n, 6, seed=4)
+    gates = build_gates(desc)
+    ref = O.run_circuit(desc, qubits=tuple(range(n)))
+    got = replay(gates, n, 12, 3, 4)
+    assert np.max(np.abs(got - ref)) < 1e-12
+
+
+def test_pass_count_28q():
+    desc = random_mixed_circuit(28, 40, seed=2024)
+    bits = [tuple(27 - q for q in g[2]) for g in desc]
+    passes = fusion.plan_passes(bits, 28, 13, 3)
+    assert len(passes) <= 60          # vs 728 gates unfused
+
+
+def test_embed_closed_form():
+    rng = np.random.default_rng(0)
+    u = rng.normal(size=(4, 4)) + 1j * rng.normal(size=(4, 4))
+    # gate on block qubits (2, 0) of 3 == full_operator on axes [2, 0]
+    e = fusion.embed(u, [2, 0], 3)
+    assert np.allclose(e, O.full_operator(u, 3, [2, 0]), atol=1e-15)
+
+
+def test_encoded_ops_validate():
+    desc = random_mixed_circuit(10, 6, seed=1)
+    gates = build_gates(desc)
+    n, T = 10, 8
+    bits_list = [tuple(n - 1 - q for q in g.qubits) for g in gates]
+    coefs, srcs, ops = [], [], []
+    for pos, members in fusion.plan_passes(bits_list, n, T, 3):
+        pidx = {b: j for j, b in enumerate(pos)}
+        for op in fusion.fuse_ops([fusion.Op(bits_list[i], gates[i].kind) for i in members], 3):
+            ops.append(fusion._encode_op(op, pidx, coefs, srcs))
+    arr = (_native.QfxTileOp * len(ops))(*ops)
+    lib = _native.load()
+    assert lib.qfx_tile_check_ops(T, arr, len(ops), len(coefs), _native.int_array(srcs),
+                                  len(srcs)) == 0
+    # a target outside the tile is rejected
+    bad = (_native.QfxTileOp * 1)(ops[0])
+    bad[0].t[0] = T
+    bad[0].sorted[0] = T
+    assert lib.qfx_tile_check_ops(T, bad, 1, len(coefs), _native.int_array(srcs),
+                                  len(srcs)) == _native.QFX_EINVAL
+
+
+def test_gatekind_classification():
+    assert bk.classify(O.gate_matrix('CNOT')).kind == 'monomial'
+    assert bk.classify(O.gate_matrix('CNOT')).phase is None
+    assert bk.classify(O.gate_matrix('ISWAP')).kind == 'monomial'
+    assert bk.classify(O.gate_matrix('RZ', 0.3)).kind == 'diagonal'
+    assert bk.classify(O.gate_matrix('H')).kind == 'dense'
+    assert bk.classify(O.gate_matrix('CCNOT')).kind == 'monomial'
