@@ -138,27 +138,46 @@ int qfx_plus_state(void* stream, void* psi, int n);
 
 typedef struct {
   int32_t kind;       /* 0 dense (coef: 4^k row-major), 1 sparse (coef: 2^k
-                         phases; srcs: 2^k source rows) */
-  int32_t k;          /* gate width, 1..5 */
-  int32_t coef;       /* first coefficient (complex128 index) */
-  int32_t src;        /* sparse: first source-row index into srcs */
-  int32_t t[5];       /* tile-local bit of gate qubit j (MSB first) */
-  int32_t sorted[5];  /* t[0..k-1] ascending */
+                         phases; srcs: 2^k source rows), 2 register phase */
+  int32_t k;          /* dense/sparse: gate width, 1..5 */
+  int32_t coef;       /* dense/sparse: first coefficient (complex128 index);
+                         register phase: number of sub-ops */
+  int32_t src;        /* sparse: first source-row index into srcs;
+                         register phase: first sub-op (4 ints each) in srcs */
+  int32_t t[5];       /* dense/sparse: tile bit of gate qubit j (MSB first);
+                         register phase: tile bit of register bit j, j < 4 */
+  int32_t sorted[5];  /* t[0..k-1] (register phase: t[0..3]) ascending */
   uint32_t active;    /* sparse: rows written */
   uint32_t phase;     /* sparse: rows multiplied by their phase */
 } qfx_tile_op;
 
-/* Host-side validation of an op list against tile width T, ncoef
- * coefficients and nsrcs source rows (host arrays).  The device entry point
- * trusts its op list; the Python planner checks every plan with this. */
-int qfx_tile_check_ops(int T, const qfx_tile_op* ops /*host*/, int nops,
-                       int ncoef, const int* srcs /*host*/, int nsrcs);
+/* A register phase keeps, per lane, the 16 amplitudes spanned by its four
+ * register bits in registers and applies its sub-ops there.  Sub-op records
+ * are 4 ints {type, position code, coef index, mask}: type 1/2/3 = dense
+ * 1/2/3-qubit matrix on register bits given by the code (1q: bit a;
+ * 2q: (a<b) pairs in lexicographic order, gate index (bit b, bit a);
+ * 3q: (a<b<c) triples in lexicographic order, gate index (c, b, a)); type 4 =
+ * 16 diagonal phases applied where `mask` has a bit. */
+#define QFX_TILE_WIDE 1   /* flags: the pass holds 4-5 qubit dense/sparse ops */
+#define QFX_TILE_PERSISTENT 2  /* flags: persistent workgroups that prefetch
+                                  the next tile during the current one (used
+                                  when 2^tile_bits / block <= 16) */
+
+/* Host-side validation of an op list against tile width T, the block size it
+ * will run with (ops on 4-5 qubits need block <= 256), ncoef coefficients
+ * and nsrcs source rows (host arrays).  The device entry point trusts its op
+ * list; the Python planner checks every plan with this before upload. */
+int qfx_tile_check_ops(int T, int block, const qfx_tile_op* ops /*host*/,
+                       int nops, int ncoef, const int* srcs /*host*/,
+                       int nsrcs);
 
 /* One pass: 6 <= tile_bits <= 13, tile_bits <= n, block in {64, 256, 512,
- * 1024} with 2^tile_bits >= block.  ops/coef/srcs in device memory. */
+ * 1024} with 2^tile_bits >= block; flags QFX_TILE_WIDE (block <= 256) when
+ * the op list holds 4-5 qubit dense/sparse ops.  ops/coef/srcs in device
+ * memory. */
 int qfx_tile_pass(void* stream, void* state, int n, int tile_bits,
                   const int* tile_pos /*host*/, const void* ops, int nops,
-                  const void* coef, const void* srcs, int block);
+                  const void* coef, const void* srcs, int block, int flags);
 
 /* ---- QAOA MaxCut (quantumflow/qaoa.py) --------------------------------- */
 

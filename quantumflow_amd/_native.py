@@ -63,8 +63,8 @@ SIGNATURES = {
     'qfx_zero_state': (_i, [_vp, _vp, _i]),
     'qfx_plus_state': (_i, [_vp, _vp, _i]),
     'qfx_cut_table': (_i, [_vp, _vp, _i, _vp, _vp, _i]),
-    'qfx_tile_check_ops': (_i, [_i, _vp, _i, _i, _ip, _i]),
-    'qfx_tile_pass': (_i, [_vp, _vp, _i, _i, _ip, _vp, _i, _vp, _vp, _i]),
+    'qfx_tile_check_ops': (_i, [_i, _i, _vp, _i, _i, _ip, _i]),
+    'qfx_tile_pass': (_i, [_vp, _vp, _i, _i, _ip, _vp, _i, _vp, _vp, _i, _i]),
 }
 
 _lib = None

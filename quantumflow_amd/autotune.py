@@ -88,8 +88,10 @@ def tune_circuit(gates, qubits, reps=2, candidates=None, store=True, verbose=Fal
     n = len(qubits)
     if candidates is None:
         tb = sorted({min(t, n) for t in (11, 12, 13)})
-        candidates = [dict(tile_bits=t, low_bits=l, block=b, fuse_width=f)
-                      for t, l, b, f in itertools.product(tb, (3, 4), (256, 512), (1, 3, 4))]
+        candidates = [dict(tile_bits=t, low_bits=3, block=b, fuse_width=1, mode='reg')
+                      for t, b in itertools.product(tb, (256, 512))]
+        candidates += [dict(tile_bits=t, low_bits=3, block=512, fuse_width=f, mode='lds')
+                       for t, f in itertools.product(tb, (1, 3))]
     work = bk.zero_state_tensor(n).reshape(-1).clone()
     results = {}
     best = None

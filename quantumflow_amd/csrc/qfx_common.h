@@ -57,6 +57,24 @@ __device__ __forceinline__ uint64_t insert_zeros_rt(uint64_t g, const int* sorte
   return g;
 }
 
+// Native 2-double vector: local arrays of it stay in registers (arrays of
+// HIP's union-based double2 can be demoted to scratch memory).
+typedef double v2d __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ void cmac(v2d& acc, const v2d a, const v2d b) {
+  acc.x = fma(a.x, b.x, acc.x);
+  acc.x = fma(-a.y, b.y, acc.x);
+  acc.y = fma(a.x, b.y, acc.y);
+  acc.y = fma(a.y, b.x, acc.y);
+}
+
+__device__ __forceinline__ v2d cmul(const v2d a, const v2d b) {
+  v2d r;
+  r.x = a.x * b.x - a.y * b.y;
+  r.y = a.x * b.y + a.y * b.x;
+  return r;
+}
+
 // complex multiply-accumulate: acc += a * b
 __device__ __forceinline__ void cmac(double2& acc, const double2 a, const double2 b) {
   acc.x = fma(a.x, b.x, acc.x);

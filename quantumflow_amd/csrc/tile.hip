@@ -17,7 +17,12 @@
 
 namespace {
 
-enum OpKind : int { OP_DENSE = 0, OP_SPARSE = 1 };
+enum OpKind : int { OP_DENSE = 0, OP_SPARSE = 1, OP_RPHASE = 2 };
+
+constexpr int kLoadBatch = 16;  // global loads in flight per lane
+constexpr int kMaxTileBits = 13;
+constexpr int kWideOpMaxBlock = 256;  // largest block running 4/5-qubit ops
+constexpr int kRphaseMaxBlock = 512;  // largest block running register phases
 
 struct TilePos {
   int s[QFX_MAX_QUBITS];  // tile bit positions, ascending
@@ -37,125 +42,448 @@ __device__ __forceinline__ unsigned loc_off(const int* t, unsigned r) {
   return o;
 }
 
+// ---- LDS layout -----------------------------------------------------------
+// Element e of a tile lives at LDS slot sw(e): the 3-bit chunks above bit 2
+// are folded into bits 0..2, so lanes whose elements differ only in higher
+// bits (strided amplitude groups) still hit different LDS banks.
+__device__ __forceinline__ unsigned sw(unsigned e) {
+  return e ^ (((e >> 3) ^ (e >> 6) ^ (e >> 9) ^ (e >> 12)) & 7u);
+}
+
+// Offset of register index r when register bit j sits at tile bit t[j].
+template <int K>
+__device__ __forceinline__ unsigned loc_off_lsb(const int* t, unsigned r) {
+  unsigned o = 0;
+#pragma unroll
+  for (int j = 0; j < K; ++j) o |= ((r >> j) & 1u) << t[j];
+  return o;
+}
+
+// groups per lane per iteration: keep ~8 LDS reads in flight
+template <int K>
+constexpr int op_groups() { return K == 1 ? 4 : (K == 2 ? 2 : 1); }
+
 template <int K, int BS>
-__device__ void op_dense(double2* lds, int T, const qfx_tile_op& op,
-                         const double2* __restrict__ coef) {
+__device__ __forceinline__ void op_dense(v2d* lds, int T, const qfx_tile_op& op,
+                         const v2d* __restrict__ coef) {
   constexpr int D = 1 << K;
-  const double2* m = coef + op.coef;
-  const int ngroups = 1 << (T - K);
-  for (int g = threadIdx.x; g < ngroups; g += BS) {
-    unsigned lb = (unsigned)g;
+  constexpr int G = op_groups<K>();
+  const v2d* m = coef + op.coef;
+  const int ngroups = 1 << (T - K);   // a multiple of BS * G for T >= 10
+  for (int g0 = threadIdx.x; g0 < ngroups; g0 += BS * G) {
+    unsigned lb[G];
+    v2d v[G][D];
 #pragma unroll
-    for (int j = 0; j < K; ++j) lb = ins0(lb, op.sorted[j]);
-    double2 v[D];
+    for (int u = 0; u < G; ++u) {
+      unsigned l = (unsigned)(g0 + u * BS);
 #pragma unroll
-    for (int c = 0; c < D; ++c) v[c] = lds[lb + loc_off<K>(op.t, c)];
+      for (int j = 0; j < K; ++j) l = ins0(l, op.sorted[j]);
+      lb[u] = l;
+      if (g0 + u * BS < ngroups) {
 #pragma unroll
-    for (int r = 0; r < D; ++r) {
-      double2 acc = make_double2(0.0, 0.0);
+        for (int c = 0; c < D; ++c) v[u][c] = lds[sw(l + loc_off<K>(op.t, c))];
+      }
+    }
 #pragma unroll
-      for (int c = 0; c < D; ++c) cmac(acc, m[r * D + c], v[c]);
-      lds[lb + loc_off<K>(op.t, r)] = acc;
+    for (int u = 0; u < G; ++u) {
+      if (g0 + u * BS < ngroups) {
+#pragma unroll
+        for (int r = 0; r < D; ++r) {
+          v2d acc = {0.0, 0.0};
+#pragma unroll
+          for (int c = 0; c < D; ++c) cmac(acc, m[r * D + c], v[u][c]);
+          lds[sw(lb[u] + loc_off<K>(op.t, r))] = acc;
+        }
+      }
     }
   }
 }
 
 template <int K, int BS>
-__device__ void op_sparse(double2* lds, int T, const qfx_tile_op& op,
-                          const double2* __restrict__ coef, const int* __restrict__ srcs) {
+__device__ __forceinline__ void op_sparse(v2d* lds, int T, const qfx_tile_op& op,
+                          const v2d* __restrict__ coef, const int* __restrict__ srcs) {
   constexpr int D = 1 << K;
-  const double2* ph = coef + op.coef;
+  constexpr int G = op_groups<K>();
+  const v2d* ph = coef + op.coef;
   const int* src = srcs + op.src;
   const int ngroups = 1 << (T - K);
-  for (int g = threadIdx.x; g < ngroups; g += BS) {
-    unsigned lb = (unsigned)g;
+  for (int g0 = threadIdx.x; g0 < ngroups; g0 += BS * G) {
+    unsigned lb[G];
+    v2d v[G][D];
 #pragma unroll
-    for (int j = 0; j < K; ++j) lb = ins0(lb, op.sorted[j]);
-    double2 v[D];
+    for (int u = 0; u < G; ++u) {
+      unsigned l = (unsigned)(g0 + u * BS);
 #pragma unroll
-    for (int r = 0; r < D; ++r)
-      if (op.active & (1u << r)) v[r] = lds[lb + loc_off<K>(op.t, (unsigned)src[r])];
+      for (int j = 0; j < K; ++j) l = ins0(l, op.sorted[j]);
+      lb[u] = l;
+      if (g0 + u * BS < ngroups) {
 #pragma unroll
-    for (int r = 0; r < D; ++r)
-      if (op.active & (1u << r))
-        lds[lb + loc_off<K>(op.t, r)] = (op.phase & (1u << r)) ? cmul(ph[r], v[r]) : v[r];
+        for (int r = 0; r < D; ++r)
+          if (op.active & (1u << r)) v[u][r] = lds[sw(l + loc_off<K>(op.t, (unsigned)src[r]))];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+      if (g0 + u * BS < ngroups) {
+#pragma unroll
+        for (int r = 0; r < D; ++r)
+          if (op.active & (1u << r))
+            lds[sw(lb[u] + loc_off<K>(op.t, r))] =
+                (op.phase & (1u << r)) ? cmul(ph[r], v[u][r]) : v[u][r];
+      }
+    }
+  }
+}
+
+
+// ---- register phases ------------------------------------------------------
+// A register phase holds, per lane, the 16 amplitudes of one group spanned by
+// 4 tile bits (register bit j <-> tile bit op.t[j]) and applies a run of
+// 1-3 qubit gates to them in registers, so the run costs one LDS sweep.
+// Sub-ops (4 ints each in `subs`): {type, position code, coef index, mask}.
+//   SUB_D1 at register bit A: 2x2 matrix
+//   SUB_D2 at bits A<B, gate index (bit B, bit A): 4x4 matrix
+//   SUB_D3 at bits A<B<C, gate index (C, B, A): 8x8 matrix
+//   SUB_DIAG: 16 phases (one per register slot), applied where mask bit set
+// Permutation gates are SUB_D* with 0/1/phase entries: every output is one
+// product plus exact zeros, so they still move amplitudes bit for bit.
+enum SubKind : int { SUB_D1 = 1, SUB_D2 = 2, SUB_D3 = 3, SUB_DIAG = 4 };
+constexpr int kRegBits = 4;
+constexpr int kRegs = 1 << kRegBits;
+
+template <int A>
+__device__ __forceinline__ void sub_d1(v2d (&v)[kRegs], const v2d* __restrict__ m) {
+  const v2d m0 = m[0], m1 = m[1], m2 = m[2], m3 = m[3];
+#pragma unroll
+  for (int r = 0; r < kRegs; ++r) {
+    if (r & (1 << A)) continue;
+    const v2d x = v[r], y = v[r | (1 << A)];
+    v2d a = {0.0, 0.0}, b = {0.0, 0.0};
+    cmac(a, m0, x);
+    cmac(a, m1, y);
+    cmac(b, m2, x);
+    cmac(b, m3, y);
+    v[r] = a;
+    v[r | (1 << A)] = b;
+  }
+}
+
+template <int A, int B>
+__device__ __forceinline__ void sub_d2(v2d (&v)[kRegs], const v2d* __restrict__ m) {
+#pragma unroll
+  for (int r = 0; r < kRegs; ++r) {
+    if (r & ((1 << A) | (1 << B))) continue;
+    v2d x[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) x[c] = v[r | ((c >> 1) << B) | ((c & 1) << A)];
+#pragma unroll
+    for (int o = 0; o < 4; ++o) {
+      v2d acc = {0.0, 0.0};
+#pragma unroll
+      for (int c = 0; c < 4; ++c) cmac(acc, m[o * 4 + c], x[c]);
+      v[r | ((o >> 1) << B) | ((o & 1) << A)] = acc;
+    }
+  }
+}
+
+template <int A, int B, int C>
+__device__ __forceinline__ void sub_d3(v2d (&v)[kRegs], const v2d* __restrict__ m) {
+#pragma unroll
+  for (int r = 0; r < kRegs; ++r) {
+    if (r & ((1 << A) | (1 << B) | (1 << C))) continue;
+    v2d x[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+      x[c] = v[r | ((c >> 2) << C) | (((c >> 1) & 1) << B) | ((c & 1) << A)];
+#pragma unroll
+    for (int o = 0; o < 8; ++o) {
+      v2d acc = {0.0, 0.0};
+#pragma unroll
+      for (int c = 0; c < 8; ++c) cmac(acc, m[o * 8 + c], x[c]);
+      v[r | ((o >> 2) << C) | (((o >> 1) & 1) << B) | ((o & 1) << A)] = acc;
+    }
+  }
+}
+
+__device__ __forceinline__ void sub_diag(v2d (&v)[kRegs], const v2d* __restrict__ ph,
+                                         unsigned mask) {
+#pragma unroll
+  for (int r = 0; r < kRegs; ++r)
+    if (mask & (1u << r)) v[r] = cmul(ph[r], v[r]);
+}
+
+// position codes: D1 a = 0..3; D2 (a<b) in lexicographic order 0..5;
+// D3 (a<b<c) in lexicographic order 0..3
+__device__ __forceinline__ void apply_sub(v2d (&v)[kRegs], const int* __restrict__ sub,
+                                          const v2d* __restrict__ coef) {
+  const int type = sub[0], code = sub[1];
+  const v2d* m = coef + sub[2];
+  switch (type) {
+    case SUB_D1:
+      switch (code) {
+        case 0: sub_d1<0>(v, m); break;
+        case 1: sub_d1<1>(v, m); break;
+        case 2: sub_d1<2>(v, m); break;
+        case 3: sub_d1<3>(v, m); break;
+      }
+      break;
+    case SUB_D2:
+      switch (code) {
+        case 0: sub_d2<0, 1>(v, m); break;
+        case 1: sub_d2<0, 2>(v, m); break;
+        case 2: sub_d2<0, 3>(v, m); break;
+        case 3: sub_d2<1, 2>(v, m); break;
+        case 4: sub_d2<1, 3>(v, m); break;
+        case 5: sub_d2<2, 3>(v, m); break;
+      }
+      break;
+    case SUB_D3:
+      switch (code) {
+        case 0: sub_d3<0, 1, 2>(v, m); break;
+        case 1: sub_d3<0, 1, 3>(v, m); break;
+        case 2: sub_d3<0, 2, 3>(v, m); break;
+        case 3: sub_d3<1, 2, 3>(v, m); break;
+      }
+      break;
+    case SUB_DIAG: sub_diag(v, m, (unsigned)sub[3]); break;
   }
 }
 
 template <int BS>
-__device__ __forceinline__ void apply_op(double2* lds, int T, const qfx_tile_op& op,
-                                         const double2* __restrict__ coef,
+__device__ __forceinline__ void op_rphase(v2d* lds, int T, const qfx_tile_op& op,
+                                          const v2d* __restrict__ coef,
+                                          const int* __restrict__ subs) {
+  const int ngroups = 1 << (T - kRegBits);
+  const int* sub0 = subs + op.src;
+  const int nsub = op.coef;
+  for (int g = threadIdx.x; g < ngroups; g += BS) {
+    unsigned lb = (unsigned)g;
+#pragma unroll
+    for (int j = 0; j < kRegBits; ++j) lb = ins0(lb, op.sorted[j]);
+    v2d v[kRegs];
+#pragma unroll
+    for (int r = 0; r < kRegs; ++r) v[r] = lds[sw(lb + loc_off_lsb<kRegBits>(op.t, r))];
+    for (int i = 0; i < nsub; ++i) apply_sub(v, sub0 + 4 * i, coef);
+#pragma unroll
+    for (int r = 0; r < kRegs; ++r) lds[sw(lb + loc_off_lsb<kRegBits>(op.t, r))] = v[r];
+  }
+}
+
+template <int BS, bool WIDE>
+__device__ __forceinline__ void apply_op(v2d* lds, int T, const qfx_tile_op& op,
+                                         const v2d* __restrict__ coef,
                                          const int* __restrict__ srcs) {
-  if (op.kind == OP_DENSE) {
+  if (op.kind == OP_RPHASE) {
+    // 16 register amplitudes per lane exceed what 1024-lane blocks leave
+    if constexpr (BS <= kRphaseMaxBlock) op_rphase<BS>(lds, T, op, coef, srcs);
+  } else if (op.kind == OP_DENSE) {
     switch (op.k) {
       case 1: op_dense<1, BS>(lds, T, op, coef); break;
       case 2: op_dense<2, BS>(lds, T, op, coef); break;
       case 3: op_dense<3, BS>(lds, T, op, coef); break;
-      case 4: op_dense<4, BS>(lds, T, op, coef); break;
-      case 5: op_dense<5, BS>(lds, T, op, coef); break;
+      // 4- and 5-qubit ops need more registers than blocks of more than
+      // 256 lanes leave a lane; the host validator rejects them there
+      case 4:
+        if constexpr (WIDE && BS <= kWideOpMaxBlock) op_dense<4, BS>(lds, T, op, coef);
+        break;
+      case 5:
+        if constexpr (WIDE && BS <= kWideOpMaxBlock) op_dense<5, BS>(lds, T, op, coef);
+        break;
     }
   } else {
     switch (op.k) {
       case 1: op_sparse<1, BS>(lds, T, op, coef, srcs); break;
       case 2: op_sparse<2, BS>(lds, T, op, coef, srcs); break;
       case 3: op_sparse<3, BS>(lds, T, op, coef, srcs); break;
-      case 4: op_sparse<4, BS>(lds, T, op, coef, srcs); break;
-      case 5: op_sparse<5, BS>(lds, T, op, coef, srcs); break;
+      case 4:
+        if constexpr (WIDE && BS <= kWideOpMaxBlock) op_sparse<4, BS>(lds, T, op, coef, srcs);
+        break;
+      case 5:
+        if constexpr (WIDE && BS <= kWideOpMaxBlock) op_sparse<5, BS>(lds, T, op, coef, srcs);
+        break;
     }
   }
 }
 
-template <int BS, int LB>
-__global__ __launch_bounds__(BS) void tile_pass_kernel(double2* state, int T, TilePos pos,
+// Global offset of the tile bits above the lane bits: bit j of kk sits at
+// index bit pos.s[LB + j].
+template <int LB>
+__device__ __forceinline__ uint64_t high_off(int kk, int T, const TilePos& pos) {
+  uint64_t o = 0;
+#pragma unroll
+  for (int j = 0; j < kMaxTileBits - LB; ++j)
+    if (j < T - LB) o |= (uint64_t)((kk >> j) & 1) << pos.s[LB + j];
+  return o;
+}
+
+template <int BS, int LB, bool WIDE>
+__global__ __launch_bounds__(BS) void tile_pass_kernel(v2d* state, int T, TilePos pos,
                                                       const qfx_tile_op* __restrict__ ops,
-                                                      int nops, const double2* __restrict__ coef,
+                                                      int nops, const v2d* __restrict__ coef,
                                                       const int* __restrict__ srcs) {
-  extern __shared__ double2 lds[];
+  extern __shared__ v2d lds[];
   const int tid = threadIdx.x;
   // tile base: block index bits spread over the non-tile positions
+  // (loops over tile bits run to a compile-time bound so pos.s is indexed
+  // with constants and stays in kernel-argument registers)
   uint64_t base = blockIdx.x;
-  for (int j = 0; j < T; ++j) {
-    const int b = pos.s[j];
-    base = ((base >> b) << (b + 1)) | (base & (((uint64_t)1 << b) - 1));
+#pragma unroll
+  for (int j = 0; j < kMaxTileBits; ++j) {
+    if (j < T) {
+      const int b = pos.s[j];
+      base = ((base >> b) << (b + 1)) | (base & (((uint64_t)1 << b) - 1));
+    }
   }
   uint64_t toff = 0;
 #pragma unroll
   for (int j = 0; j < LB; ++j) toff |= (uint64_t)((tid >> j) & 1) << pos.s[j];
-  const int per = 1 << (T - LB);
-  const double2* src = state + base + toff;
-  for (int kk = 0; kk < per; ++kk) {
-    uint64_t koff = 0;
-    for (int j = 0; j < T - LB; ++j) koff |= (uint64_t)((kk >> j) & 1) << pos.s[LB + j];
-    lds[tid + kk * BS] = src[koff];
+  const int per = 1 << (T - LB);   // amplitudes per lane
+  const v2d* src = state + base + toff;
+  // issue kLoadBatch global loads before the first LDS write
+  for (int k0 = 0; k0 < per; k0 += kLoadBatch) {
+    v2d v[kLoadBatch];
+#pragma unroll
+    for (int u = 0; u < kLoadBatch; ++u) {
+      const int kk = k0 + u;
+      if (kk < per) {
+        v[u] = src[high_off<LB>(kk, T, pos)];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kLoadBatch; ++u)
+      if (k0 + u < per) lds[sw(tid + (k0 + u) * BS)] = v[u];
   }
   __syncthreads();
   for (int i = 0; i < nops; ++i) {
-    apply_op<BS>(lds, T, ops[i], coef, srcs);
+    apply_op<BS, WIDE>(lds, T, ops[i], coef, srcs);
     __syncthreads();
   }
-  double2* dst = state + base + toff;
-  for (int kk = 0; kk < per; ++kk) {
-    uint64_t koff = 0;
-    for (int j = 0; j < T - LB; ++j) koff |= (uint64_t)((kk >> j) & 1) << pos.s[LB + j];
-    dst[koff] = lds[tid + kk * BS];
+  v2d* dst = state + base + toff;
+  for (int k0 = 0; k0 < per; k0 += kLoadBatch) {
+    v2d v[kLoadBatch];
+#pragma unroll
+    for (int u = 0; u < kLoadBatch; ++u)
+      if (k0 + u < per) v[u] = lds[sw(tid + (k0 + u) * BS)];
+#pragma unroll
+    for (int u = 0; u < kLoadBatch; ++u) {
+      const int kk = k0 + u;
+      if (kk < per) {
+        dst[high_off<LB>(kk, T, pos)] = v[u];
+      }
+    }
   }
 }
 
-template <int BS, int LB>
-int launch_pass(hipStream_t s, double2* state, int n, int T, const TilePos& pos,
-                const qfx_tile_op* ops, int nops, const double2* coef, const int* srcs) {
-  const size_t lds = sizeof(double2) << T;
+
+// Persistent, software-pipelined variant: each workgroup walks tiles
+// blockIdx.x, blockIdx.x + gridDim.x, ...; while the current tile is
+// processed in LDS, the next tile's amplitudes are already in flight into
+// registers (pf), so HBM traffic overlaps the LDS work.  Needs
+// 2^T / BS <= kPrefetch amplitudes per lane.
+constexpr int kPrefetch = 16;
+
+template <int BS, int LB, bool WIDE>
+__global__ __launch_bounds__(BS) void tile_pipe_kernel(v2d* state, int T, TilePos pos,
+                                                      const qfx_tile_op* __restrict__ ops,
+                                                      int nops, const v2d* __restrict__ coef,
+                                                      const int* __restrict__ srcs,
+                                                      uint32_t ntiles) {
+  extern __shared__ v2d lds[];
+  const int tid = threadIdx.x;
+  uint64_t toff = 0;
+#pragma unroll
+  for (int j = 0; j < LB; ++j) toff |= (uint64_t)((tid >> j) & 1) << pos.s[j];
+  const int per = 1 << (T - LB);
+  uint64_t hoff[kPrefetch];
+#pragma unroll
+  for (int u = 0; u < kPrefetch; ++u) hoff[u] = high_off<LB>(u, T, pos);
+
+  auto tile_base = [&](uint32_t tile) {
+    uint64_t base = tile;
+#pragma unroll
+    for (int j = 0; j < kMaxTileBits; ++j) {
+      if (j < T) {
+        const int b = pos.s[j];
+        base = ((base >> b) << (b + 1)) | (base & (((uint64_t)1 << b) - 1));
+      }
+    }
+    return base + toff;
+  };
+
+  v2d pf[kPrefetch];
+  uint32_t tile = blockIdx.x;
+  if (tile < ntiles) {
+    const v2d* src = state + tile_base(tile);
+#pragma unroll
+    for (int u = 0; u < kPrefetch; ++u)
+      if (u < per) pf[u] = src[hoff[u]];
+  }
+  for (; tile < ntiles; tile += gridDim.x) {
+#pragma unroll
+    for (int u = 0; u < kPrefetch; ++u)
+      if (u < per) lds[sw(tid + u * BS)] = pf[u];
+    const uint32_t next = tile + gridDim.x;
+    if (next < ntiles) {
+      const v2d* src = state + tile_base(next);
+#pragma unroll
+      for (int u = 0; u < kPrefetch; ++u)
+        if (u < per) pf[u] = src[hoff[u]];
+    }
+    __syncthreads();
+    for (int i = 0; i < nops; ++i) {
+      apply_op<BS, WIDE>(lds, T, ops[i], coef, srcs);
+      __syncthreads();
+    }
+    v2d* dst = state + tile_base(tile);
+#pragma unroll
+    for (int u = 0; u < kPrefetch; ++u)
+      if (u < per) dst[hoff[u]] = lds[sw(tid + u * BS)];
+    __syncthreads();
+  }
+}
+
+template <int BS, int LB, bool WIDE>
+int launch_pass(hipStream_t s, v2d* state, int n, int T, const TilePos& pos,
+                const qfx_tile_op* ops, int nops, const v2d* coef, const int* srcs,
+                bool persistent) {
+  const size_t lds = sizeof(v2d) << T;
   static bool attr_set = false;
+  static int occ[kMaxTileBits + 1];
   if (!attr_set) {
-    int rc = qfx::check_hip(hipFuncSetAttribute((const void*)tile_pass_kernel<BS, LB>,
+    int rc = qfx::check_hip(hipFuncSetAttribute((const void*)tile_pass_kernel<BS, LB, WIDE>,
                                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                                 160 * 1024),
                             "hipFuncSetAttribute");
     if (rc) return rc;
+    rc = qfx::check_hip(hipFuncSetAttribute((const void*)tile_pipe_kernel<BS, LB, WIDE>,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                                            160 * 1024),
+                        "hipFuncSetAttribute");
+    if (rc) return rc;
+    for (int t = 0; t <= kMaxTileBits; ++t) occ[t] = 0;
     attr_set = true;
   }
   const uint64_t tiles = (uint64_t)1 << (n - T);
-  hipLaunchKernelGGL((tile_pass_kernel<BS, LB>), dim3((unsigned)tiles), dim3(BS), lds, s, state, T,
+  if (persistent && BS <= 512 && (T - LB) <= 4) {
+    if (occ[T] == 0) {
+      int per_cu = 0, dev = 0, cus = 0;
+      int rc = qfx::check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                                  &per_cu, (const void*)tile_pipe_kernel<BS, LB, WIDE>, BS, lds),
+                              "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+      if (rc) return rc;
+      rc = qfx::check_hip(hipGetDevice(&dev), "hipGetDevice");
+      if (rc) return rc;
+      rc = qfx::check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev),
+                          "hipDeviceGetAttribute");
+      if (rc) return rc;
+      occ[T] = (per_cu > 0 ? per_cu : 1) * cus;
+    }
+    const uint64_t grid = tiles < (uint64_t)occ[T] ? tiles : (uint64_t)occ[T];
+    hipLaunchKernelGGL((tile_pipe_kernel<BS, LB, WIDE>), dim3((unsigned)grid), dim3(BS), lds, s,
+                       state, T, pos, ops, nops, coef, srcs, (uint32_t)tiles);
+    return qfx::check_launch("tile_pipe_kernel");
+  }
+  hipLaunchKernelGGL((tile_pass_kernel<BS, LB, WIDE>), dim3((unsigned)tiles), dim3(BS), lds, s, state, T,
                      pos, ops, nops, coef, srcs);
   return qfx::check_launch("tile_pass_kernel");
 }
@@ -182,12 +510,39 @@ int check_pass(int n, int T, const int* tile_pos, int nops, int block, const cha
 
 extern "C" {
 
-int qfx_tile_check_ops(int T, const qfx_tile_op* ops, int nops, int ncoef, const int* srcs,
-                       int nsrcs) {
+int qfx_tile_check_ops(int T, int block, const qfx_tile_op* ops, int nops, int ncoef,
+                       const int* srcs, int nsrcs) {
+  const int kmax = block <= kWideOpMaxBlock ? 5 : 3;
   for (int i = 0; i < nops; ++i) {
     const qfx_tile_op& o = ops[i];
-    if ((o.kind != OP_DENSE && o.kind != OP_SPARSE) || o.k < 1 || o.k > 5 || o.k > T)
-      return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: bad op kind or width");
+    if (o.kind == OP_RPHASE) {
+      if (T < kRegBits) return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: tile too small");
+      if (block > kRphaseMaxBlock)
+        return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: register phases need block <= 512");
+      for (int j = 0; j < kRegBits; ++j) {
+        if (o.t[j] < 0 || o.t[j] >= T)
+          return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: register bit outside the tile");
+        if (j > 0 && o.sorted[j] <= o.sorted[j - 1])
+          return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: register bits not distinct");
+        bool found = false;
+        for (int i2 = 0; i2 < kRegBits; ++i2) found |= o.sorted[i2] == o.t[j];
+        if (!found) return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: sorted != register bits");
+      }
+      if (o.coef < 0 || o.src < 0 || o.src + 4 * o.coef > nsrcs)
+        return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: sub-ops out of range");
+      for (int q = 0; q < o.coef; ++q) {
+        const int* sub = srcs + o.src + 4 * q;
+        static const int ncode[5] = {0, 4, 6, 4, 1};
+        static const int need[5] = {0, 4, 16, 64, kRegs};
+        if (sub[0] < SUB_D1 || sub[0] > SUB_DIAG || sub[1] < 0 || sub[1] >= ncode[sub[0]])
+          return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: bad sub-op");
+        if (sub[2] < 0 || sub[2] + need[sub[0]] > ncoef)
+          return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: sub-op coefficients out of range");
+      }
+      continue;
+    }
+    if ((o.kind != OP_DENSE && o.kind != OP_SPARSE) || o.k < 1 || o.k > kmax || o.k > T)
+      return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: bad op kind or width for block");
     const int D = 1 << o.k;
     const int need = o.kind == OP_DENSE ? D * D : D;
     if (o.coef < 0 || o.coef + need > ncoef)
@@ -217,24 +572,33 @@ int qfx_tile_check_ops(int T, const qfx_tile_op* ops, int nops, int ncoef, const
 }
 
 int qfx_tile_pass(void* stream, void* state, int n, int tile_bits, const int* tile_pos,
-                  const void* ops, int nops, const void* coef, const void* srcs, int block) {
+                  const void* ops, int nops, const void* coef, const void* srcs, int block,
+                  int flags) {
   int rc = check_pass(n, tile_bits, tile_pos, nops, block, "qfx_tile_pass");
   if (rc) return rc;
+  const bool wide = (flags & QFX_TILE_WIDE) != 0;
+  const bool pers = (flags & QFX_TILE_PERSISTENT) != 0;
+  if (wide && block > kWideOpMaxBlock)
+    return qfx::set_error(QFX_EINVAL, "qfx_tile_pass: 4-5 qubit ops need block <= 256");
   if (!state || (nops > 0 && (!ops || !coef)))
     return qfx::set_error(QFX_EINVAL, "qfx_tile_pass: null pointer");
   TilePos pos;
   std::memset(&pos, 0, sizeof(pos));
   for (int j = 0; j < tile_bits; ++j) pos.s[j] = tile_pos[j];
   hipStream_t s = qfx::as_stream(stream);
-  auto* st = static_cast<double2*>(state);
+  auto* st = static_cast<v2d*>(state);
   auto* o = static_cast<const qfx_tile_op*>(ops);
-  auto* c = static_cast<const double2*>(coef);
+  auto* c = static_cast<const v2d*>(coef);
   auto* sr = static_cast<const int*>(srcs);
   switch (block) {
-    case 64: return launch_pass<64, 6>(s, st, n, tile_bits, pos, o, nops, c, sr);
-    case 256: return launch_pass<256, 8>(s, st, n, tile_bits, pos, o, nops, c, sr);
-    case 512: return launch_pass<512, 9>(s, st, n, tile_bits, pos, o, nops, c, sr);
-    default: return launch_pass<1024, 10>(s, st, n, tile_bits, pos, o, nops, c, sr);
+    case 64:
+      return wide ? launch_pass<64, 6, true>(s, st, n, tile_bits, pos, o, nops, c, sr, pers)
+                  : launch_pass<64, 6, false>(s, st, n, tile_bits, pos, o, nops, c, sr, pers);
+    case 256:
+      return wide ? launch_pass<256, 8, true>(s, st, n, tile_bits, pos, o, nops, c, sr, pers)
+                  : launch_pass<256, 8, false>(s, st, n, tile_bits, pos, o, nops, c, sr, pers);
+    case 512: return launch_pass<512, 9, false>(s, st, n, tile_bits, pos, o, nops, c, sr, pers);
+    default: return launch_pass<1024, 10, false>(s, st, n, tile_bits, pos, o, nops, c, sr, pers);
   }
 }
 

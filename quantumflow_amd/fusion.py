@@ -32,9 +32,16 @@ from . import _native
 from . import backend as bk
 from .states import State
 
-OP_DENSE, OP_SPARSE = 0, 1
+OP_DENSE, OP_SPARSE, OP_RPHASE = 0, 1, 2
+SUB_D1, SUB_D2, SUB_D3, SUB_DIAG = 1, 2, 3, 4
+REG_BITS = 4
 MAX_TILE_BITS = 13
 MIN_TILE_BITS = 6
+TILE_WIDE = 1
+TILE_PERSISTENT = 2
+# position codes of the register-phase kernels (lexicographic)
+_D2_CODES = {p: i for i, p in enumerate([(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)])}
+_D3_CODES = {p: i for i, p in enumerate([(0, 1, 2), (0, 1, 3), (0, 2, 3), (1, 2, 3)])}
 
 
 class KernelTimer:
@@ -256,11 +263,118 @@ def _encode_op(op, pos_index, coefs, srcs):
     return o
 
 
+
+def plan_phases(ops, width=REG_BITS):
+    """Group a pass's ops (in order) into register phases of <= width bits.
+    Same rule as plan_passes: an op joins the phase if its bits fit and it
+    does not touch a bit of an op already deferred, so every op it overtakes
+    acts on other qubits and commutes with it."""
+    pending = list(range(len(ops)))
+    phases = []
+    while pending:
+        R = set(ops[pending[0]].bits)
+        blocked = set()
+        taken, rest = [], []
+        for i in pending:
+            b = set(ops[i].bits)
+            if b & blocked:
+                blocked |= b
+                rest.append(i)
+                continue
+            u = R | b
+            if len(u) <= width:
+                R = u
+                taken.append(i)
+            else:
+                blocked |= b
+                rest.append(i)
+        phases.append((R, [ops[i] for i in taken]))
+        pending = rest
+    return phases
+
+
+def _reorder_qubits(matrix, perm):
+    """Matrix of the same gate with its qubits listed in order perm (new
+    qubit j = old qubit perm[j])."""
+    k = len(perm)
+    t = np.asarray(matrix, dtype=np.complex128).reshape([2] * (2 * k))
+    axes = list(perm) + [k + p for p in perm]
+    return np.transpose(t, axes).reshape(2 ** k, 2 ** k)
+
+
+def _encode_phase(R, members, pos_index, T, coefs, subs):
+    """qfx_tile_op of a register phase over tile bits R plus its sub-ops."""
+    tl = sorted(pos_index[b] for b in R)
+    for j in range(T):                      # pad to 4 register bits
+        if len(tl) >= REG_BITS:
+            break
+        if j not in tl:
+            tl.append(j)
+    regs = tl[:REG_BITS]                    # register bit j <-> tile bit regs[j]
+    reg_of = {t: j for j, t in enumerate(regs)}
+    o = _native.QfxTileOp()
+    o.kind = OP_RPHASE
+    o.k = REG_BITS
+    for j in range(5):
+        o.t[j] = regs[j] if j < REG_BITS else 0
+    st = sorted(regs)
+    for j in range(5):
+        o.sorted[j] = st[j] if j < REG_BITS else 0
+    o.src = len(subs)
+    nsub = 0
+    pending_diag = None
+
+    def flush_diag():
+        nonlocal pending_diag, nsub
+        if pending_diag is None:
+            return
+        mask = 0
+        for r in range(2 ** REG_BITS):
+            if pending_diag[r] != 1.0:
+                mask |= 1 << r
+        if mask:
+            subs.extend([SUB_DIAG, 0, len(coefs), mask])
+            coefs.extend(pending_diag)
+            nsub += 1
+        pending_diag = None
+
+    for op in members:
+        p = [reg_of[pos_index[b]] for b in op.bits]        # register bit per gate qubit
+        k = len(p)
+        kd = op.kind
+        if op.matrix is None and kd.kind == bk.GateKind.DIAGONAL:
+            ph = np.empty(2 ** REG_BITS, dtype=np.complex128)
+            for r in range(2 ** REG_BITS):
+                idx = 0
+                for j in range(k):
+                    idx |= ((r >> p[j]) & 1) << (k - 1 - j)
+                ph[r] = kd.diag[idx]
+            pending_diag = ph if pending_diag is None else pending_diag * ph
+            continue
+        flush_diag()
+        m = op.dense_matrix()
+        order = sorted(range(k), key=lambda j: -p[j])       # gate MSB <-> highest bit
+        m = _reorder_qubits(m, order)
+        ps = tuple(sorted(p))
+        if k == 1:
+            typ, code = SUB_D1, ps[0]
+        elif k == 2:
+            typ, code = SUB_D2, _D2_CODES[ps]
+        else:
+            typ, code = SUB_D3, _D3_CODES[ps]
+        subs.extend([typ, code, len(coefs), 0])
+        coefs.extend(np.asarray(m, dtype=np.complex128).reshape(-1))
+        nsub += 1
+    flush_diag()
+    o.coef = nsub
+    o.active = o.phase = 0
+    return o
+
 class TilePlan:
     """A gate list compiled into LDS-tiled passes on the device."""
 
     def __init__(self, gates, qubits, tile_bits=MAX_TILE_BITS, low_bits=3, block=256,
-                 fuse_width=4):
+                 fuse_width=4, mode='reg', persistent=True):
         qubits = tuple(qubits)
         n = len(qubits)
         self.n = n
@@ -269,12 +383,19 @@ class TilePlan:
         self.low_bits = low_bits
         from .autotune import block_for
         self.block = block_for(T, block)
+        self.mode = mode if T >= REG_BITS + 2 else 'lds'
+        # register phases run in blocks of <= 512 lanes; 4- and 5-qubit ops
+        # only in blocks of <= 256
+        if self.mode == 'reg' and self.block > 512:
+            self.block = 512
         self.fuse_width = fuse_width
+        self.persistent = persistent
         self.gates = list(gates)
         bits_list = [tuple(n - 1 - qubits.index(q) for q in g.qubits) for g in gates]
         tileable = [len(b) <= 5 and len(b) <= T for b in bits_list]
         self.passes = []           # (kind, payload)
         ops_host, coefs, srcs = [], [], []
+        self.n_phases = 0
         for pos, members in plan_passes(bits_list, n, T, low_bits, tileable):
             if pos is None:
                 g = gates[members[0]]
@@ -282,19 +403,38 @@ class TilePlan:
                 continue
             pos_index = {b: j for j, b in enumerate(pos)}
             ops = [Op(bits_list[i], gates[i].kind) for i in members]
-            ops = fuse_ops(ops, fuse_width)
+            wide = any(len(op.bits) > 3 for op in ops)
             start = len(ops_host)
-            for op in ops:
-                ops_host.append(_encode_op(op, pos_index, coefs, srcs))
-            self.passes.append(('tile', (pos, start, len(ops))))
+            if self.mode == 'reg' and not wide:
+                for R, mem in plan_phases(ops):
+                    ops_host.append(_encode_phase(R, mem, pos_index, T, coefs, srcs))
+                block, flags = self.block, 0
+                self.n_phases += len(ops_host) - start
+            else:
+                block = min(self.block, 256) if wide else self.block
+                width = self.fuse_width if block <= 256 else min(self.fuse_width, 3)
+                for op in fuse_ops(ops, width):
+                    ops_host.append(_encode_op(op, pos_index, coefs, srcs))
+                wide = wide or any(o.k > 3 for o in ops_host[start:] if o.kind != OP_RPHASE)
+                flags = TILE_WIDE if wide else 0
+                if wide:
+                    block = min(block, 256)
+            if persistent:
+                flags |= TILE_PERSISTENT
+            self.passes.append(('tile', (pos, start, len(ops_host) - start, block, flags)))
         self.n_passes = len(self.passes)
         self.n_ops = len(ops_host)
-        # validate on the host, then upload once
-        nops = len(ops_host)
-        arr = (_native.QfxTileOp * max(1, nops))(*ops_host)
+        # validate on the host (each pass with its own block), then upload once
+        lib = _native.load()
         src_arr = _native.int_array(srcs)
-        _native.check(_native.load().qfx_tile_check_ops(T, arr, nops, len(coefs), src_arr,
-                                                        len(srcs)), 'qfx_tile_check_ops')
+        for kind, payload in self.passes:
+            if kind != 'tile':
+                continue
+            _, start, nops, block, _ = payload
+            arr = (_native.QfxTileOp * max(1, nops))(*ops_host[start:start + nops])
+            _native.check(lib.qfx_tile_check_ops(T, block, arr, nops, len(coefs), src_arr,
+                                                 len(srcs)), 'qfx_tile_check_ops')
+        arr = (_native.QfxTileOp * max(1, self.n_ops))(*ops_host)
         raw = bytes(arr)
         dev = bk.DEVICE
         self._ops = torch.frombuffer(bytearray(raw), dtype=torch.uint8).to(dev)
@@ -320,11 +460,11 @@ class TilePlan:
                 tok = prof.start(kernel_class(gk)) if prof is not None else None
                 bk.apply_gate(work, work, n, bits, gk, inplace_sparse=True)
             else:
-                pos, start, nops = payload
+                pos, start, nops, block, flags = payload
                 tok = prof.start('tile_pass') if prof is not None else None
                 rc = lib.qfx_tile_pass(s, wp, n, posarr[1], posarr[0],
                                        _native.ctypes.c_void_p(base_ops + start * self._opsize),
-                                       nops, coef, srcs, self.block)
+                                       nops, coef, srcs, block, flags)
                 _native.check(rc, 'qfx_tile_pass')
             if tok is not None:
                 prof.stop(tok)
@@ -340,6 +480,7 @@ class TilePlan:
     def stats(self):
         ng = len(self.gates)
         return {'gates': ng, 'passes': self.n_passes, 'ops': self.n_ops,
+                'phases': self.n_phases, 'mode': self.mode, 'persistent': self.persistent,
                 'tile_bits': self.tile_bits, 'low_bits': self.low_bits,
                 'block': self.block, 'fuse_width': self.fuse_width}
 

@@ -83,13 +83,13 @@ def test_encoded_ops_validate():
             ops.append(fusion._encode_op(op, pidx, coefs, srcs))
     arr = (_native.QfxTileOp * len(ops))(*ops)
     lib = _native.load()
-    assert lib.qfx_tile_check_ops(T, arr, len(ops), len(coefs), _native.int_array(srcs),
+    assert lib.qfx_tile_check_ops(T, 256, arr, len(ops), len(coefs), _native.int_array(srcs),
                                   len(srcs)) == 0
     # a target outside the tile is rejected
     bad = (_native.QfxTileOp * 1)(ops[0])
     bad[0].t[0] = T
     bad[0].sorted[0] = T
-    assert lib.qfx_tile_check_ops(T, bad, 1, len(coefs), _native.int_array(srcs),
+    assert lib.qfx_tile_check_ops(T, 256, bad, 1, len(coefs), _native.int_array(srcs),
                                   len(srcs)) == _native.QFX_EINVAL
 
 
@@ -100,3 +100,68 @@ def test_gatekind_classification():
     assert bk.classify(O.gate_matrix('RZ', 0.3)).kind == 'diagonal'
     assert bk.classify(O.gate_matrix('H')).kind == 'dense'
     assert bk.classify(O.gate_matrix('CCNOT')).kind == 'monomial'
+
+
+def _emulate_ops(ket, n, pos, ops, coefs, srcs):
+    """Apply encoded qfx_tile_op records to a numpy state, as the tile kernel
+    defines them (register phases: register bit j <-> tile bit t[j])."""
+    coefs = np.asarray(coefs)
+    for o in ops:
+        if o.kind == fusion.OP_RPHASE:
+            gb = [pos[o.t[j]] for j in range(fusion.REG_BITS)]
+            u = np.eye(16, dtype=complex)
+            d2 = [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]
+            d3 = [(0, 1, 2), (0, 1, 3), (0, 2, 3), (1, 2, 3)]
+            for q in range(o.coef):
+                typ, code, c0, mask = srcs[o.src + 4 * q: o.src + 4 * q + 4]
+                if typ == fusion.SUB_DIAG:
+                    ph = np.where([(mask >> r) & 1 for r in range(16)], coefs[c0:c0 + 16], 1.0)
+                    u = np.diag(ph) @ u
+                    continue
+                if typ == fusion.SUB_D1:
+                    bits = [code]
+                elif typ == fusion.SUB_D2:
+                    bits = list(d2[code])
+                else:
+                    bits = list(d3[code])
+                k = len(bits)
+                m = coefs[c0:c0 + 4 ** k].reshape(2 ** k, 2 ** k)
+                axes = [3 - b for b in reversed(bits)]      # gate MSB <-> highest bit
+                u = O.full_operator(m, 4, axes) @ u
+            axes = [n - 1 - gb[j] for j in (3, 2, 1, 0)]
+            ket = O.tensormul(u.reshape([2] * 8), ket, axes)
+        else:
+            k = o.k
+            axes = [n - 1 - pos[o.t[j]] for j in range(k)]
+            if o.kind == fusion.OP_DENSE:
+                m = coefs[o.coef:o.coef + 4 ** k].reshape(2 ** k, 2 ** k)
+            else:
+                m = np.zeros((2 ** k, 2 ** k), dtype=complex)
+                for r in range(2 ** k):
+                    m[r, srcs[o.src + r]] = coefs[o.coef + r]
+            ket = O.tensormul(m.reshape([2] * (2 * k)), ket, axes)
+    return ket
+
+
+@pytest.mark.parametrize("T,seed", [(7, 0), (8, 1), (10, 2)])
+def test_register_phase_encoding_replays(T, seed):
+    n = 11
+    desc = random_mixed_circuit(n, 10, seed=seed)
+    gates = build_gates(desc)
+    ref = O.run_circuit(desc, qubits=tuple(range(n)))
+    bits_list = [tuple(n - 1 - q for q in g.qubits) for g in gates]
+    ket = O.zero_state(n)
+    nphase = 0
+    for pos, members in fusion.plan_passes(bits_list, n, T, 3):
+        pidx = {b: j for j, b in enumerate(pos)}
+        coefs, srcs, ops = [], [], []
+        for R, mem in fusion.plan_phases([fusion.Op(bits_list[i], gates[i].kind) for i in members]):
+            assert len(R) <= fusion.REG_BITS
+            ops.append(fusion._encode_phase(R, mem, pidx, T, coefs, srcs))
+        nphase += len(ops)
+        arr = (_native.QfxTileOp * len(ops))(*ops)
+        assert _native.load().qfx_tile_check_ops(T, 256, arr, len(ops), len(coefs),
+                                                 _native.int_array(srcs), len(srcs)) == 0
+        ket = _emulate_ops(ket, n, pos, ops, coefs, srcs)
+    assert nphase < len(gates)
+    assert np.max(np.abs(ket - ref)) < 1e-12

@@ -175,20 +175,25 @@ def test_errors_map_to_reference_types(bk):
         bk.tensormul(np.eye(4).reshape([2] * 4), ket.tensor, [1, 1])  # repeated target
 
 
-@pytest.mark.parametrize("T,L,block,width", [
-    (6, 3, 64, 1), (8, 3, 256, 3), (10, 4, 256, 4), (11, 3, 512, 4), (12, 5, 1024, 3),
-    (13, 3, 256, 5), (13, 4, 512, 4)])
-def test_tile_plan_configs(bk, T, L, block, width):
+@pytest.mark.parametrize("persistent", [False, True])
+@pytest.mark.parametrize("T,L,block,width,mode", [
+    (6, 3, 64, 1, 'lds'), (8, 3, 256, 3, 'lds'), (10, 4, 256, 4, 'lds'),
+    (11, 3, 512, 4, 'lds'), (12, 5, 1024, 3, 'lds'), (13, 3, 256, 5, 'lds'),
+    (13, 4, 512, 4, 'lds'), (6, 3, 64, 1, 'reg'), (8, 2, 256, 1, 'reg'),
+    (11, 3, 256, 1, 'reg'), (12, 3, 256, 1, 'reg'), (12, 4, 512, 1, 'reg'),
+    (13, 3, 512, 1, 'reg')])
+def test_tile_plan_configs(bk, T, L, block, width, mode, persistent):
     """Every tiled-executor configuration reproduces the oracle's state."""
     import quantumflow_amd as qf
     from quantumflow_amd.fusion import TilePlan
     from quantumflow_amd.workloads import random_mixed_circuit, build_gates
     n = 15
-    print("tile T=%d L=%d block=%d width=%d" % (T, L, block, width), flush=True)
+    print("tile T=%d L=%d block=%d width=%d mode=%s persistent=%s"
+          % (T, L, block, width, mode, persistent), flush=True)
     desc = random_mixed_circuit(n, 10, seed=T * 7 + L)
     ref = O.run_circuit(desc, qubits=tuple(range(n)))
     plan = TilePlan(build_gates(desc), tuple(range(n)), tile_bits=T, low_bits=L,
-                    block=block, fuse_width=width)
+                    block=block, fuse_width=width, mode=mode, persistent=persistent)
     got = plan.run(qf.zero_state(n)).vec.asarray()
     assert np.max(np.abs(got - ref)) <= TOL
 
@@ -207,6 +212,8 @@ def test_tile_plan_permutations_exact(bk):
     psi = rand_state(n, rng)
     ref = O.run_circuit(desc, qubits=tuple(range(n)), ket=psi)
     from quantumflow_amd.workloads import build_gates
-    plan = TilePlan(build_gates(desc), tuple(range(n)), tile_bits=10, low_bits=3, fuse_width=1)
-    got = plan.run(qf.State(psi)).vec.asarray()
-    assert np.array_equal(got, ref)
+    for mode in ('lds', 'reg'):
+        plan = TilePlan(build_gates(desc), tuple(range(n)), tile_bits=10, low_bits=3,
+                        fuse_width=1, mode=mode)
+        got = plan.run(qf.State(psi)).vec.asarray()
+        assert np.array_equal(got, ref), mode
