@@ -139,15 +139,19 @@ __device__ __forceinline__ void op_sparse(v2d* lds, int T, const qfx_tile_op& op
 // ---- register phases ------------------------------------------------------
 // A register phase holds, per lane, the 16 amplitudes of one group spanned by
 // 4 tile bits (register bit j <-> tile bit op.t[j]) and applies a run of
-// 1-3 qubit gates to them in registers, so the run costs one LDS sweep.
-// Sub-ops (4 ints each in `subs`): {type, position code, coef index, mask}.
-//   SUB_D1 at register bit A: 2x2 matrix
-//   SUB_D2 at bits A<B, gate index (bit B, bit A): 4x4 matrix
-//   SUB_D3 at bits A<B<C, gate index (C, B, A): 8x8 matrix
-//   SUB_DIAG: 16 phases (one per register slot), applied where mask bit set
-// Permutation gates are SUB_D* with 0/1/phase entries: every output is one
-// product plus exact zeros, so they still move amplitudes bit for bit.
-enum SubKind : int { SUB_D1 = 1, SUB_D2 = 2, SUB_D3 = 3, SUB_DIAG = 4 };
+// gates to them in registers, so the run costs one LDS sweep.
+// Sub-ops (4 ints each in `subs`): {type, code, coef index, mask}.
+//   SUB_D1  2x2 matrix on register bit `code`
+//   SUB_D2  4x4 matrix on register bits (a<b) = pair `code` (lexicographic),
+//           gate index (bit b, bit a)
+//   SUB_DIAG  16 phases (one per slot), applied where `mask` has a bit
+//   SUB_CX  X on register bit t = code & 3 where every bit of the control
+//           mask code >> 2 is set: a pure register exchange (X, CNOT, CCNOT)
+//   SUB_SWAP  exchange of register bits (a<b) = pair `code & 7` where every
+//           bit of the control mask code >> 3 is set (SWAP, CSWAP)
+// Exchanges move amplitudes bit for bit; phases of monomial gates (ISWAP,
+// Y, ...) follow as a SUB_DIAG.
+enum SubKind : int { SUB_D1 = 1, SUB_D2 = 2, SUB_DIAG = 4, SUB_CX = 5, SUB_SWAP = 6 };
 constexpr int kRegBits = 4;
 constexpr int kRegs = 1 << kRegBits;
 
@@ -186,25 +190,6 @@ __device__ __forceinline__ void sub_d2(v2d (&v)[kRegs], const v2d* __restrict__ 
   }
 }
 
-template <int A, int B, int C>
-__device__ __forceinline__ void sub_d3(v2d (&v)[kRegs], const v2d* __restrict__ m) {
-#pragma unroll
-  for (int r = 0; r < kRegs; ++r) {
-    if (r & ((1 << A) | (1 << B) | (1 << C))) continue;
-    v2d x[8];
-#pragma unroll
-    for (int c = 0; c < 8; ++c)
-      x[c] = v[r | ((c >> 2) << C) | (((c >> 1) & 1) << B) | ((c & 1) << A)];
-#pragma unroll
-    for (int o = 0; o < 8; ++o) {
-      v2d acc = {0.0, 0.0};
-#pragma unroll
-      for (int c = 0; c < 8; ++c) cmac(acc, m[o * 8 + c], x[c]);
-      v[r | ((o >> 2) << C) | (((o >> 1) & 1) << B) | ((o & 1) << A)] = acc;
-    }
-  }
-}
-
 __device__ __forceinline__ void sub_diag(v2d (&v)[kRegs], const v2d* __restrict__ ph,
                                          unsigned mask) {
 #pragma unroll
@@ -212,8 +197,34 @@ __device__ __forceinline__ void sub_diag(v2d (&v)[kRegs], const v2d* __restrict_
     if (mask & (1u << r)) v[r] = cmul(ph[r], v[r]);
 }
 
-// position codes: D1 a = 0..3; D2 (a<b) in lexicographic order 0..5;
-// D3 (a<b<c) in lexicographic order 0..3
+// Exchanges with a runtime control mask: the pair structure (target bits)
+// is compile time, the control test a per-slot select, so no register is
+// ever indexed at run time.
+template <int T>
+__device__ __forceinline__ void sub_cx(v2d (&v)[kRegs], unsigned cm) {
+#pragma unroll
+  for (int r = 0; r < kRegs; ++r) {
+    if (r & (1 << T)) continue;
+    const bool on = ((unsigned)r & cm) == cm;
+    const v2d x = v[r], y = v[r | (1 << T)];
+    v[r] = on ? y : x;
+    v[r | (1 << T)] = on ? x : y;
+  }
+}
+
+template <int A, int B>
+__device__ __forceinline__ void sub_swap(v2d (&v)[kRegs], unsigned cm) {
+#pragma unroll
+  for (int r = 0; r < kRegs; ++r) {
+    if (!(r & (1 << A)) || (r & (1 << B))) continue;
+    constexpr int d = (1 << A) ^ (1 << B);
+    const bool on = ((unsigned)r & cm) == cm;
+    const v2d x = v[r], y = v[r ^ d];
+    v[r] = on ? y : x;
+    v[r ^ d] = on ? x : y;
+  }
+}
+
 __device__ __forceinline__ void apply_sub(v2d (&v)[kRegs], const int* __restrict__ sub,
                                           const v2d* __restrict__ coef) {
   const int type = sub[0], code = sub[1];
@@ -237,15 +248,29 @@ __device__ __forceinline__ void apply_sub(v2d (&v)[kRegs], const int* __restrict
         case 5: sub_d2<2, 3>(v, m); break;
       }
       break;
-    case SUB_D3:
-      switch (code) {
-        case 0: sub_d3<0, 1, 2>(v, m); break;
-        case 1: sub_d3<0, 1, 3>(v, m); break;
-        case 2: sub_d3<0, 2, 3>(v, m); break;
-        case 3: sub_d3<1, 2, 3>(v, m); break;
+    case SUB_DIAG: sub_diag(v, m, (unsigned)sub[3]); break;
+    case SUB_CX: {
+      const unsigned cm = (unsigned)code >> 2;
+      switch (code & 3) {
+        case 0: sub_cx<0>(v, cm); break;
+        case 1: sub_cx<1>(v, cm); break;
+        case 2: sub_cx<2>(v, cm); break;
+        case 3: sub_cx<3>(v, cm); break;
       }
       break;
-    case SUB_DIAG: sub_diag(v, m, (unsigned)sub[3]); break;
+    }
+    case SUB_SWAP: {
+      const unsigned cm = (unsigned)code >> 3;
+      switch (code & 7) {
+        case 0: sub_swap<0, 1>(v, cm); break;
+        case 1: sub_swap<0, 2>(v, cm); break;
+        case 2: sub_swap<0, 3>(v, cm); break;
+        case 3: sub_swap<1, 2>(v, cm); break;
+        case 4: sub_swap<1, 3>(v, cm); break;
+        case 5: sub_swap<2, 3>(v, cm); break;
+      }
+      break;
+    }
   }
 }
 
@@ -256,16 +281,21 @@ __device__ __forceinline__ void op_rphase(v2d* lds, int T, const qfx_tile_op& op
   const int ngroups = 1 << (T - kRegBits);
   const int* sub0 = subs + op.src;
   const int nsub = op.coef;
+  // sw is XOR-linear on disjoint bit sets: sw(lb | off) = sw(lb) ^ sw(off)
+  unsigned soff[kRegs];
+#pragma unroll
+  for (int r = 0; r < kRegs; ++r) soff[r] = sw(loc_off_lsb<kRegBits>(op.t, r));
   for (int g = threadIdx.x; g < ngroups; g += BS) {
     unsigned lb = (unsigned)g;
 #pragma unroll
     for (int j = 0; j < kRegBits; ++j) lb = ins0(lb, op.sorted[j]);
+    const unsigned slb = sw(lb);
     v2d v[kRegs];
 #pragma unroll
-    for (int r = 0; r < kRegs; ++r) v[r] = lds[sw(lb + loc_off_lsb<kRegBits>(op.t, r))];
+    for (int r = 0; r < kRegs; ++r) v[r] = lds[slb ^ soff[r]];
     for (int i = 0; i < nsub; ++i) apply_sub(v, sub0 + 4 * i, coef);
 #pragma unroll
-    for (int r = 0; r < kRegs; ++r) lds[sw(lb + loc_off_lsb<kRegBits>(op.t, r))] = v[r];
+    for (int r = 0; r < kRegs; ++r) lds[slb ^ soff[r]] = v[r];
   }
 }
 
@@ -532,11 +562,21 @@ int qfx_tile_check_ops(int T, int block, const qfx_tile_op* ops, int nops, int n
         return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: sub-ops out of range");
       for (int q = 0; q < o.coef; ++q) {
         const int* sub = srcs + o.src + 4 * q;
-        static const int ncode[5] = {0, 4, 6, 4, 1};
-        static const int need[5] = {0, 4, 16, 64, kRegs};
-        if (sub[0] < SUB_D1 || sub[0] > SUB_DIAG || sub[1] < 0 || sub[1] >= ncode[sub[0]])
+        // codes per type: D1 4, D2 6, DIAG 1, CX 4 targets x 16 masks,
+        // SWAP 6 pairs x 16 masks; control masks may not touch the targets
+        static const int ncode[7] = {0, 4, 6, 0, 1, 64, 128};
+        static const int need[7] = {0, 4, 16, 0, kRegs, 0, 0};
+        const int ty = sub[0], cd = sub[1];
+        if (ty < SUB_D1 || ty > SUB_SWAP || ty == 3 || cd < 0 || cd >= ncode[ty])
           return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: bad sub-op");
-        if (sub[2] < 0 || sub[2] + need[sub[0]] > ncoef)
+        if (ty == SUB_CX && (((cd >> 2) >> (cd & 3)) & 1))
+          return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: CX control on its target");
+        if (ty == SUB_SWAP) {
+          static const int pa[6] = {0, 0, 0, 1, 1, 2}, pb[6] = {1, 2, 3, 2, 3, 3};
+          if ((cd & 7) > 5 || (((cd >> 3) >> pa[cd & 7]) & 1) || (((cd >> 3) >> pb[cd & 7]) & 1))
+            return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: bad SWAP sub-op");
+        }
+        if (need[ty] && (sub[2] < 0 || sub[2] + need[ty] > ncoef))
           return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: sub-op coefficients out of range");
       }
       continue;

@@ -33,7 +33,7 @@ from . import backend as bk
 from .states import State
 
 OP_DENSE, OP_SPARSE, OP_RPHASE = 0, 1, 2
-SUB_D1, SUB_D2, SUB_D3, SUB_DIAG = 1, 2, 3, 4
+SUB_D1, SUB_D2, SUB_DIAG, SUB_CX, SUB_SWAP = 1, 2, 4, 5, 6
 REG_BITS = 4
 MAX_TILE_BITS = 13
 MIN_TILE_BITS = 6
@@ -41,7 +41,7 @@ TILE_WIDE = 1
 TILE_PERSISTENT = 2
 # position codes of the register-phase kernels (lexicographic)
 _D2_CODES = {p: i for i, p in enumerate([(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)])}
-_D3_CODES = {p: i for i, p in enumerate([(0, 1, 2), (0, 1, 3), (0, 2, 3), (1, 2, 3)])}
+_PAIRS = [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]
 
 
 class KernelTimer:
@@ -264,20 +264,76 @@ def _encode_op(op, pos_index, coefs, srcs):
 
 
 
+def exchange_pattern(kind):
+    """Describe a monomial gate's row permutation as a controlled X or a
+    controlled SWAP of its qubits (controls on |1>): ('cx', t, controls) or
+    ('swap', a, b, controls) in gate-qubit numbers, or None."""
+    if kind.kind != bk.GateKind.MONOMIAL:
+        return None
+    k = kind.k
+    src = [int(x) for x in kind.src]
+    q = list(range(k))
+
+    def bit(r, j):                      # value of gate qubit j in row r
+        return (r >> (k - 1 - j)) & 1
+
+    for t in q:
+        others = [j for j in q if j != t]
+        for m in range(1 << len(others)):
+            ctl = [others[i] for i in range(len(others)) if (m >> i) & 1]
+            want = [r ^ (1 << (k - 1 - t)) if all(bit(r, c) for c in ctl) else r
+                    for r in range(2 ** k)]
+            if want == src:
+                return ('cx', t, ctl)
+    for a in q:
+        for b in q:
+            if b <= a:
+                continue
+            others = [j for j in q if j not in (a, b)]
+            for m in range(1 << len(others)):
+                ctl = [others[i] for i in range(len(others)) if (m >> i) & 1]
+                want = []
+                for r in range(2 ** k):
+                    if all(bit(r, c) for c in ctl) and bit(r, a) != bit(r, b):
+                        r = r ^ (1 << (k - 1 - a)) ^ (1 << (k - 1 - b))
+                    want.append(r)
+                if want == src:
+                    return ('swap', a, b, ctl)
+    return None
+
+
+def phaseable(op):
+    """Can the op run inside a register phase?"""
+    if op.matrix is not None:
+        return len(op.bits) <= 2
+    kd = op.kind
+    if kd.kind == bk.GateKind.DIAGONAL:
+        return kd.k <= REG_BITS
+    if kd.kind == bk.GateKind.MONOMIAL and exchange_pattern(kd) is not None:
+        return kd.k <= REG_BITS
+    return kd.k <= 2
+
+
 def plan_phases(ops, width=REG_BITS):
-    """Group a pass's ops (in order) into register phases of <= width bits.
-    Same rule as plan_passes: an op joins the phase if its bits fit and it
-    does not touch a bit of an op already deferred, so every op it overtakes
-    acts on other qubits and commutes with it."""
+    """Group a pass's ops (in order) into register phases of <= width bits
+    and standalone LDS ops.  Same rule as plan_passes: an op joins the phase
+    if its bits fit and it does not touch a bit of an op already deferred, so
+    every op it overtakes acts on other qubits and commutes with it.  Returns
+    [('phase', bits, ops) | ('lds', op)]."""
     pending = list(range(len(ops)))
-    phases = []
+    items = []
     while pending:
-        R = set(ops[pending[0]].bits)
+        first = ops[pending[0]]
+        if not phaseable(first):
+            items.append(('lds', first))
+            pending = pending[1:]
+            continue
+        R = set(first.bits)
         blocked = set()
         taken, rest = [], []
         for i in pending:
             b = set(ops[i].bits)
-            if b & blocked:
+            if (b & blocked) or not phaseable(ops[i]):
                 blocked |= b
                 rest.append(i)
                 continue
@@ -288,9 +344,9 @@ def plan_phases(ops, width=REG_BITS):
             else:
                 blocked |= b
                 rest.append(i)
-        phases.append((R, [ops[i] for i in taken]))
+        items.append(('phase', R, [ops[i] for i in taken]))
         pending = rest
-    return phases
+    return items
 
 
 def _reorder_qubits(matrix, perm):
@@ -300,6 +356,19 @@ def _reorder_qubits(matrix, perm):
     t = np.asarray(matrix, dtype=np.complex128).reshape([2] * (2 * k))
     axes = list(perm) + [k + p for p in perm]
     return np.transpose(t, axes).reshape(2 ** k, 2 ** k)
+
+
+def _slot_phases(diag, p):
+    """16 register-slot phases of a diagonal on register bits p (gate MSB
+    first)."""
+    k = len(p)
+    ph = np.empty(2 ** REG_BITS, dtype=np.complex128)
+    for r in range(2 ** REG_BITS):
+        idx = 0
+        for j in range(k):
+            idx |= ((r >> p[j]) & 1) << (k - 1 - j)
+        ph[r] = diag[idx]
+    return ph
 
 
 def _encode_phase(R, members, pos_index, T, coefs, subs):
@@ -321,51 +390,65 @@ def _encode_phase(R, members, pos_index, T, coefs, subs):
     for j in range(5):
         o.sorted[j] = st[j] if j < REG_BITS else 0
     o.src = len(subs)
-    nsub = 0
-    pending_diag = None
+    out = []                                # pending sub-ops: [type, code, matrix/phases]
 
-    def flush_diag():
-        nonlocal pending_diag, nsub
-        if pending_diag is None:
-            return
-        mask = 0
-        for r in range(2 ** REG_BITS):
-            if pending_diag[r] != 1.0:
-                mask |= 1 << r
-        if mask:
-            subs.extend([SUB_DIAG, 0, len(coefs), mask])
-            coefs.extend(pending_diag)
-            nsub += 1
-        pending_diag = None
+    def push_diag(ph):
+        if out and out[-1][0] == SUB_DIAG:
+            out[-1][2] = out[-1][2] * ph
+        else:
+            out.append([SUB_DIAG, 0, ph])
 
     for op in members:
         p = [reg_of[pos_index[b]] for b in op.bits]        # register bit per gate qubit
         k = len(p)
         kd = op.kind
         if op.matrix is None and kd.kind == bk.GateKind.DIAGONAL:
-            ph = np.empty(2 ** REG_BITS, dtype=np.complex128)
-            for r in range(2 ** REG_BITS):
-                idx = 0
-                for j in range(k):
-                    idx |= ((r >> p[j]) & 1) << (k - 1 - j)
-                ph[r] = kd.diag[idx]
-            pending_diag = ph if pending_diag is None else pending_diag * ph
+            push_diag(_slot_phases(kd.diag, p))
             continue
-        flush_diag()
+        pat = exchange_pattern(kd) if op.matrix is None else None
+        if pat is not None:
+            if pat[0] == 'cx':
+                cm = 0
+                for c in pat[2]:
+                    cm |= 1 << p[c]
+                out.append([SUB_CX, p[pat[1]] | (cm << 2), None])
+            else:
+                a, b = sorted((p[pat[1]], p[pat[2]]))
+                cm = 0
+                for c in pat[3]:
+                    cm |= 1 << p[c]
+                out.append([SUB_SWAP, _PAIRS.index((a, b)) | (cm << 3), None])
+            if kd.phase is not None:
+                push_diag(_slot_phases(kd.phase, p))
+            continue
         m = op.dense_matrix()
         order = sorted(range(k), key=lambda j: -p[j])       # gate MSB <-> highest bit
         m = _reorder_qubits(m, order)
         ps = tuple(sorted(p))
         if k == 1:
-            typ, code = SUB_D1, ps[0]
-        elif k == 2:
-            typ, code = SUB_D2, _D2_CODES[ps]
+            if out and out[-1][0] == SUB_D1 and out[-1][1] == ps[0]:
+                out[-1][2] = m @ out[-1][2]                 # same bit: one 2x2
+                continue
+            out.append([SUB_D1, ps[0], m])
         else:
-            typ, code = SUB_D3, _D3_CODES[ps]
-        subs.extend([typ, code, len(coefs), 0])
-        coefs.extend(np.asarray(m, dtype=np.complex128).reshape(-1))
+            out.append([SUB_D2, _D2_CODES[ps], m])
+    nsub = 0
+    for typ, code, data in out:
+        if typ == SUB_DIAG:
+            mask = 0
+            for r in range(2 ** REG_BITS):
+                if data[r] != 1.0:
+                    mask |= 1 << r
+            if not mask:
+                continue
+            subs.extend([SUB_DIAG, 0, len(coefs), mask])
+            coefs.extend(data)
+        elif typ in (SUB_CX, SUB_SWAP):
+            subs.extend([typ, code, 0, 0])
+        else:
+            subs.extend([typ, code, len(coefs), 0])
+            coefs.extend(np.asarray(data, dtype=np.complex128).reshape(-1))
         nsub += 1
-    flush_diag()
     o.coef = nsub
     o.active = o.phase = 0
     return o
@@ -406,10 +489,13 @@ class TilePlan:
             wide = any(len(op.bits) > 3 for op in ops)
             start = len(ops_host)
             if self.mode == 'reg' and not wide:
-                for R, mem in plan_phases(ops):
-                    ops_host.append(_encode_phase(R, mem, pos_index, T, coefs, srcs))
+                for item in plan_phases(ops):
+                    if item[0] == 'phase':
+                        ops_host.append(_encode_phase(item[1], item[2], pos_index, T, coefs, srcs))
+                        self.n_phases += 1
+                    else:
+                        ops_host.append(_encode_op(item[1], pos_index, coefs, srcs))
                 block, flags = self.block, 0
-                self.n_phases += len(ops_host) - start
             else:
                 block = min(self.block, 256) if wide else self.block
                 width = self.fuse_width if block <= 256 else min(self.fuse_width, 3)

@@ -111,19 +111,30 @@ def _emulate_ops(ket, n, pos, ops, coefs, srcs):
             gb = [pos[o.t[j]] for j in range(fusion.REG_BITS)]
             u = np.eye(16, dtype=complex)
             d2 = [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]
-            d3 = [(0, 1, 2), (0, 1, 3), (0, 2, 3), (1, 2, 3)]
             for q in range(o.coef):
                 typ, code, c0, mask = srcs[o.src + 4 * q: o.src + 4 * q + 4]
                 if typ == fusion.SUB_DIAG:
                     ph = np.where([(mask >> r) & 1 for r in range(16)], coefs[c0:c0 + 16], 1.0)
                     u = np.diag(ph) @ u
                     continue
+                if typ in (fusion.SUB_CX, fusion.SUB_SWAP):
+                    perm = np.zeros((16, 16))
+                    for r in range(16):
+                        if typ == fusion.SUB_CX:
+                            t, cm = code & 3, code >> 2
+                            dst = r ^ (1 << t) if (r & cm) == cm else r
+                        else:
+                            a, b = d2[code & 7]
+                            cm = code >> 3
+                            on = (r & cm) == cm and ((r >> a) & 1) != ((r >> b) & 1)
+                            dst = r ^ (1 << a) ^ (1 << b) if on else r
+                        perm[dst, r] = 1
+                    u = perm @ u
+                    continue
                 if typ == fusion.SUB_D1:
                     bits = [code]
-                elif typ == fusion.SUB_D2:
-                    bits = list(d2[code])
                 else:
-                    bits = list(d3[code])
+                    bits = list(d2[code])
                 k = len(bits)
                 m = coefs[c0:c0 + 4 ** k].reshape(2 ** k, 2 ** k)
                 axes = [3 - b for b in reversed(bits)]      # gate MSB <-> highest bit
@@ -155,13 +166,58 @@ def test_register_phase_encoding_replays(T, seed):
     for pos, members in fusion.plan_passes(bits_list, n, T, 3):
         pidx = {b: j for j, b in enumerate(pos)}
         coefs, srcs, ops = [], [], []
-        for R, mem in fusion.plan_phases([fusion.Op(bits_list[i], gates[i].kind) for i in members]):
-            assert len(R) <= fusion.REG_BITS
-            ops.append(fusion._encode_phase(R, mem, pidx, T, coefs, srcs))
+        for item in fusion.plan_phases([fusion.Op(bits_list[i], gates[i].kind) for i in members]):
+            if item[0] == 'phase':
+                assert len(item[1]) <= fusion.REG_BITS
+                ops.append(fusion._encode_phase(item[1], item[2], pidx, T, coefs, srcs))
+            else:
+                ops.append(fusion._encode_op(item[1], pidx, coefs, srcs))
         nphase += len(ops)
         arr = (_native.QfxTileOp * len(ops))(*ops)
         assert _native.load().qfx_tile_check_ops(T, 256, arr, len(ops), len(coefs),
                                                  _native.int_array(srcs), len(srcs)) == 0
         ket = _emulate_ops(ket, n, pos, ops, coefs, srcs)
     assert nphase < len(gates)
+    assert np.max(np.abs(ket - ref)) < 1e-12
+
+
+def test_exchange_patterns():
+    from quantumflow_amd import stdgates as sg
+    assert fusion.exchange_pattern(sg.X().kind) == ('cx', 0, [])
+    assert fusion.exchange_pattern(sg.CNOT().kind) == ('cx', 1, [0])
+    assert fusion.exchange_pattern(sg.CCNOT().kind) == ('cx', 2, [0, 1])
+    assert fusion.exchange_pattern(sg.SWAP().kind) == ('swap', 0, 1, [])
+    assert fusion.exchange_pattern(sg.CSWAP().kind) == ('swap', 1, 2, [0])
+    assert fusion.exchange_pattern(sg.ISWAP().kind) == ('swap', 0, 1, [])
+    assert fusion.exchange_pattern(sg.Y().kind) == ('cx', 0, [])
+    assert fusion.exchange_pattern(sg.H().kind) is None
+
+
+@pytest.mark.parametrize("seed", [5, 6])
+def test_register_phases_permutation_heavy(seed):
+    """Circuits of X/CNOT/SWAP/ISWAP/CCNOT/CSWAP/Y and phases replay exactly."""
+    rng = np.random.default_rng(seed)
+    n, T = 9, 7
+    names = [('X', 1), ('Y', 1), ('CNOT', 2), ('SWAP', 2), ('ISWAP', 2), ('CCNOT', 3),
+             ('CSWAP', 3), ('CZ', 2), ('S', 1), ('H', 1)]
+    desc = []
+    for _ in range(80):
+        nm, k = names[int(rng.integers(len(names)))]
+        desc.append((nm, (), tuple(int(q) for q in rng.permutation(n)[:k])))
+    gates = build_gates(desc)
+    ref = O.run_circuit(desc, qubits=tuple(range(n)))
+    bits_list = [tuple(n - 1 - q for q in g.qubits) for g in gates]
+    ket = O.zero_state(n)
+    for pos, members in fusion.plan_passes(bits_list, n, T, 3):
+        pidx = {b: j for j, b in enumerate(pos)}
+        coefs, srcs, ops = [], [], []
+        for item in fusion.plan_phases([fusion.Op(bits_list[i], gates[i].kind) for i in members]):
+            if item[0] == 'phase':
+                ops.append(fusion._encode_phase(item[1], item[2], pidx, T, coefs, srcs))
+            else:
+                ops.append(fusion._encode_op(item[1], pidx, coefs, srcs))
+        arr = (_native.QfxTileOp * len(ops))(*ops)
+        assert _native.load().qfx_tile_check_ops(T, 256, arr, len(ops), len(coefs),
+                                                 _native.int_array(srcs), len(srcs)) == 0
+        ket = _emulate_ops(ket, n, pos, ops, coefs, srcs)
     assert np.max(np.abs(ket - ref)) < 1e-12
