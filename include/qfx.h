@@ -189,6 +189,85 @@ int qfx_tile_pass(void* stream, void* state, int n, int tile_bits,
 int qfx_cut_table(void* stream, void* out, int n, const void* edges,
                   const void* weights, int m);
 
+/* Unit-weight cut counts as uint16 (the same table as integers). */
+int qfx_cut_counts(void* stream, void* out, int n, const void* edges, int m);
+
+/* ---- batched QAOA engine (quantumflow_amd/qaoa_engine.py) -------------
+ * Expectation <C> and its adjoint gradient for many MaxCut instances in one
+ * persistent launch.  Replaces per instance the chain qubo_circuit ->
+ * Circuit.run -> State.expectation(graph_cuts) (quantumflow/qaoa.py:22-81,
+ * quantumflow/states.py:131-147) and the autograd gradient of
+ * examples/qaoa_maxcut.py:53-64.  The host builds, per instance, a program
+ * of tiled passes (tile bit set + stages); global pass k of all instances
+ * runs between grid barriers.  Every array below is device memory. */
+
+#define QFX_QAOA_MAX_LAYERS 16
+#define QFX_QAOA_MAX_STAGES 40
+#define QFX_QAOA_FORWARD 0   /* width 1: <C> */
+#define QFX_QAOA_BACKWARD 1  /* width 1 + 2p: <C>, d/dbeta_l, d/dgamma_l */
+
+typedef struct {
+  int32_t n;          /* qubits (graph nodes 0..n-1, node u <-> bit n-1-u) */
+  int32_t p;          /* layers */
+  int32_t weighted;   /* 0: uint16 cut counts; 1: float64 cut values */
+  int32_t nedges;     /* edges (phase-table size for unit weights) */
+  double wsum;        /* sum of edge weights W */
+  int64_t cut_off;    /* first cut-table entry in cnt_pool / val_pool */
+  int64_t psi_off;    /* first amplitude of the state in psi_pool */
+  int64_t lam_off;    /* first amplitude of lambda in lam_pool (backward) */
+  int32_t angle_off;  /* beta[0..p-1] then gamma[0..p-1] in angles */
+  int32_t slot0;      /* first partial slot of this instance */
+  int32_t nslots;     /* partial slots of this instance */
+  int32_t pad;
+} qfx_qaoa_inst;
+
+typedef struct {
+  int32_t T;          /* tile bits, <= 13 forward, <= 12 backward */
+  int32_t pos[13];    /* tile bit positions, ascending */
+  int32_t nstage;
+  int32_t flags;      /* 1 load psi, 2 load lambda, 4 store psi, 8 store lambda */
+  int32_t stage[QFX_QAOA_MAX_STAGES][3];  /* {type, layer, tile-bit mask} */
+} qfx_qaoa_pass;
+
+typedef struct {
+  int32_t inst;       /* instance */
+  int32_t pass;       /* pass descriptor */
+  int32_t tile;       /* tile index within the pass */
+  int32_t partial;    /* partial slot written by this item */
+} qfx_qaoa_item;
+
+typedef struct {
+  const void* insts;      /* qfx_qaoa_inst[ninst] */
+  const void* passes;     /* qfx_qaoa_pass[] */
+  const void* items;      /* qfx_qaoa_item[], grouped by global pass */
+  const void* kstart;     /* int32[nk + 1]: items of global pass k */
+  const void* angles;     /* float64 */
+  void* psi_pool;         /* complex128 */
+  void* lam_pool;         /* complex128 (backward) */
+  const void* cnt_pool;   /* uint16 cut counts */
+  const void* val_pool;   /* float64 cut values */
+  void* partials;         /* float64 [nslots][width] */
+  void* out;              /* float64 [ninst][width] */
+  void* sync;             /* >= 8 bytes of scratch: barrier counter + error flag */
+  int32_t ninst, nk, width, mode, block, max_items_per_k;
+} qfx_qaoa_args;
+
+/* Host-side validation of a program (host copies of the device arrays). */
+int qfx_qaoa_check(const qfx_qaoa_inst* insts, int ninst,
+                   const qfx_qaoa_pass* passes, int npasses,
+                   const qfx_qaoa_item* items, const int* kstart, int nk,
+                   int width, int64_t psi_pool_len, int64_t lam_pool_len,
+                   int64_t cnt_pool_len, int64_t val_pool_len,
+                   int64_t npartial, int mode);
+
+/* Enqueue the program: a barrier reset, the persistent kernel (grid = the
+ * workgroups that fit the device at once) and the per-instance reduction
+ * into out.  Capturable into a HIP graph. */
+int qfx_qaoa_run(void* stream, const qfx_qaoa_args* args);
+
+/* After the stream completes: *failed = 1 if a grid barrier timed out. */
+int qfx_qaoa_status(const void* sync, int* failed);
+
 #ifdef __cplusplus
 }
 #endif

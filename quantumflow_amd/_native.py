@@ -63,6 +63,10 @@ SIGNATURES = {
     'qfx_zero_state': (_i, [_vp, _vp, _i]),
     'qfx_plus_state': (_i, [_vp, _vp, _i]),
     'qfx_cut_table': (_i, [_vp, _vp, _i, _vp, _vp, _i]),
+    'qfx_cut_counts': (_i, [_vp, _vp, _i, _vp, _i]),
+    'qfx_qaoa_check': (_i, [_vp, _i, _vp, _i, _vp, _vp, _i, _i, _i64, _i64, _i64, _i64, _i64, _i]),
+    'qfx_qaoa_run': (_i, [_vp, _vp]),
+    'qfx_qaoa_status': (_i, [_vp, _vp]),
     'qfx_tile_check_ops': (_i, [_i, _i, _vp, _i, _i, _ip, _i]),
     'qfx_tile_pass': (_i, [_vp, _vp, _i, _i, _ip, _vp, _i, _vp, _vp, _i, _i]),
 }

@@ -33,9 +33,39 @@ __global__ __launch_bounds__(kBlock) void cut_table_kernel(double* __restrict__ 
   out[z] = c;
 }
 
+// Unit-weight cut counts as uint16 (sum over edges in edge order).
+__global__ __launch_bounds__(kBlock) void cut_count_kernel(uint16_t* __restrict__ out, int n,
+                                                          const int* __restrict__ edges, int m) {
+  __shared__ int s_u[kMaxEdges];
+  __shared__ int s_v[kMaxEdges];
+  for (int e = threadIdx.x; e < m; e += kBlock) {
+    s_u[e] = n - 1 - edges[2 * e];
+    s_v[e] = n - 1 - edges[2 * e + 1];
+  }
+  __syncthreads();
+  const uint64_t len = (uint64_t)1 << n;
+  const uint64_t z = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (z >= len) return;
+  unsigned c = 0;
+  for (int e = 0; e < m; ++e) c += (unsigned)(((z >> s_u[e]) ^ (z >> s_v[e])) & 1);
+  out[z] = (uint16_t)c;
+}
+
 }  // namespace
 
 extern "C" {
+
+int qfx_cut_counts(void* stream, void* out, int n, const void* edges, int m) {
+  if (!out || (!edges && m > 0)) return qfx::set_error(QFX_EINVAL, "qfx_cut_counts: null pointer");
+  if (n < 1 || n > QFX_MAX_QUBITS) return qfx::set_error(QFX_ERANGE, "qfx_cut_counts: n out of range");
+  if (m < 0 || m > kMaxEdges || m > 65535)
+    return qfx::set_error(QFX_ERANGE, "qfx_cut_counts: too many edges");
+  const uint64_t len = (uint64_t)1 << n;
+  hipLaunchKernelGGL(cut_count_kernel, dim3((unsigned)((len + kBlock - 1) / kBlock)), dim3(kBlock),
+                     0, qfx::as_stream(stream), static_cast<uint16_t*>(out), n,
+                     static_cast<const int*>(edges), m);
+  return qfx::check_launch("qfx_cut_counts");
+}
 
 int qfx_cut_table(void* stream, void* out, int n, const void* edges, const void* weights,
                   int m) {

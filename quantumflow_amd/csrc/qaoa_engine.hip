@@ -1,0 +1,578 @@
+// Batched QAOA MaxCut engine: expectation values and adjoint gradients of
+// many instances (graphs of different sizes) in one persistent launch.
+//
+// Replaces, for a whole batch at once, the reference's per-instance chain
+//   qubo_circuit (quantumflow/qaoa.py:22-64) -> Circuit.run
+//   (quantumflow/circuits.py:87-99) -> State.expectation(graph_cuts)
+//   (quantumflow/states.py:131-147, quantumflow/qaoa.py:68-81)
+// and the TensorFlow autograd gradient of examples/qaoa_maxcut.py:53-64.
+//
+// Per instance the state is |+>^n, then per layer l the cost layer
+// prod_e ZZ(-w gamma_l / pi) = exp(i gamma_l (W/2 - C(z))) (W = sum of edge
+// weights, C(z) the cut value) and the mixer RX(beta_l) on every qubit.
+//
+// Execution: every instance has a host-built program of passes.  A pass
+// names a tile bit set S (|S| = T <= 13) and a list of stages; each of its
+// 2^(n-T) tiles is one work item.  Global pass k runs the k-th pass of every
+// instance; a grid barrier separates k from k+1.  Instances with n <= T run
+// entirely in LDS inside one item.  Within a tile the mixer is applied as
+// register butterflies over up to 4 bits at a time, the cost layer as a
+// table lookup of exp(i gamma (W/2 - c)) by integer cut count c.
+//
+// Adjoint backward (mode QFX_QAOA_BACKWARD): starting from the stored final
+// state psi and lambda = C psi, the stages are undone in reverse order; the
+// gradient of <C> is
+//   d/d beta_l  = Im sum_j <lambda|X_j|psi>          (any point of mixer l)
+//   d/d gamma_l = 2 Im <lambda|C|psi> - W Im <lambda|psi>   (after cost l)
+// with lambda, psi carried back by RX(-beta) and the conjugate phases.
+#include <cmath>
+#include <cstring>
+
+#include "qfx_common.h"
+
+namespace {
+
+enum Stage : int {
+  ST_INIT = 1,         // tile <- 2^(-n/2)
+  ST_COST = 2,         // psi *= exp(i g (W/2 - C))
+  ST_RX = 3,           // RX(beta) on the tile bits in mask
+  ST_EXPECT = 4,       // acc += sum |psi|^2 C
+  ST_MAKE_LAMBDA = 5,  // lambda <- C psi
+  ST_RX_ADJ = 6,       // grad_beta += Im sum <lam|X_j|psi>; RX(-beta) on psi, lam
+  ST_COST_ADJ = 7,     // grad_gamma += 2 Im <lam|C|psi> - W Im <lam|psi>; conj phases
+};
+
+enum PassFlags : int { PF_LOAD_PSI = 1, PF_LOAD_LAM = 2, PF_STORE_PSI = 4, PF_STORE_LAM = 8 };
+
+constexpr int kMaxT = 13;
+constexpr int kMaxPLayers = QFX_QAOA_MAX_LAYERS;
+constexpr int kAccum = 1 + 2 * kMaxPLayers;  // value, d/dbeta, d/dgamma
+constexpr int kMaxEdgesTab = 512;  // cut counts 0..kMaxEdgesTab-1 tabulated
+constexpr unsigned kSpinLimit = 1u << 22;
+
+__device__ __forceinline__ unsigned swz(unsigned e) {
+  return e ^ (((e >> 3) ^ (e >> 6) ^ (e >> 9) ^ (e >> 12)) & 7u);
+}
+
+__device__ __forceinline__ v2d cmulc(const v2d a, const v2d b) {  // conj(a) * b
+  v2d r;
+  r.x = a.x * b.x + a.y * b.y;
+  r.y = a.x * b.y - a.y * b.x;
+  return r;
+}
+
+struct Ctx {
+  const qfx_qaoa_inst* inst;
+  const qfx_qaoa_pass* ps;
+  uint64_t base;      // tile base index (non-tile bits)
+  int T;
+  int n;
+  int p;
+  const double* beta;
+  const double* gamma;
+  const uint16_t* cnt;  // cut counts of this instance (or null)
+  const double* cval;   // weighted cut values (or null)
+};
+
+// global index of tile element e
+__device__ __forceinline__ uint64_t gidx(const Ctx& c, unsigned e) {
+  uint64_t z = c.base;
+#pragma unroll
+  for (int j = 0; j < kMaxT; ++j)
+    if (j < c.T) z |= (uint64_t)((e >> j) & 1u) << c.ps->pos[j];
+  return z;
+}
+
+__device__ __forceinline__ double cut_of(const Ctx& c, uint64_t z) {
+  return c.cnt ? (double)c.cnt[z] : c.cval[z];
+}
+
+// Block-wide sum; result valid in thread 0.  red must hold >= 16 doubles.
+template <int BS>
+__device__ __forceinline__ double block_total(double v, double* red) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < BS / 64; ++w) t += red[w];
+  return t;
+}
+
+// RX(theta) butterflies on register bits 0..R-1 of v (register bit j <->
+// tile bit b[j]); c = cos(theta/2), s = sin(theta/2).
+// new_x = c x - i s y ; new_y = c y - i s x
+template <int R>
+__device__ __forceinline__ void rx_regs(v2d (&v)[16], double c, double s) {
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+#pragma unroll
+    for (int r = 0; r < (1 << R); ++r) {
+      if (r & (1 << j)) continue;
+      const v2d x = v[r], y = v[r | (1 << j)];
+      v2d nx, ny;
+      nx.x = fma(c, x.x, s * y.y);
+      nx.y = fma(c, x.y, -s * y.x);
+      ny.x = fma(c, y.x, s * x.y);
+      ny.y = fma(c, y.y, -s * x.x);
+      v[r] = nx;
+      v[r | (1 << j)] = ny;
+    }
+  }
+}
+
+// Im of sum over register pairs across bit j of conj(l_x) p_y + conj(l_y) p_x
+template <int R>
+__device__ __forceinline__ double xgrad_regs(const v2d (&l)[16], const v2d (&pv)[16]) {
+  double g = 0.0;
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+#pragma unroll
+    for (int r = 0; r < (1 << R); ++r) {
+      if (r & (1 << j)) continue;
+      const int q = r | (1 << j);
+      g += cmulc(l[r], pv[q]).y + cmulc(l[q], pv[r]).y;
+    }
+  }
+  return g;
+}
+
+// Apply RX to the tile bits in `mask` (chunks of <= 4 bits in registers);
+// with ADJ also to lambda (angle -beta) and accumulate the beta gradient.
+template <int BS, int R, bool ADJ>
+__device__ double rx_chunk(v2d* L, v2d* LL, int T, const int* bits, double c, double s) {
+  const int ngroups = 1 << (T - R);
+  int sorted[4];
+#pragma unroll
+  for (int j = 0; j < R; ++j) sorted[j] = bits[j];
+  // ascending insertion order
+#pragma unroll
+  for (int a = 0; a < R; ++a)
+#pragma unroll
+    for (int b = a + 1; b < R; ++b)
+      if (sorted[b] < sorted[a]) {
+        const int t = sorted[a];
+        sorted[a] = sorted[b];
+        sorted[b] = t;
+      }
+  unsigned soff[1 << R];
+#pragma unroll
+  for (int r = 0; r < (1 << R); ++r) {
+    unsigned o = 0;
+#pragma unroll
+    for (int j = 0; j < R; ++j) o |= ((unsigned)(r >> j) & 1u) << bits[j];
+    soff[r] = swz(o);
+  }
+  double g = 0.0;
+  for (int grp = threadIdx.x; grp < ngroups; grp += BS) {
+    unsigned lb = (unsigned)grp;
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      const int b = sorted[j];
+      lb = ((lb >> b) << (b + 1)) | (lb & ((1u << b) - 1u));
+    }
+    const unsigned slb = swz(lb);
+    v2d v[16];
+#pragma unroll
+    for (int r = 0; r < (1 << R); ++r) v[r] = L[slb ^ soff[r]];
+    if (ADJ) {
+      v2d l[16];
+#pragma unroll
+      for (int r = 0; r < (1 << R); ++r) l[r] = LL[slb ^ soff[r]];
+      g += xgrad_regs<R>(l, v);
+      rx_regs<R>(l, c, s);
+#pragma unroll
+      for (int r = 0; r < (1 << R); ++r) LL[slb ^ soff[r]] = l[r];
+    }
+    rx_regs<R>(v, c, s);
+#pragma unroll
+    for (int r = 0; r < (1 << R); ++r) L[slb ^ soff[r]] = v[r];
+  }
+  return g;
+}
+
+template <int BS, bool ADJ>
+__device__ double rx_mask(v2d* L, v2d* LL, int T, unsigned mask, double theta) {
+  const double c = cos(0.5 * theta), s = sin(0.5 * theta);
+  double g = 0.0;
+  while (mask) {
+    int bits[4];
+    int r = 0;
+    while (mask && r < 4) {
+      const int b = __builtin_ctz(mask);
+      mask &= mask - 1;
+      bits[r++] = b;
+    }
+    switch (r) {
+      case 1: g += rx_chunk<BS, 1, ADJ>(L, LL, T, bits, c, s); break;
+      case 2: g += rx_chunk<BS, 2, ADJ>(L, LL, T, bits, c, s); break;
+      case 3: g += rx_chunk<BS, 3, ADJ>(L, LL, T, bits, c, s); break;
+      default: g += rx_chunk<BS, 4, ADJ>(L, LL, T, bits, c, s); break;
+    }
+    __syncthreads();
+  }
+  return g;
+}
+
+// exp(i g (W/2 - c)) for c = 0..ncut into tab (conj if CONJ)
+template <int BS>
+__device__ void phase_table(v2d* tab, int ncut, double g, double wsum, bool conj) {
+  for (int c = threadIdx.x; c <= ncut; c += BS) {
+    double sn, cs;
+    sincos(g * (0.5 * wsum - (double)c), &sn, &cs);
+    v2d t;
+    t.x = cs;
+    t.y = conj ? -sn : sn;
+    tab[c] = t;
+  }
+  __syncthreads();
+}
+
+template <int BS>
+__device__ void run_item(const qfx_qaoa_inst& in, const qfx_qaoa_pass& ps, uint32_t tile,
+                         const double* __restrict__ angles, v2d* __restrict__ psi_pool,
+                         v2d* __restrict__ lam_pool, const uint16_t* __restrict__ cnt_pool,
+                         const double* __restrict__ val_pool, double* __restrict__ partial,
+                         int width, v2d* lds, double* red, double* accum, v2d* tab) {
+  Ctx c;
+  c.inst = &in;
+  c.ps = &ps;
+  c.T = ps.T;
+  c.n = in.n;
+  c.p = in.p;
+  c.beta = angles + in.angle_off;
+  c.gamma = angles + in.angle_off + in.p;
+  c.cnt = in.weighted ? nullptr : cnt_pool + in.cut_off;
+  c.cval = in.weighted ? val_pool + in.cut_off : nullptr;
+  uint64_t base = tile;
+#pragma unroll
+  for (int j = 0; j < kMaxT; ++j) {
+    if (j < c.T) {
+      const int b = ps.pos[j];
+      base = ((base >> b) << (b + 1)) | (base & (((uint64_t)1 << b) - 1));
+    }
+  }
+  c.base = base;
+  const unsigned tsize = 1u << c.T;
+  v2d* L = lds;
+  v2d* LL = lds + tsize;
+  v2d* psi = psi_pool + in.psi_off;
+  v2d* lam = lam_pool ? lam_pool + in.lam_off : nullptr;
+  if (threadIdx.x < kAccum) accum[threadIdx.x] = 0.0;
+  if (ps.flags & PF_LOAD_PSI)
+    for (unsigned e = threadIdx.x; e < tsize; e += BS) L[swz(e)] = psi[gidx(c, e)];
+  if (ps.flags & PF_LOAD_LAM)
+    for (unsigned e = threadIdx.x; e < tsize; e += BS) LL[swz(e)] = lam[gidx(c, e)];
+  __syncthreads();
+  const bool unit = !in.weighted;
+  for (int si = 0; si < ps.nstage; ++si) {
+    const int type = ps.stage[si][0], layer = ps.stage[si][1];
+    const unsigned mask = (unsigned)ps.stage[si][2];
+    switch (type) {
+      case ST_INIT: {
+        const double a = ldexp(1.0, -(c.n / 2)) * ((c.n & 1) ? 0.70710678118654752440 : 1.0);
+        for (unsigned e = threadIdx.x; e < tsize; e += BS) L[swz(e)] = v2d{a, 0.0};
+        __syncthreads();
+        break;
+      }
+      case ST_COST:
+      case ST_COST_ADJ: {
+        const double g = c.gamma[layer];
+        const bool adj = type == ST_COST_ADJ;
+        if (unit) phase_table<BS>(tab, in.nedges, g, in.wsum, adj);
+        double acc = 0.0;
+        for (unsigned e = threadIdx.x; e < tsize; e += BS) {
+          const uint64_t z = gidx(c, e);
+          v2d ph;
+          double cz;
+          if (unit) {
+            const int k = c.cnt[z];
+            cz = (double)k;
+            ph = tab[k];
+          } else {
+            cz = c.cval[z];
+            double sn, cs;
+            sincos(g * (0.5 * in.wsum - cz), &sn, &cs);
+            ph = v2d{cs, adj ? -sn : sn};
+          }
+          const unsigned sl = swz(e);
+          const v2d x = L[sl];
+          if (adj) {
+            const v2d l = LL[sl];
+            const v2d lp = cmulc(l, x);  // conj(l) psi
+            acc += 2.0 * cz * lp.y - in.wsum * lp.y;
+            LL[sl] = cmul(ph, l);
+          }
+          L[sl] = cmul(ph, x);
+        }
+        if (adj) {
+          const double t = block_total<BS>(acc, red);
+          if (threadIdx.x == 0) accum[1 + c.p + layer] += t;
+        }
+        __syncthreads();
+        break;
+      }
+      case ST_RX: {
+        rx_mask<BS, false>(L, LL, c.T, mask, c.beta[layer]);
+        break;
+      }
+      case ST_RX_ADJ: {
+        const double gsum = rx_mask<BS, true>(L, LL, c.T, mask, -c.beta[layer]);
+        const double t = block_total<BS>(gsum, red);
+        if (threadIdx.x == 0) accum[1 + layer] += t;
+        __syncthreads();
+        break;
+      }
+      case ST_EXPECT: {
+        double acc = 0.0;
+        for (unsigned e = threadIdx.x; e < tsize; e += BS) {
+          const v2d x = L[swz(e)];
+          acc += fma(x.x, x.x, x.y * x.y) * cut_of(c, gidx(c, e));
+        }
+        const double t = block_total<BS>(acc, red);
+        if (threadIdx.x == 0) accum[0] += t;
+        __syncthreads();
+        break;
+      }
+      case ST_MAKE_LAMBDA: {
+        for (unsigned e = threadIdx.x; e < tsize; e += BS) {
+          const unsigned sl = swz(e);
+          const double cz = cut_of(c, gidx(c, e));
+          const v2d x = L[sl];
+          LL[sl] = v2d{cz * x.x, cz * x.y};
+        }
+        __syncthreads();
+        break;
+      }
+    }
+  }
+  if (ps.flags & PF_STORE_PSI)
+    for (unsigned e = threadIdx.x; e < tsize; e += BS) psi[gidx(c, e)] = L[swz(e)];
+  if (ps.flags & PF_STORE_LAM)
+    for (unsigned e = threadIdx.x; e < tsize; e += BS) lam[gidx(c, e)] = LL[swz(e)];
+  __syncthreads();
+  if ((int)threadIdx.x < width) partial[threadIdx.x] = accum[threadIdx.x];
+  __syncthreads();
+}
+
+// Grid barrier: monotone counter, bounded spin; on timeout sets *err and
+// returns false (every wave then leaves the kernel).
+__device__ bool grid_barrier(unsigned* counter, unsigned target, int* err) {
+  __syncthreads();
+  __shared__ int s_ok;
+  if (threadIdx.x == 0) {
+    __threadfence();
+    atomicAdd(counter, 1u);
+    unsigned spins = 0;
+    int ok = 1;
+    while (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      __builtin_amdgcn_s_sleep(8);
+      if (++spins > kSpinLimit || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+        atomicExch(err, 1);
+        ok = 0;
+        break;
+      }
+    }
+    __threadfence();
+    s_ok = ok;
+  }
+  __syncthreads();
+  return s_ok != 0;
+}
+
+template <int BS>
+__global__ __launch_bounds__(BS) void qaoa_kernel(
+    const qfx_qaoa_inst* __restrict__ insts, const qfx_qaoa_pass* __restrict__ passes,
+    const qfx_qaoa_item* __restrict__ items, const int* __restrict__ kstart, int nk,
+    const double* __restrict__ angles, v2d* __restrict__ psi_pool, v2d* __restrict__ lam_pool,
+    const uint16_t* __restrict__ cnt_pool, const double* __restrict__ val_pool,
+    double* __restrict__ partials, int width, unsigned* sync, int* err) {
+  extern __shared__ v2d lds[];
+  __shared__ double red[16];
+  __shared__ double accum[kAccum];
+  __shared__ v2d tab[kMaxEdgesTab];
+  for (int k = 0; k < nk; ++k) {
+    for (int it = kstart[k] + (int)blockIdx.x; it < kstart[k + 1]; it += gridDim.x) {
+      const qfx_qaoa_item item = items[it];
+      const qfx_qaoa_inst& in = insts[item.inst];
+      run_item<BS>(in, passes[item.pass], (uint32_t)item.tile, angles, psi_pool, lam_pool,
+                   cnt_pool, val_pool, partials + (uint64_t)item.partial * width, width, lds,
+                   red, accum, tab);
+    }
+    if (k + 1 < nk && !grid_barrier(sync, (unsigned)(k + 1) * gridDim.x, err)) return;
+  }
+}
+
+// out[i * width + j] = sum of the instance's partial slots, in slot order
+__global__ void qaoa_reduce_kernel(const qfx_qaoa_inst* __restrict__ insts, int ninst,
+                                   const double* __restrict__ partials, int width,
+                                   double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ninst * width) return;
+  const int inst = i / width, j = i % width;
+  const qfx_qaoa_inst& in = insts[inst];
+  double s = 0.0;
+  for (int q = 0; q < in.nslots; ++q) s += partials[(uint64_t)(in.slot0 + q) * width + j];
+  out[i] = s;
+}
+
+__global__ void zero_sync_kernel(unsigned* sync, int* err) {
+  if (threadIdx.x == 0) {
+    sync[0] = 0u;
+    err[0] = 0;
+  }
+}
+
+template <int BS>
+int launch_qaoa(hipStream_t s, const qfx_qaoa_args& a, int T) {
+  const size_t lds = (sizeof(v2d) << T) * (a.mode == QFX_QAOA_BACKWARD ? 2 : 1);
+  static int grid = 0;
+  static size_t grid_lds = 0;
+  if (grid == 0 || grid_lds != lds) {
+    int rc = qfx::check_hip(hipFuncSetAttribute((const void*)qaoa_kernel<BS>,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                (int)lds),
+                            "hipFuncSetAttribute");
+    if (rc) return rc;
+    int per_cu = 0, dev = 0, cus = 0;
+    rc = qfx::check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                            &per_cu, (const void*)qaoa_kernel<BS>, BS, lds),
+                        "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+    if (rc) return rc;
+    if (per_cu < 1) return qfx::set_error(QFX_ERANGE, "qfx_qaoa_run: kernel does not fit a CU");
+    rc = qfx::check_hip(hipGetDevice(&dev), "hipGetDevice");
+    if (rc) return rc;
+    rc = qfx::check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev),
+                        "hipDeviceGetAttribute");
+    if (rc) return rc;
+    grid = per_cu * cus;
+    grid_lds = lds;
+  }
+  int g = grid;
+  if (a.max_items_per_k > 0 && a.max_items_per_k < g) g = a.max_items_per_k;
+  auto* sync = static_cast<unsigned*>(a.sync);
+  auto* err = reinterpret_cast<int*>(sync + 1);
+  hipLaunchKernelGGL(zero_sync_kernel, dim3(1), dim3(64), 0, s, sync, err);
+  int rc = qfx::check_launch("zero_sync_kernel");
+  if (rc) return rc;
+  hipLaunchKernelGGL((qaoa_kernel<BS>), dim3(g), dim3(BS), lds, s,
+                     static_cast<const qfx_qaoa_inst*>(a.insts),
+                     static_cast<const qfx_qaoa_pass*>(a.passes),
+                     static_cast<const qfx_qaoa_item*>(a.items), static_cast<const int*>(a.kstart),
+                     a.nk, static_cast<const double*>(a.angles), static_cast<v2d*>(a.psi_pool),
+                     static_cast<v2d*>(a.lam_pool), static_cast<const uint16_t*>(a.cnt_pool),
+                     static_cast<const double*>(a.val_pool), static_cast<double*>(a.partials),
+                     a.width, sync, err);
+  rc = qfx::check_launch("qaoa_kernel");
+  if (rc) return rc;
+  const int nthr = a.ninst * a.width;
+  hipLaunchKernelGGL(qaoa_reduce_kernel, dim3((nthr + 255) / 256), dim3(256), 0, s,
+                     static_cast<const qfx_qaoa_inst*>(a.insts), a.ninst,
+                     static_cast<const double*>(a.partials), a.width,
+                     static_cast<double*>(a.out));
+  return qfx::check_launch("qaoa_reduce_kernel");
+}
+
+}  // namespace
+
+extern "C" {
+
+int qfx_qaoa_check(const qfx_qaoa_inst* insts, int ninst, const qfx_qaoa_pass* passes,
+                   int npasses, const qfx_qaoa_item* items, const int* kstart, int nk, int width,
+                   int64_t psi_pool_len, int64_t lam_pool_len, int64_t cnt_pool_len,
+                   int64_t val_pool_len, int64_t npartial, int mode) {
+  const int tmax = mode == QFX_QAOA_BACKWARD ? 12 : 13;
+  for (int i = 0; i < ninst; ++i) {
+    const qfx_qaoa_inst& in = insts[i];
+    if (in.n < 1 || in.n > 30 || in.p < 1 || in.p > kMaxPLayers)
+      return qfx::set_error(QFX_ERANGE, "qfx_qaoa_check: n or p out of range");
+    const int64_t len = (int64_t)1 << in.n;
+    if (in.weighted) {
+      if (in.cut_off < 0 || in.cut_off + len > val_pool_len)
+        return qfx::set_error(QFX_EINVAL, "qfx_qaoa_check: cut values out of range");
+    } else {
+      if (in.cut_off < 0 || in.cut_off + len > cnt_pool_len)
+        return qfx::set_error(QFX_EINVAL, "qfx_qaoa_check: cut counts out of range");
+      if (in.nedges < 0 || in.nedges >= kMaxEdgesTab)
+        return qfx::set_error(QFX_ERANGE, "qfx_qaoa_check: too many edges for the phase table");
+    }
+    if (in.psi_off < 0 || in.psi_off + len > psi_pool_len)
+      return qfx::set_error(QFX_EINVAL, "qfx_qaoa_check: state out of range");
+    if (mode == QFX_QAOA_BACKWARD && (in.lam_off < 0 || in.lam_off + len > lam_pool_len))
+      return qfx::set_error(QFX_EINVAL, "qfx_qaoa_check: lambda out of range");
+    if (in.slot0 < 0 || in.nslots < 0 || in.slot0 + in.nslots > npartial)
+      return qfx::set_error(QFX_EINVAL, "qfx_qaoa_check: partial slots out of range");
+  }
+  if (width < 1 || width > kAccum)
+    return qfx::set_error(QFX_ERANGE, "qfx_qaoa_check: width out of range");
+  for (int q = 0; q < npasses; ++q) {
+    const qfx_qaoa_pass& ps = passes[q];
+    if (ps.T < 1 || ps.T > tmax || ps.nstage < 0 || ps.nstage > QFX_QAOA_MAX_STAGES)
+      return qfx::set_error(QFX_ERANGE, "qfx_qaoa_check: pass tile or stage count out of range");
+    for (int j = 0; j < ps.T; ++j)
+      if (ps.pos[j] < 0 || ps.pos[j] > 30 || (j && ps.pos[j] <= ps.pos[j - 1]))
+        return qfx::set_error(QFX_EINVAL, "qfx_qaoa_check: tile positions not ascending");
+    for (int si = 0; si < ps.nstage; ++si) {
+      const int ty = ps.stage[si][0], l = ps.stage[si][1];
+      const unsigned mask = (unsigned)ps.stage[si][2];
+      if (ty < ST_INIT || ty > ST_COST_ADJ || l < 0 || l >= kMaxPLayers)
+        return qfx::set_error(QFX_EINVAL, "qfx_qaoa_check: bad stage");
+      if ((ty == ST_RX || ty == ST_RX_ADJ) && (mask >> ps.T))
+        return qfx::set_error(QFX_EINVAL, "qfx_qaoa_check: RX mask outside the tile");
+    }
+  }
+  if (nk < 0 || kstart[0] != 0)
+    return qfx::set_error(QFX_EINVAL, "qfx_qaoa_check: bad pass index table");
+  for (int k = 0; k < nk; ++k)
+    if (kstart[k + 1] < kstart[k]) return qfx::set_error(QFX_EINVAL, "qfx_qaoa_check: kstart");
+  for (int it = 0; it < kstart[nk]; ++it) {
+    const qfx_qaoa_item& item = items[it];
+    if (item.inst < 0 || item.inst >= ninst || item.pass < 0 || item.pass >= npasses ||
+        item.partial < 0 || item.partial >= npartial)
+      return qfx::set_error(QFX_EINVAL, "qfx_qaoa_check: item out of range");
+    const qfx_qaoa_inst& in = insts[item.inst];
+    const qfx_qaoa_pass& ps = passes[item.pass];
+    if (ps.T > in.n || item.tile < 0 || ((int64_t)item.tile >> (in.n - ps.T)) != 0)
+      return qfx::set_error(QFX_EINVAL, "qfx_qaoa_check: tile out of range");
+    for (int j = 0; j < ps.T; ++j)
+      if (ps.pos[j] >= in.n) return qfx::set_error(QFX_EINVAL, "qfx_qaoa_check: tile bit >= n");
+    for (int si = 0; si < ps.nstage; ++si)
+      if (ps.stage[si][1] >= in.p) return qfx::set_error(QFX_EINVAL, "qfx_qaoa_check: layer >= p");
+  }
+  return QFX_OK;
+}
+
+int qfx_qaoa_run(void* stream, const qfx_qaoa_args* args) {
+  if (!args) return qfx::set_error(QFX_EINVAL, "qfx_qaoa_run: null args");
+  const qfx_qaoa_args& a = *args;
+  if (!a.insts || !a.passes || !a.items || !a.kstart || !a.angles || !a.psi_pool ||
+      !a.partials || !a.out || !a.sync || (!a.cnt_pool && !a.val_pool))
+    return qfx::set_error(QFX_EINVAL, "qfx_qaoa_run: null pointer");
+  if (a.mode == QFX_QAOA_BACKWARD && !a.lam_pool)
+    return qfx::set_error(QFX_EINVAL, "qfx_qaoa_run: backward needs lam_pool");
+  if (a.width < 1 || a.width > kAccum || a.ninst < 1 || a.nk < 1)
+    return qfx::set_error(QFX_ERANGE, "qfx_qaoa_run: sizes out of range");
+  const int T = a.mode == QFX_QAOA_BACKWARD ? 12 : 13;
+  hipStream_t s = qfx::as_stream(stream);
+  switch (a.block) {
+    case 256: return launch_qaoa<256>(s, a, T);
+    case 512: return launch_qaoa<512>(s, a, T);
+    case 1024: return launch_qaoa<1024>(s, a, T);
+  }
+  return qfx::set_error(QFX_EINVAL, "qfx_qaoa_run: block must be 256, 512 or 1024");
+}
+
+int qfx_qaoa_status(const void* sync, int* failed) {
+  if (!sync || !failed) return qfx::set_error(QFX_EINVAL, "qfx_qaoa_status: null pointer");
+  int e = 0;
+  int rc = qfx::check_hip(hipMemcpy(&e, static_cast<const unsigned*>(sync) + 1, sizeof(int),
+                                    hipMemcpyDeviceToHost),
+                          "hipMemcpy");
+  if (rc) return rc;
+  *failed = e;
+  return QFX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,426 @@
+"""Batched QAOA MaxCut on the GPU: values, adjoint gradients, optimizer.
+
+The reference evaluates one graph at a time: qubo_circuit
+(quantumflow/qaoa.py:22-64) -> Circuit.run -> State.expectation(graph_cuts)
+(quantumflow/states.py:131-147, quantumflow/qaoa.py:68-81), and gets
+gradients from TensorFlow autograd, which stores every intermediate state
+(examples/qaoa_maxcut.py:53-64).  ``QAOABatch`` evaluates many graphs of
+different sizes in one persistent kernel launch (csrc/qaoa_engine.hip), and
+in QFX_QAOA_BACKWARD mode also returns d<C>/d(beta, gamma) by the adjoint
+method, keeping two states (psi and lambda) per instance instead of one per
+gate.  ``QAOAExpectation`` puts it behind torch.autograd; ``optimize`` runs
+the reference's plain gradient ascent for all instances at once.
+
+Program of one instance (built on the host by ``build_program``): the gate
+sequence H^n, [cost(l), RX(l) on every qubit] for l < p, then <C>, and in
+backward mode lambda = C psi followed by the stages undone in reverse.  It is
+split into passes over tile bit sets of at most T bits (13 forward, 12
+backward): the low 13 (12) bits, then the low 3 bits plus the next chunk of
+high bits, and so on; an RX stage on bits outside the current set closes the
+pass.  Instances with n <= T run as one pass entirely in LDS.
+"""
+
+import ctypes
+import math
+
+import networkx as nx
+import numpy as np
+import torch
+
+from . import _native
+from . import backend as bk
+
+MODE_FORWARD, MODE_BACKWARD = 0, 1
+ST_INIT, ST_COST, ST_RX, ST_EXPECT, ST_MAKE_LAMBDA, ST_RX_ADJ, ST_COST_ADJ = 1, 2, 3, 4, 5, 6, 7
+PF_LOAD_PSI, PF_LOAD_LAM, PF_STORE_PSI, PF_STORE_LAM = 1, 2, 4, 8
+MAX_LAYERS = 16
+MAX_STAGES = 40
+LOW_BITS = 3
+
+
+class QaoaInst(ctypes.Structure):
+    _fields_ = [('n', ctypes.c_int32), ('p', ctypes.c_int32), ('weighted', ctypes.c_int32),
+                ('nedges', ctypes.c_int32), ('wsum', ctypes.c_double),
+                ('cut_off', ctypes.c_int64), ('psi_off', ctypes.c_int64),
+                ('lam_off', ctypes.c_int64), ('angle_off', ctypes.c_int32),
+                ('slot0', ctypes.c_int32), ('nslots', ctypes.c_int32), ('pad', ctypes.c_int32)]
+
+
+class QaoaPass(ctypes.Structure):
+    _fields_ = [('T', ctypes.c_int32), ('pos', ctypes.c_int32 * 13), ('nstage', ctypes.c_int32),
+                ('flags', ctypes.c_int32), ('stage', (ctypes.c_int32 * 3) * MAX_STAGES)]
+
+
+class QaoaItem(ctypes.Structure):
+    _fields_ = [('inst', ctypes.c_int32), ('pass_', ctypes.c_int32), ('tile', ctypes.c_int32),
+                ('partial', ctypes.c_int32)]
+
+
+class QaoaArgs(ctypes.Structure):
+    _fields_ = [(nm, ctypes.c_void_p) for nm in
+                ('insts', 'passes', 'items', 'kstart', 'angles', 'psi_pool', 'lam_pool',
+                 'cnt_pool', 'val_pool', 'partials', 'out', 'sync')] + \
+               [(nm, ctypes.c_int32) for nm in
+                ('ninst', 'nk', 'width', 'mode', 'block', 'max_items_per_k')]
+
+
+def tile_sets(n, T, low=None):
+    """Tile bit sets covering n index bits: the lowest T bits, then the low
+    `low` bits plus successive chunks of T - low higher bits, each padded to
+    T bits with the lowest bits it does not hold (bigger tiles, fewer work
+    items, and more bits on which pending RX stages can run).  By default
+    `low` is 3 (128-byte contiguous runs), lowered to 2 when that is what
+    lets two sets cover all bits."""
+    if n <= T:
+        return [tuple(range(n))]
+    if low is None:
+        low = LOW_BITS if 2 * T - n >= LOW_BITS else (2 if 2 * T - n == 2 else LOW_BITS)
+    sets = [tuple(range(T))]
+    lo = list(range(low))
+    b = T
+    while b < n:
+        chunk = list(range(b, min(n, b + T - low)))
+        st = set(lo + chunk)
+        j = 0
+        while len(st) < T:
+            st.add(j)
+            j += 1
+        sets.append(tuple(sorted(st)))
+        b += T - low
+    return sets
+
+
+def build_program(n, p, mode, T):
+    """Passes of one instance: list of (tile bits, [(stage, layer, global bit
+    set)], flags)."""
+    allbits = frozenset(range(n))
+    seq = [(ST_INIT, 0, None)]
+    for l in range(p):
+        seq.append((ST_COST, l, None))
+        seq.append((ST_RX, l, allbits))
+    seq.append((ST_EXPECT, 0, None))
+    if mode == MODE_BACKWARD:
+        seq.append((ST_MAKE_LAMBDA, 0, None))
+        for l in reversed(range(p)):
+            seq.append((ST_RX_ADJ, l, allbits))
+            seq.append((ST_COST_ADJ, l, None))
+    sets = tile_sets(n, T)
+    passes = []
+    cur = [sets[0], []]
+    for typ, layer, bits in seq:
+        if bits is None:
+            cur[1].append((typ, layer, None))
+            continue
+        remaining = set(bits)
+        while remaining:
+            here = remaining & set(cur[0])
+            if here:
+                cur[1].append((typ, layer, frozenset(here)))
+                remaining -= here
+            if remaining:
+                passes.append(cur)
+                best = max(sets, key=lambda s_: len(remaining & set(s_)))
+                cur = [best, []]
+    passes.append(cur)
+    # load/store flags
+    out = []
+    lam_live = False
+    for q, (tset, stages) in enumerate(passes):
+        flags = 0
+        if q > 0:
+            flags |= PF_LOAD_PSI
+            if lam_live:
+                flags |= PF_LOAD_LAM
+        if any(st[0] == ST_MAKE_LAMBDA for st in stages):
+            lam_live = True
+        if q < len(passes) - 1:
+            flags |= PF_STORE_PSI
+            if lam_live:
+                flags |= PF_STORE_LAM
+        out.append((tuple(tset), stages, flags))
+    return out
+
+
+def graph_arrays(graph):
+    n = len(graph)
+    if sorted(graph.nodes()) != list(range(n)):
+        raise ValueError('QAOA graphs need node labels 0..n-1')
+    edges = [(u, v) for u, v in graph.edges()]
+    ws = [graph[u][v].get('weight', None) for u, v in edges]
+    weighted = any(w is not None and float(w) != 1.0 for w in ws)
+    w = np.array([1.0 if x is None else float(x) for x in ws], dtype=np.float64)
+    return n, np.array(edges, dtype=np.int32).reshape(-1, 2), w, weighted
+
+
+class QAOABatch:
+    """A fixed batch of MaxCut instances compiled for the GPU engine.
+
+    graphs: networkx graphs with nodes 0..n-1 (edge attribute 'weight'
+    optional); p: layers; mode: MODE_FORWARD (values) or MODE_BACKWARD
+    (values and gradients)."""
+
+    def __init__(self, graphs, p, mode=MODE_BACKWARD, block=256):
+        if not 1 <= p <= MAX_LAYERS:
+            raise ValueError('p must be 1..%d' % MAX_LAYERS)
+        self.graphs = list(graphs)
+        self.p = p
+        self.mode = mode
+        self.block = block
+        self.width = 1 if mode == MODE_FORWARD else 1 + 2 * p
+        T = 13 if mode == MODE_FORWARD else 12
+        self.T = T
+        dev = bk.DEVICE
+        bk.stream_handle()                          # initialise the device
+        ninst = len(self.graphs)
+        insts = (QaoaInst * ninst)()
+        pass_list, items_by_k = [], []
+        psi_len = lam_len = 0
+        cnt_len = val_len = 0
+        slot = 0
+        self.n = []
+        self.maxcut = np.zeros(ninst)
+        cut_jobs = []
+        for i, g in enumerate(self.graphs):
+            n, edges, w, weighted = graph_arrays(g)
+            self.n.append(n)
+            prog = build_program(n, p, mode, T)
+            inst = insts[i]
+            inst.n, inst.p, inst.weighted, inst.nedges = n, p, int(weighted), len(edges)
+            inst.wsum = float(np.sum(w))
+            if weighted:
+                inst.cut_off = val_len
+                val_len += 2 ** n
+            else:
+                inst.cut_off = cnt_len
+                cnt_len += 2 ** n
+            cut_jobs.append((n, edges, w, weighted, inst.cut_off))
+            if len(prog) > 1:
+                inst.psi_off = psi_len
+                psi_len += 2 ** n
+                if mode == MODE_BACKWARD:
+                    inst.lam_off = lam_len
+                    lam_len += 2 ** n
+            inst.angle_off = 2 * p * i
+            inst.slot0 = slot
+            for q, (tset, stages, flags) in enumerate(prog):
+                ps = QaoaPass()
+                ps.T = len(tset)
+                for j, b in enumerate(tset):
+                    ps.pos[j] = b
+                ps.nstage = len(stages)
+                if len(stages) > MAX_STAGES:
+                    raise ValueError('program pass has too many stages (p too large)')
+                local = {b: j for j, b in enumerate(tset)}
+                accum = False
+                for si, (typ, layer, bits) in enumerate(stages):
+                    mask = 0
+                    if bits is not None:
+                        for b in bits:
+                            mask |= 1 << local[b]
+                    ps.stage[si][0], ps.stage[si][1], ps.stage[si][2] = typ, layer, mask
+                    accum |= typ in (ST_EXPECT, ST_RX_ADJ, ST_COST_ADJ)
+                ps.flags = flags
+                pidx = len(pass_list)
+                pass_list.append(ps)
+                while len(items_by_k) <= q:
+                    items_by_k.append([])
+                for t in range(2 ** (n - ps.T)):
+                    items_by_k[q].append((i, pidx, t, slot if accum else -1, 2 ** n))
+                    if accum:
+                        slot += 1
+            inst.nslots = slot - inst.slot0
+        self.dummy_slot = slot            # items without a reduction write here
+        nslots = slot + 1
+        # global pass k: single-pass (whole-instance) items first, then big tiles
+        items, kstart = [], [0]
+        for k, lst in enumerate(items_by_k):
+            lst.sort(key=lambda x: (-(pass_list[x[1]].T == self.n[x[0]]) * x[4], x[0], x[2]))
+            for (i, pidx, t, sl, _) in lst:
+                it = QaoaItem()
+                it.inst, it.pass_, it.tile = i, pidx, t
+                it.partial = sl if sl >= 0 else self.dummy_slot
+                items.append(it)
+            kstart.append(len(items))
+        self.nk = len(items_by_k)
+        self.nitems = len(items)
+        psi_len = max(psi_len, 1)
+        lam_len = max(lam_len, 1)
+        # host-side validation of the whole program
+        passes_arr = (QaoaPass * len(pass_list))(*pass_list)
+        items_arr = (QaoaItem * len(items))(*items)
+        ks = (ctypes.c_int * len(kstart))(*kstart)
+        lib = _native.load()
+        _native.check(lib.qfx_qaoa_check(insts, ninst, passes_arr, len(pass_list), items_arr, ks,
+                                         self.nk, self.width, psi_len,
+                                         lam_len if mode == MODE_BACKWARD else 0,
+                                         max(cnt_len, 1) if cnt_len else 0,
+                                         max(val_len, 1) if val_len else 0, nslots, mode),
+                      'qfx_qaoa_check')
+
+        def up(arr):
+            return torch.frombuffer(bytearray(bytes(arr)), dtype=torch.uint8).to(dev)
+        self._insts = up(insts)
+        self._passes = up(passes_arr)
+        self._items = up(items_arr)
+        self._kstart = torch.tensor(kstart, dtype=torch.int32, device=dev)
+        self.psi_pool = torch.empty(psi_len, dtype=torch.complex128, device=dev)
+        self.lam_pool = torch.empty(lam_len if mode == MODE_BACKWARD else 1,
+                                    dtype=torch.complex128, device=dev)
+        self.cnt_pool = torch.zeros(max(cnt_len, 1), dtype=torch.int16, device=dev)
+        self.val_pool = torch.zeros(max(val_len, 1), dtype=torch.float64, device=dev)
+        self.partials = torch.zeros(nslots * self.width, dtype=torch.float64, device=dev)
+        self.out = torch.zeros(ninst * self.width, dtype=torch.float64, device=dev)
+        self.sync = torch.zeros(4, dtype=torch.int32, device=dev)
+        self.angles = torch.zeros(ninst * 2 * p, dtype=torch.float64, device=dev)
+        # cut tables on the GPU
+        s = bk.stream_handle()
+        for (n, edges, w, weighted, off) in cut_jobs:
+            e = torch.from_numpy(edges.copy() if len(edges) else np.zeros((1, 2), np.int32)).to(dev)
+            if weighted:
+                wt = torch.from_numpy(w).to(dev)
+                _native.call('qfx_cut_table', s, self._vp(self.val_pool, off * 8), n,
+                             self._vp(e), self._vp(wt), len(edges))
+            else:
+                _native.call('qfx_cut_counts', s, self._vp(self.cnt_pool, off * 2), n,
+                             self._vp(e), len(edges))
+        for i, (n, edges, w, weighted, off) in enumerate(cut_jobs):
+            tab = self.val_pool[off:off + 2 ** n] if weighted else \
+                self.cnt_pool[off:off + 2 ** n]
+            self.maxcut[i] = float(tab.max().item()) if weighted else \
+                float((tab.to(torch.int32) & 0xFFFF).max().item())
+        self._args = QaoaArgs()
+        a = self._args
+        a.insts, a.passes, a.items = self._insts.data_ptr(), self._passes.data_ptr(), \
+            self._items.data_ptr()
+        a.kstart, a.angles = self._kstart.data_ptr(), self.angles.data_ptr()
+        a.psi_pool, a.lam_pool = self.psi_pool.data_ptr(), self.lam_pool.data_ptr()
+        a.cnt_pool, a.val_pool = self.cnt_pool.data_ptr(), self.val_pool.data_ptr()
+        a.partials, a.out, a.sync = self.partials.data_ptr(), self.out.data_ptr(), \
+            self.sync.data_ptr()
+        a.ninst, a.nk, a.width, a.mode, a.block = ninst, self.nk, self.width, mode, block
+        a.max_items_per_k = max(kstart[k + 1] - kstart[k] for k in range(self.nk))
+        self._graph = None
+
+    @staticmethod
+    def _vp(t, byte_off=0):
+        return ctypes.c_void_p(t.data_ptr() + byte_off)
+
+    @property
+    def ninst(self):
+        return len(self.graphs)
+
+    def set_angles(self, beta, gamma):
+        """beta, gamma: [ninst, p] arrays or tensors (copied to the device)."""
+        b = torch.as_tensor(beta, dtype=torch.float64).reshape(self.ninst, self.p)
+        g = torch.as_tensor(gamma, dtype=torch.float64).reshape(self.ninst, self.p)
+        self.angles.copy_(torch.cat([b, g], dim=1).reshape(-1).to(self.angles.device))
+
+    def launch(self):
+        """Enqueue one evaluation on torch's current stream; results in
+        ``self.out`` ([ninst, width]: <C>, then d/dbeta, d/dgamma)."""
+        _native.call('qfx_qaoa_run', bk.stream_handle(), ctypes.byref(self._args))
+
+    def capture(self):
+        """Record launch() into a HIP graph (replayed by ``replay``)."""
+        self.launch()                               # sets kernel attributes
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.launch()
+        self._graph = g
+        return g
+
+    def replay(self):
+        if self._graph is None:
+            self.capture()
+        self._graph.replay()
+
+    def check(self):
+        """Synchronise and raise if a grid barrier timed out."""
+        failed = ctypes.c_int(0)
+        torch.cuda.synchronize()
+        _native.call('qfx_qaoa_status', ctypes.c_void_p(self.sync.data_ptr()), ctypes.byref(failed))
+        if failed.value:
+            raise RuntimeError('qaoa engine: grid barrier timed out')
+
+    def evaluate(self, beta, gamma, graph=False):
+        """(values [ninst], grads [ninst, 2p] or None) as device tensors."""
+        self.set_angles(beta, gamma)
+        if graph:
+            self.replay()
+        else:
+            self.launch()
+        out = self.out.view(self.ninst, self.width)
+        if self.mode == MODE_FORWARD:
+            return out[:, 0], None
+        return out[:, 0], out[:, 1:]
+
+
+class QAOAExpectation(torch.autograd.Function):
+    """<C>(beta, gamma) for every instance of a QAOABatch; the backward pass
+    uses the engine's adjoint gradients (computed in the same launch)."""
+
+    @staticmethod
+    def forward(ctx, beta, gamma, batch):
+        val, grad = batch.evaluate(beta.detach(), gamma.detach())
+        ctx.p = batch.p
+        ctx.save_for_backward(grad.clone() if grad is not None else val.new_zeros(0))
+        return val.clone().to(beta.device)
+
+    @staticmethod
+    def backward(ctx, gout):
+        (grad,) = ctx.saved_tensors
+        if grad.numel() == 0:
+            raise RuntimeError('QAOABatch built in MODE_FORWARD has no gradients')
+        g = grad.to(gout.device) * gout[:, None]
+        return g[:, :ctx.p], g[:, ctx.p:], None
+
+
+def qaoa_expectation(beta, gamma, batch):
+    """Differentiable batched QAOA expectation (see QAOAExpectation)."""
+    return QAOAExpectation.apply(beta, gamma, batch)
+
+
+def optimize(batch, beta0, gamma0, learning_rate=0.01, max_steps=10000, block=10,
+             min_gain=1e-4, use_graph=True, optimizer='gd'):
+    """Gradient ascent on <C> for all instances at once, with the reference's
+    stopping rule per instance (examples/qaoa_maxcut.py:63-82): 10-step
+    blocks, an instance stops when its ratio <C>/maxcut gains less than
+    1e-4 over a block.  optimizer 'gd' is the reference's plain gradient
+    descent on -<C>; 'adam' uses Adam steps.  Returns (ratios, beta, gamma,
+    steps per instance)."""
+    if batch.mode != MODE_BACKWARD:
+        raise ValueError('optimize needs a MODE_BACKWARD batch')
+    dev = batch.angles.device
+    beta = torch.as_tensor(beta0, dtype=torch.float64, device=dev).reshape(batch.ninst, batch.p).clone()
+    gamma = torch.as_tensor(gamma0, dtype=torch.float64, device=dev).reshape(batch.ninst, batch.p).clone()
+    maxcut = torch.as_tensor(batch.maxcut, dtype=torch.float64, device=dev)
+    active = torch.ones(batch.ninst, dtype=torch.bool, device=dev)
+    steps = torch.zeros(batch.ninst, dtype=torch.int64, device=dev)
+    last = torch.full((batch.ninst,), -1.0, dtype=torch.float64, device=dev)
+    ratio = torch.zeros(batch.ninst, dtype=torch.float64, device=dev)
+    m = torch.zeros(batch.ninst, 2 * batch.p, dtype=torch.float64, device=dev)
+    v = torch.zeros_like(m)
+    t = 0
+    for step in range(0, max_steps, block):
+        for _ in range(block):
+            val, grad = batch.evaluate(beta, gamma, graph=use_graph)
+            if optimizer == 'adam':
+                t += 1
+                m.mul_(0.9).add_(0.1 * grad)
+                v.mul_(0.999).add_(0.001 * grad * grad)
+                upd = learning_rate * (m / (1 - 0.9 ** t)) / (torch.sqrt(v / (1 - 0.999 ** t)) + 1e-8)
+            else:
+                upd = learning_rate * grad
+            upd = upd * active[:, None]
+            beta = beta + upd[:, :batch.p]
+            gamma = gamma + upd[:, batch.p:]
+            steps += active.to(torch.int64)
+        val, _ = batch.evaluate(beta, gamma, graph=use_graph)
+        ratio = torch.where(active, val / maxcut, ratio)
+        done = (ratio - last) < min_gain
+        active = active & ~done
+        last = torch.where(active, ratio, last)
+        if not bool(active.any()):
+            break
+    return ratio, beta, gamma, steps

@@ -1,0 +1,120 @@
+"""Host checks of the batched QAOA engine's programs (no GPU).
+
+``emulate`` runs a program's stages, in pass order, on a full numpy state
+with the stage semantics the kernel implements.  Matching the oracle's
+gate-by-gate <C> and its exact parameter-shift gradient pins the pass
+scheduling (every RX bit once per layer, cost layers in order) and the
+adjoint formulas
+  d<C>/dbeta_l  = Im sum_j <lambda|X_j|psi>,
+  d<C>/dgamma_l = 2 Im <lambda|C|psi> - W Im <lambda|psi>.
+"""
+import networkx as nx
+import numpy as np
+import pytest
+
+from oracle import qf_oracle as O
+from quantumflow_amd import qaoa_engine as E
+
+
+def emulate(n, edges, weights, beta, gamma, mode, T):
+    cuts = O.graph_cuts(n, edges, weights).reshape(-1)
+    W = float(len(edges) if weights is None else np.sum(weights))
+    p = len(beta)
+    prog = E.build_program(n, p, mode, T)
+    idx = np.arange(2 ** n)
+    psi = lam = None
+    value = 0.0
+    gb = np.zeros(p)
+    gg = np.zeros(p)
+
+    def rx(v, bits, theta):
+        c, s = np.cos(theta / 2), np.sin(theta / 2)
+        for b in bits:
+            lo = v[(idx >> b) & 1 == 0]
+            hi = v[(idx >> b) & 1 == 1]
+            v = v.copy()
+            v[(idx >> b) & 1 == 0] = c * lo - 1j * s * hi
+            v[(idx >> b) & 1 == 1] = c * hi - 1j * s * lo
+        return v
+
+    seen = {}
+    for tset, stages, flags in prog:
+        for typ, layer, bits in stages:
+            if bits is not None:
+                assert set(bits) <= set(tset)
+            if typ == E.ST_INIT:
+                psi = np.full(2 ** n, 2.0 ** (-n / 2), dtype=complex)
+            elif typ == E.ST_COST:
+                psi = psi * np.exp(1j * gamma[layer] * (W / 2 - cuts))
+            elif typ == E.ST_RX:
+                for b in bits:
+                    seen[(layer, b)] = seen.get((layer, b), 0) + 1
+                psi = rx(psi, sorted(bits), beta[layer])
+            elif typ == E.ST_EXPECT:
+                value += np.sum(np.abs(psi) ** 2 * cuts)
+            elif typ == E.ST_MAKE_LAMBDA:
+                lam = cuts * psi
+            elif typ == E.ST_RX_ADJ:
+                for b in bits:
+                    partner = psi[idx ^ (1 << b)]
+                    gb[layer] += np.sum(np.conj(lam) * partner).imag
+                psi = rx(psi, sorted(bits), -beta[layer])
+                lam = rx(lam, sorted(bits), -beta[layer])
+            elif typ == E.ST_COST_ADJ:
+                lp = np.conj(lam) * psi
+                gg[layer] += np.sum(2 * cuts * lp.imag - W * lp.imag)
+                ph = np.exp(-1j * gamma[layer] * (W / 2 - cuts))
+                psi, lam = psi * ph, lam * ph
+    assert all(seen.get((l, b), 0) == 1 for l in range(p) for b in range(n))
+    return value, gb, gg
+
+
+@pytest.mark.parametrize("n,T,p", [(5, 12, 2), (7, 4, 2), (8, 5, 3), (9, 6, 1)])
+def test_program_emulation_matches_oracle(n, T, p):
+    rng = np.random.default_rng(n * 10 + T)
+    g = nx.gnp_random_graph(n, 0.5, seed=int(rng.integers(1000)))
+    edges = list(g.edges())
+    beta = rng.uniform(0, np.pi / 2, size=p)
+    gamma = rng.uniform(0, np.pi, size=p)
+    ref = O.qaoa_expectation(n, edges, beta, gamma)
+    v, gb, gg = emulate(n, edges, None, beta, gamma, E.MODE_BACKWARD, T)
+    assert abs(v - ref) < 1e-12
+    sb, sg = O.qaoa_grad_shift(n, edges, beta, gamma)
+    assert np.max(np.abs(gb - sb)) < 1e-12
+    assert np.max(np.abs(gg - sg)) < 1e-12
+    v2, _, _ = emulate(n, edges, None, beta, gamma, E.MODE_FORWARD, T + 1)
+    assert abs(v2 - ref) < 1e-12
+
+
+def test_weighted_program_gradient():
+    rng = np.random.default_rng(3)
+    n, p = 6, 2
+    edges = [(0, 1), (1, 2), (2, 3), (3, 4), (4, 5), (0, 5), (1, 4)]
+    w = list(rng.uniform(0.2, 2.0, size=len(edges)))
+    beta = rng.uniform(0, 1, size=p)
+    gamma = rng.uniform(0, 1, size=p)
+    v, gb, gg = emulate(n, edges, w, beta, gamma, E.MODE_BACKWARD, 4)
+    assert abs(v - O.qaoa_expectation(n, edges, beta, gamma, w)) < 1e-12
+    sb, sg = O.qaoa_grad_shift(n, edges, beta, gamma, w)
+    assert np.max(np.abs(gb - sb)) < 1e-12 and np.max(np.abs(gg - sg)) < 1e-12
+
+
+@pytest.mark.parametrize("n", [14, 20, 22, 24])
+def test_pass_counts(n):
+    fwd = E.build_program(n, 4, E.MODE_FORWARD, 13)
+    bwd = E.build_program(n, 4, E.MODE_BACKWARD, 12)
+    assert len(fwd) <= 2 * 4 + 1
+    assert len(bwd) <= 4 * 4 + 1
+    for prog in (fwd, bwd):
+        assert prog[0][2] & E.PF_LOAD_PSI == 0          # first pass builds |+>
+        assert prog[-1][2] & E.PF_STORE_PSI == 0        # last pass stores nothing
+        for tset, stages, flags in prog:
+            assert len(tset) <= 13 and list(tset) == sorted(tset)
+
+
+def test_tile_sets_cover():
+    for n in range(1, 27):
+        for T in (12, 13):
+            sets = E.tile_sets(n, T)
+            assert set().union(*map(set, sets)) == set(range(n))
+            assert all(len(s) == min(T, n) for s in sets)
