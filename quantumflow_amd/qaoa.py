@@ -67,3 +67,61 @@ def graph_cuts(graph: nx.Graph) -> bk.BKTensor:
                  _native.ctypes.c_void_p(e.data_ptr()),
                  None if w is None else _native.ctypes.c_void_p(w.data_ptr()), len(edges))
     return out.view([2] * n)
+
+
+def plain_value(graph: nx.Graph, beta: Sequence, gamma: Sequence):
+    """<C> by the reference's chain, one kernel launch per gate:
+    qubo_circuit -> Circuit.run -> State.expectation(graph_cuts)
+    (examples/qaoa_maxcut.py:56-59).  Returns a 0-dim device tensor."""
+    from .states import zero_state
+    circ = qubo_circuit(graph, len(beta), beta, gamma)
+    ket = circ.run(zero_state(list(graph.nodes())), fused=False)
+    return ket.expectation(graph_cuts(graph))
+
+
+def plain_value_and_grad(graph: nx.Graph, beta: Sequence, gamma: Sequence):
+    """<C> and d<C>/d(beta, gamma) on the plain per-gate path: the forward
+    gate by gate, then an adjoint sweep gate by gate (psi and lambda carried
+    back by each gate's inverse; every RX and ZZ occurrence adds its
+    <lambda|dG|psi> term).  Returns (value, grad [2p]) as device tensors."""
+    from .fusion import apply_gates_inplace
+    from . import stdgates
+    p = len(beta)
+    n = len(graph)
+    qubits = tuple(range(n))
+    circ = qubo_circuit(graph, p, beta, gamma)
+    gates = circ.elements
+    psi = bk.zero_state_tensor(n).reshape(-1).clone()
+    apply_gates_inplace(gates, qubits, psi)
+    cuts = graph_cuts(graph).reshape(-1)
+    value = bk.expectation(cuts, psi)
+    lam = (psi * cuts).contiguous()
+    tmp = torch.empty_like(psi)
+    grad = torch.zeros(2 * p, dtype=torch.float64, device=psi.device)
+    zz_kind = bk.classify(np.diag([1.0, -1.0, -1.0, 1.0]))
+    # layer of each parametrised gate occurrence, in circuit order
+    layer_of = []
+    for l in range(p):
+        layer_of += [l] * (graph.number_of_edges() + n)
+    pos = len(gates) - 1
+    occ = len(layer_of) - 1
+    while pos >= 0:
+        g = gates[pos]
+        bits = [n - 1 - qubits.index(q) for q in g.qubits]
+        if isinstance(g, (stdgates.RX, stdgates.ZZ)):
+            l = layer_of[occ]
+            occ -= 1
+            if isinstance(g, stdgates.RX):
+                # d/dbeta RX(beta) = (-i X/2) RX: term Im <lam|X|psi>
+                bk.apply_gate(tmp, psi, n, bits, stdgates.X().kind)
+                grad[l] += bk.inner(lam, tmp).imag
+            else:
+                # ZZ(t), t = -w gamma/pi: d/dgamma = (i w / 2) Z.Z ZZ(t)
+                w = float(graph[g.qubits[0]][g.qubits[1]].get('weight', 1.0))
+                bk.apply_gate(tmp, psi, n, bits, zz_kind)
+                grad[p + l] += -w * bk.inner(lam, tmp).imag
+        inv = g.H.kind
+        bk.apply_gate(psi, psi, n, bits, inv, inplace_sparse=True)
+        bk.apply_gate(lam, lam, n, bits, inv, inplace_sparse=True)
+        pos -= 1
+    return value, grad

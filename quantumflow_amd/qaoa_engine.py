@@ -177,6 +177,7 @@ class QAOABatch:
         psi_len = lam_len = 0
         cnt_len = val_len = 0
         slot = 0
+        single_max = 1
         self.n = []
         self.maxcut = np.zeros(ninst)
         cut_jobs = []
@@ -200,6 +201,10 @@ class QAOABatch:
                 if mode == MODE_BACKWARD:
                     inst.lam_off = lam_len
                     lam_len += 2 ** n
+            else:
+                # a one-pass instance never touches the pools; its offsets
+                # point at the start, which the pools are sized to cover
+                single_max = max(single_max, 2 ** n)
             inst.angle_off = 2 * p * i
             inst.slot0 = slot
             for q, (tset, stages, flags) in enumerate(prog):
@@ -243,8 +248,8 @@ class QAOABatch:
             kstart.append(len(items))
         self.nk = len(items_by_k)
         self.nitems = len(items)
-        psi_len = max(psi_len, 1)
-        lam_len = max(lam_len, 1)
+        psi_len = max(psi_len, single_max)
+        lam_len = max(lam_len, single_max)
         # host-side validation of the whole program
         passes_arr = (QaoaPass * len(pass_list))(*pass_list)
         items_arr = (QaoaItem * len(items))(*items)

@@ -188,3 +188,19 @@ def test_optimizer_star_kat(E):
     g0 = rng.normal(0.5, 0.01, size=(1, 5))
     ratio, beta, gamma, steps = E.optimize(batch, b0, g0, learning_rate=0.01, max_steps=4000)
     assert ratio[0].item() > 0.95
+
+
+def test_plain_path_matches_engine(E):
+    from quantumflow_amd import qaoa as Q
+    rng = np.random.default_rng(8)
+    graphs = [er(7, 0.5, 81), er(11, 0.5, 82)]
+    p = 2
+    beta = rng.uniform(0, 1, size=(2, p))
+    gamma = rng.uniform(0, 1, size=(2, p))
+    vb, gb = E.QAOABatch(graphs, p).evaluate(beta, gamma)
+    for i, g in enumerate(graphs):
+        v1 = Q.plain_value(g, beta[i], gamma[i]).item()
+        v2, g2 = Q.plain_value_and_grad(g, beta[i], gamma[i])
+        assert abs(v1 - vb[i].item()) <= 1e-10
+        assert abs(v2.item() - vb[i].item()) <= 1e-10
+        assert np.max(np.abs(g2.cpu().numpy() - gb[i].cpu().numpy())) <= 1e-10
