@@ -52,6 +52,86 @@ def cpu_baseline(desc, n, ngates):
                       % (ngates, n, DEPTH, os.cpu_count(), len(os.sched_getaffinity(0)))}
 
 
+def qaoa_section(steps, plain_sample=24, grad_sample=4):
+    """Batched QAOA (512 ER graphs, n 10..22, p=4: values; 64 3-regular
+    graphs, n 16..24, p=8: values + adjoint gradients) against the plain
+    per-gate GPU path on samples of the same instances."""
+    import numpy as np
+    import torch
+    from quantumflow_amd import qaoa as Q
+    from quantumflow_amd import qaoa_engine as E
+    from quantumflow_amd.workloads import qaoa_batch_instances, regular_instances
+
+    def timed(fn, reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    out = {}
+    # values: 512 ER instances
+    inst = qaoa_batch_instances(512, 10, 22, 4, seed=11)
+    graphs = [g for g, _, _ in inst]
+    beta = np.array([b for _, b, _ in inst])
+    gamma = np.array([g for _, _, g in inst])
+    t0 = time.perf_counter()
+    batch = E.QAOABatch(graphs, 4, mode=E.MODE_FORWARD)
+    batch.set_angles(beta, gamma)
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+    batch.capture()
+    dt = timed(batch.replay, steps)
+    batch.check()
+    vals = batch.out.view(-1).cpu().numpy()
+    tp = timed(lambda: [Q.plain_value(graphs[i], beta[i], gamma[i]) for i in range(plain_sample)], 1)
+    plain_vals = [Q.plain_value(graphs[i], beta[i], gamma[i]).item() for i in range(plain_sample)]
+    out["forward"] = {
+        "workload": "512 ER graphs n=10..22 p_edge=0.3..0.6, p=4, complex128",
+        "batched_instances_per_s": round(512 / dt, 1), "batched_ms": round(dt * 1e3, 3),
+        "setup_s (programs + cut tables, once per graph set)": round(setup, 3),
+        "plain_instances_per_s": round(plain_sample / tp, 2),
+        "plain_sample": plain_sample,
+        "speedup_vs_plain": round((512 / dt) / (plain_sample / tp), 1),
+        "max_abs_diff_vs_plain": float(np.max(np.abs(vals[:plain_sample] - np.array(plain_vals)))),
+    }
+    # values + gradients: 64 3-regular instances, p=8
+    inst = regular_instances(64, 16, 24, 8, seed=12)
+    graphs = [g for g, _, _ in inst]
+    beta = np.array([b for _, b, _ in inst])
+    gamma = np.array([g for _, _, g in inst])
+    torch.cuda.reset_peak_memory_stats()
+    base = torch.cuda.memory_allocated()
+    batch = E.QAOABatch(graphs, 8, mode=E.MODE_BACKWARD)
+    batch.set_angles(beta, gamma)
+    batch.capture()
+    dt = timed(batch.replay, steps)
+    batch.check()
+    peak = torch.cuda.max_memory_allocated() - base
+    res = batch.out.view(64, -1).cpu().numpy()
+    errs, tps = [], 0.0
+    for i in range(grad_sample):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        v, g = Q.plain_value_and_grad(graphs[i], beta[i], gamma[i])
+        torch.cuda.synchronize()
+        tps += time.perf_counter() - t0
+        errs.append(float(np.max(np.abs(g.cpu().numpy() - res[i, 1:]))))
+        errs.append(abs(v.item() - res[i, 0]))
+    stored = sum((len(g) + 8 * (g.number_of_edges() + len(g))) * 16 * 2 ** len(g) for g in graphs)
+    out["value_and_grad"] = {
+        "workload": "64 random 3-regular graphs n=16..24 (even), p=8, complex128",
+        "batched_value_grad_per_s": round(64 / dt, 2), "batched_ms": round(dt * 1e3, 3),
+        "plain_value_grad_per_s": round(grad_sample / tps, 3), "plain_sample": grad_sample,
+        "speedup_vs_plain": round((64 / dt) / (grad_sample / tps), 1),
+        "max_abs_diff_vs_plain_adjoint": max(errs),
+        "engine_peak_bytes": int(peak),
+        "stored_intermediates_bytes (one state per gate, as autograd keeps)": int(stored),
+    }
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -61,6 +141,7 @@ def main():
     ap.add_argument("--executor", choices=["auto", "fused", "unfused"], default="auto")
     ap.add_argument("--cpu-gates", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-qaoa", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -129,8 +210,24 @@ def main():
         "gates_per_s_pct_of_hbm_roofline": round(100.0 * value * bytes_per_pass / (HBM_PEAK_GBS * 1e9), 2),
         "roofline": roof,
     }
-    if not args.no_cpu and args.cpu_gates > 0:
+    if plan is not None:
+        out["config"].update(plan.stats())
+    unf = fusion.KernelTimer()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fusion.apply_gates_inplace(gates, qubits, work, unf)
+    torch.cuda.synchronize()
+    dtu = time.perf_counter() - t0
+    un_name, un_ms, _ = unf.dominant()
+    out["unfused"] = {"gates_per_s": round(len(gates) / dtu, 2),
+                      "dominant_kernel": un_name, "avg_ms": round(un_ms, 4),
+                      "achieved_GBs": round(bytes_per_pass / (un_ms * 1e-3) / 1e9, 1)}
+    if not args.no_qaoa:
         del state
+        out["qaoa"] = qaoa_section(max(2, args.steps))
+    if not args.no_cpu and args.cpu_gates > 0:
+        del work
+        torch.cuda.empty_cache()
         out["cpu_baseline"] = cpu_baseline(desc, n, args.cpu_gates)
     print(json.dumps(out), flush=True)
 

@@ -17,8 +17,9 @@ from . import config
 _lock = threading.Lock()
 _cache = None
 
-# (tile bits, low bits, block, fuse width)
-DEFAULTS = {'tile_bits': 13, 'low_bits': 3, 'block': 256, 'fuse_width': 4}
+# measured best on MI355X for the 28-qubit random-circuit workload
+DEFAULTS = {'tile_bits': 12, 'low_bits': 3, 'block': 256, 'fuse_width': 1, 'mode': 'reg',
+            'persistent': True}
 
 
 def _load():
