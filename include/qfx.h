@@ -268,6 +268,31 @@ int qfx_qaoa_run(void* stream, const qfx_qaoa_args* args);
 /* After the stream completes: *failed = 1 if a grid barrier timed out. */
 int qfx_qaoa_status(const void* sync, int* failed);
 
+/* ---- MLP angle proposer kernels (fp32) ----------------------------------
+ * No counterpart in the reference (which has no learned component): the
+ * proposer warm-starts the optimizer of examples/qaoa_maxcut.py:37-87. */
+
+/* C[m][n] = sum_k A[m*sam + k*sak] * B[k*sbk + n*sbn] (+ bias[n]) (relu);
+ * C is row-major [M][N].  Fixed k order per element: reproducible. */
+int qfx_mlp_gemm(void* stream, void* C, const void* A, const void* B, int M,
+                 int N, int K, int64_t sam, int64_t sak, int64_t sbk,
+                 int64_t sbn, const void* bias /*[N] or NULL*/, int relu);
+/* dz = dy * (y > 0) */
+int qfx_mlp_relu_bwd(void* stream, void* dz, const void* dy, const void* y,
+                     int64_t len);
+/* out[n] = sum_b x[b][n]  (bias gradient) */
+int qfx_mlp_colsum(void* stream, void* out, const void* x, int B, int N);
+/* Bytes of workspace qfx_mlp_mse needs. */
+int64_t qfx_mlp_mse_workspace(int64_t len);
+/* loss[0] = mean (y - t)^2 ; dy = 2 (y - t) / len */
+int qfx_mlp_mse(void* stream, void* dy, void* loss, const void* y,
+                const void* t, int64_t len, void* workspace);
+/* Adam (Kingma & Ba 2015) with bias correction for step >= 1 and optional
+ * L2 weight decay added to the gradient. */
+int qfx_mlp_adam(void* stream, void* params, const void* grads, void* m,
+                 void* v, int64_t len, float lr, float beta1, float beta2,
+                 float eps, int step, float weight_decay);
+
 #ifdef __cplusplus
 }
 #endif

@@ -67,6 +67,13 @@ SIGNATURES = {
     'qfx_qaoa_check': (_i, [_vp, _i, _vp, _i, _vp, _vp, _i, _i, _i64, _i64, _i64, _i64, _i64, _i]),
     'qfx_qaoa_run': (_i, [_vp, _vp]),
     'qfx_qaoa_status': (_i, [_vp, _vp]),
+    'qfx_mlp_gemm': (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i64, _i64, _i64, _i64, _vp, _i]),
+    'qfx_mlp_relu_bwd': (_i, [_vp, _vp, _vp, _vp, _i64]),
+    'qfx_mlp_colsum': (_i, [_vp, _vp, _vp, _i, _i]),
+    'qfx_mlp_mse_workspace': (_i64, [_i64]),
+    'qfx_mlp_mse': (_i, [_vp, _vp, _vp, _vp, _vp, _i64, _vp]),
+    'qfx_mlp_adam': (_i, [_vp, _vp, _vp, _vp, _vp, _i64, ctypes.c_float, ctypes.c_float,
+                          ctypes.c_float, ctypes.c_float, _i, ctypes.c_float]),
     'qfx_tile_check_ops': (_i, [_i, _i, _vp, _i, _i, _ip, _i]),
     'qfx_tile_pass': (_i, [_vp, _vp, _i, _i, _ip, _vp, _i, _vp, _vp, _i, _i]),
 }
