@@ -206,7 +206,7 @@ def random_graphs(count, n_lo=8, n_hi=20, pe_lo=0.3, pe_hi=0.6, seed=0):
 
 
 def optimise_graphs(graphs, p, beta0, gamma0, steps, lr=0.02, chunk=1024, optimizer='adam',
-                    trace_every=0):
+                    trace_every=0, progress=False):
     """Run the batched optimizer on graphs in chunks; returns final
     (beta, gamma, ratio) and, with trace_every > 0, the ratio after every
     trace_every steps ([N, steps // trace_every + 1], column 0 = start)."""
@@ -253,6 +253,9 @@ def optimise_graphs(graphs, p, beta0, gamma0, steps, lr=0.02, chunk=1024, optimi
         beta[c0:c0 + len(sub)] = b.cpu().numpy()
         gamma[c0:c0 + len(sub)] = g.cpu().numpy()
         ratio[c0:c0 + len(sub)] = r
+        if progress:
+            print('optimised graphs %d..%d of %d, mean ratio %.4f'
+                  % (c0, c0 + len(sub), N, float(np.mean(r))), flush=True)
     return beta, gamma, ratio, trace
 
 
@@ -267,3 +270,75 @@ def expectation_ratios(graphs, p, beta, gamma, chunk=2048):
         val, _ = batch.evaluate(beta[c0:c0 + chunk], gamma[c0:c0 + chunk])
         out[c0:c0 + len(sub)] = val.cpu().numpy() / batch.maxcut
     return out
+
+
+def run_pipeline(p=4, ntrain=20000, neval=1000, n_lo=8, n_hi=20, opt_steps=200, lr=0.02,
+                 epochs=100, batch=512, seed=0, hidden=HIDDEN, verbose=True):
+    """Learned warm start end to end: optimise ntrain graphs from the linear
+    ramp with the batched engine, train the proposer on (features,
+    optimised angles), then on neval fresh graphs compare proposer vs
+    linear-ramp starts: starting ratio <C>/maxcut, final ratio after
+    opt_steps optimizer steps, and steps to reach 0.95 x the linear-ramp
+    run's final ratio (maxcut = exact brute force)."""
+    import time
+    log = (lambda *a: print(*a, flush=True)) if verbose else (lambda *a: None)
+    res = {'p': p, 'ntrain': ntrain, 'neval': neval, 'n_range': [n_lo, n_hi],
+           'optimizer': 'adam lr=%g, %d steps' % (lr, opt_steps)}
+    t0 = time.perf_counter()
+    train = random_graphs(ntrain, n_lo, n_hi, seed=seed)
+    rb, rg = linear_ramp(p)
+    bopt, gopt, ratio, _ = optimise_graphs(train, p, np.tile(rb, (ntrain, 1)),
+                                           np.tile(rg, (ntrain, 1)), opt_steps, lr=lr,
+                                           progress=verbose)
+    res['data_seconds'] = round(time.perf_counter() - t0, 2)
+    res['train_mean_optimised_ratio'] = float(np.mean(ratio))
+    log('dataset', res['data_seconds'], 's, mean ratio', res['train_mean_optimised_ratio'])
+    X = np.stack([graph_features(g) for g in train]).astype(np.float32)
+    T = np.concatenate([bopt, gopt], axis=1).astype(np.float32)
+    t0 = time.perf_counter()
+    prop = AngleProposer(p=p, hidden=hidden, seed=seed + 1)
+    hist = prop.fit(X, T, epochs=epochs, batch=batch, lr=1e-3, seed=seed, verbose=verbose)
+    res['train_seconds'] = round(time.perf_counter() - t0, 2)
+    res['train_loss_first_last'] = [hist[0], hist[-1]]
+    log('training', res['train_seconds'], 's, loss', hist[0], '->', hist[-1])
+    test = random_graphs(neval, n_lo, n_hi, seed=seed + 1000)
+    pb, pg = prop.propose(test)
+    starts = {'proposer': (pb, pg),
+              'linear_ramp': (np.tile(rb, (neval, 1)), np.tile(rg, (neval, 1)))}
+    traces = {}
+    for name, (b0, g0) in starts.items():
+        _, _, fin, tr = optimise_graphs(test, p, b0, g0, opt_steps, lr=lr, trace_every=1,
+                                        progress=verbose)
+        traces[name] = tr
+        res[name] = {'start_ratio_mean': float(np.mean(tr[:, 0])),
+                     'final_ratio_mean': float(np.mean(fin))}
+    target = 0.95 * traces['linear_ramp'][:, -1]
+    for name, tr in traces.items():
+        hit = tr >= target[:, None]
+        steps = np.where(hit.any(axis=1), hit.argmax(axis=1), tr.shape[1])
+        res[name]['steps_to_0.95_of_ramp_optimum_mean'] = float(np.mean(steps))
+        res[name]['steps_to_0.95_of_ramp_optimum_median'] = float(np.median(steps))
+        res[name]['reached_fraction'] = float(np.mean(hit.any(axis=1)))
+    res['proposer_wins_start_ratio_fraction'] = float(np.mean(traces['proposer'][:, 0] >
+                                                              traces['linear_ramp'][:, 0]))
+    log(res)
+    return res, prop
+
+
+if __name__ == '__main__':
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(description='train and evaluate the QAOA angle proposer')
+    ap.add_argument('--train', type=int, default=20000)
+    ap.add_argument('--eval', type=int, default=1000)
+    ap.add_argument('--steps', type=int, default=200)
+    ap.add_argument('--epochs', type=int, default=100)
+    ap.add_argument('--out', default='')
+    ap.add_argument('--model', default='')
+    a = ap.parse_args()
+    r, model = run_pipeline(ntrain=a.train, neval=a.eval, opt_steps=a.steps, epochs=a.epochs)
+    if a.out:
+        with open(a.out, 'w') as f:
+            json.dump(r, f, indent=1)
+    if a.model:
+        model.save(a.model)
