@@ -138,8 +138,11 @@ int qfx_plus_state(void* stream, void* psi, int n);
 
 typedef struct {
   int32_t kind;       /* 0 dense (coef: 4^k row-major), 1 sparse (coef: 2^k
-                         phases; srcs: 2^k source rows), 2 register phase */
-  int32_t k;          /* dense/sparse: gate width, 1..5 */
+                         phases; srcs: 2^k source rows), 2 register phase,
+                         3 dense block on the matrix cores (coef: 2*4^k
+                         slots holding the packed real form, see below) */
+  int32_t k;          /* dense/sparse: gate width, 1..5; matrix-core block:
+                         3..5 */
   int32_t coef;       /* dense/sparse: first coefficient (complex128 index);
                          register phase: number of sub-ops */
   int32_t src;        /* sparse: first source-row index into srcs;
@@ -153,28 +156,41 @@ typedef struct {
 
 /* A register phase keeps, per lane, the 16 amplitudes spanned by its four
  * register bits in registers and applies its sub-ops there.  Sub-op records
- * are 4 ints {type, position code, coef index, mask}: type 1/2/3 = dense
- * 1/2/3-qubit matrix on register bits given by the code (1q: bit a;
- * 2q: (a<b) pairs in lexicographic order, gate index (bit b, bit a);
- * 3q: (a<b<c) triples in lexicographic order, gate index (c, b, a)); type 4 =
- * 16 diagonal phases applied where `mask` has a bit. */
+ * are 4 ints {type, position code, coef index, mask}:
+ *   1  2x2 matrix (4 coefs) on register bit `code`;
+ *   2  4x4 matrix (16 coefs) on register bits (a<b) = pair `code` in
+ *      lexicographic order, gate index (bit b, bit a);
+ *   4  16 phases, one per register slot, applied where `mask` has a bit;
+ *   5  X on register bit code & 3 where every bit of the control mask
+ *      code >> 2 is set (X, CNOT, CCNOT as register exchanges);
+ *   6  exchange of register bits (a<b) = pair code & 7 where every bit of
+ *      the control mask code >> 3 is set (SWAP, CSWAP).
+ *
+ * A matrix-core block (kind 3) applies a dense 2^k x 2^k complex matrix M
+ * with FP64 MFMA.  Its coefficients are the real form
+ * R = [[Re M, -Im M], [Im M, Re M]] (2^(k+1) square) packed in operand order:
+ * double P[(rb * NK + ks) * 64 + l] = R[16 rb + l % 16][4 ks + l / 16] for
+ * row block rb < 2^(k+1) / 16, k-step ks < NK = 2^(k+1) / 4 and lane l < 64,
+ * i.e. 4^(k+1) doubles = 2 * 4^k complex128 slots.  It needs
+ * tile_bits >= k + 4. */
 #define QFX_TILE_WIDE 1   /* flags: the pass holds 4-5 qubit dense/sparse ops */
 #define QFX_TILE_PERSISTENT 2  /* flags: persistent workgroups that prefetch
                                   the next tile during the current one (used
                                   when 2^tile_bits / block <= 16) */
+#define QFX_TILE_MFMA 4   /* flags: the pass holds matrix-core blocks */
 
-/* Host-side validation of an op list against tile width T, the block size it
- * will run with (ops on 4-5 qubits need block <= 256), ncoef coefficients
+/* Host-side validation of an op list against tile width T, the block size
+ * and flags it will run with (ops on 4-5 qubits need QFX_TILE_WIDE and
+ * block <= 256; matrix-core blocks need QFX_TILE_MFMA), ncoef coefficients
  * and nsrcs source rows (host arrays).  The device entry point trusts its op
  * list; the Python planner checks every plan with this before upload. */
-int qfx_tile_check_ops(int T, int block, const qfx_tile_op* ops /*host*/,
+int qfx_tile_check_ops(int T, int block, int flags, const qfx_tile_op* ops /*host*/,
                        int nops, int ncoef, const int* srcs /*host*/,
                        int nsrcs);
 
 /* One pass: 6 <= tile_bits <= 13, tile_bits <= n, block in {64, 256, 512,
- * 1024} with 2^tile_bits >= block; flags QFX_TILE_WIDE (block <= 256) when
- * the op list holds 4-5 qubit dense/sparse ops.  ops/coef/srcs in device
- * memory. */
+ * 1024} with 2^tile_bits >= block; flags as accepted by qfx_tile_check_ops
+ * for this op list.  ops/coef/srcs in device memory. */
 int qfx_tile_pass(void* stream, void* state, int n, int tile_bits,
                   const int* tile_pos /*host*/, const void* ops, int nops,
                   const void* coef, const void* srcs, int block, int flags);

@@ -24,6 +24,10 @@ COMMON = [
     "--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17",
     # packed FP32 ops disabled on purpose (some forms miscompute here)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # MFMA accumulators in ordinary VGPRs: the tile kernels' other paths
+    # already use most of the VGPR budget, and AGPRs would be allocated on
+    # top of it (halving occupancy)
+    "-mllvm", "-amdgpu-mfma-vgpr-form=1",
     "-Wall", "-Wno-unused-function", "-Wno-unused-result",
 ]
 

@@ -17,12 +17,13 @@
 
 namespace {
 
-enum OpKind : int { OP_DENSE = 0, OP_SPARSE = 1, OP_RPHASE = 2 };
+enum OpKind : int { OP_DENSE = 0, OP_SPARSE = 1, OP_RPHASE = 2, OP_MFMA = 3 };
 
 constexpr int kLoadBatch = 16;  // global loads in flight per lane
 constexpr int kMaxTileBits = 13;
 constexpr int kWideOpMaxBlock = 256;  // largest block running 4/5-qubit ops
 constexpr int kRphaseMaxBlock = 512;  // largest block running register phases
+constexpr int kMfmaMaxBlock = 512;    // largest block running matrix-core blocks
 
 struct TilePos {
   int s[QFX_MAX_QUBITS];  // tile bit positions, ascending
@@ -299,13 +300,96 @@ __device__ __forceinline__ void op_rphase(v2d* lds, int T, const qfx_tile_op& op
   }
 }
 
-template <int BS, bool WIDE>
+// ---- matrix-core blocks ---------------------------------------------------
+// A dense k-qubit block Y = M X over all 2^(T-k) amplitude groups of a tile,
+// computed as the real product [Re Y; Im Y] = R [Re X; Im X] with
+// R = [[Re M, -Im M], [Im M, Re M]] on v_mfma_f64_16x16x4f64: A = a 16x4
+// block of R, B = a 4x16 block of the real-form amplitudes (16 groups as
+// columns), D = a 16x16 block of the result.  Operand layout of that
+// instruction: lane l holds A[l%16][l/16], B[l/16][l%16] and, in register j
+// of D, D[4j + l/16][l%16].  Lane l = (q = l/16, column c = l%16) therefore
+// reads amplitudes 4s + q (s < 2^k/4) of group c, and the result rows it
+// receives are the same amplitudes: each lane overwrites only what it read,
+// and a column block belongs to one wave, so the op runs in place.
+// R is packed on the host in operand order (P[(rb*NK + ks)*64 + l]).
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+template <int K, int BS>
+__device__ __forceinline__ void op_mfma(v2d* lds, int T, const qfx_tile_op& op,
+                                        const v2d* __restrict__ coef) {
+  constexpr int D = 1 << K;
+  constexpr int NA = D / 4;       // amplitudes per lane and group
+  constexpr int NK = 2 * D / 4;   // k-steps over the real form's columns
+  constexpr int NR = 2 * D / 16;  // 16-row blocks of the real form
+  const unsigned lane = threadIdx.x & 63u;
+  const unsigned q = lane >> 4, c = lane & 15u;
+  const double* __restrict__ P = reinterpret_cast<const double*>(coef + op.coef) + lane;
+  unsigned soff[NA];
+#pragma unroll
+  for (int s = 0; s < NA; ++s) soff[s] = sw(loc_off<K>(op.t, 4u * s + q));
+  const int ncb = 1 << (T - K - 4);   // column blocks of 16 groups
+  for (int cb = threadIdx.x >> 6; cb < ncb; cb += BS / 64) {
+    unsigned l = (unsigned)cb * 16u + c;
+#pragma unroll
+    for (int j = 0; j < K; ++j) l = ins0(l, op.sorted[j]);
+    const unsigned sl = sw(l);
+    v2d x[NA];
+#pragma unroll
+    for (int s = 0; s < NA; ++s) x[s] = lds[sl ^ soff[s]];
+    // real-form row 4ks + q of the input: Re x[4ks + q] for ks < NA, else Im
+    auto bval = [&](int ks) { return ks < NA ? x[ks].x : x[ks - NA].y; };
+    if constexpr (K == 3) {
+      // one row block: register j is row 4j + q = Re y[4j + q] (j < 2) or
+      // Im y[4(j - 2) + q]
+      v4d acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int ks = 0; ks < NK; ++ks)
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(P[ks * 64], bval(ks), acc, 0, 0, 0);
+      lds[sl ^ soff[0]] = v2d{acc[0], acc[2]};
+      lds[sl ^ soff[1]] = v2d{acc[1], acc[3]};
+    } else {
+      // row blocks rp (real parts) and rp + NR/2 (imaginary parts) hold
+      // amplitudes 4(4rp + j) + q in register j.  The 5-qubit block walks rp
+      // at run time so its 4^6/64 packed-matrix values are re-read per
+      // column block (L1 hits) instead of living in registers.
+      constexpr int NP = NR / 2;
+#pragma nounroll
+      for (int rp = 0; rp < NP; ++rp) {
+        v4d re = {0.0, 0.0, 0.0, 0.0}, im = {0.0, 0.0, 0.0, 0.0};
+        const double* Pr = P + rp * NK * 64;
+        const double* Pi = P + (rp + NP) * NK * 64;
+#pragma unroll
+        for (int ks = 0; ks < NK; ++ks) {
+          const double b = bval(ks);
+          re = __builtin_amdgcn_mfma_f64_16x16x4f64(Pr[ks * 64], b, re, 0, 0, 0);
+          im = __builtin_amdgcn_mfma_f64_16x16x4f64(Pi[ks * 64], b, im, 0, 0, 0);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const unsigned off = NP == 1 ? soff[j & (NA - 1)]
+                                       : sw(loc_off<K>(op.t, 4u * (4u * rp + j) + q));
+          lds[sl ^ off] = v2d{re[j], im[j]};
+        }
+      }
+    }
+  }
+}
+
+template <int BS, bool WIDE, bool MF>
 __device__ __forceinline__ void apply_op(v2d* lds, int T, const qfx_tile_op& op,
                                          const v2d* __restrict__ coef,
                                          const int* __restrict__ srcs) {
   if (op.kind == OP_RPHASE) {
     // 16 register amplitudes per lane exceed what 1024-lane blocks leave
     if constexpr (BS <= kRphaseMaxBlock) op_rphase<BS>(lds, T, op, coef, srcs);
+  } else if (op.kind == OP_MFMA) {
+    if constexpr (MF && BS <= kMfmaMaxBlock) {
+      switch (op.k) {
+        case 3: op_mfma<3, BS>(lds, T, op, coef); break;
+        case 4: op_mfma<4, BS>(lds, T, op, coef); break;
+        case 5: op_mfma<5, BS>(lds, T, op, coef); break;
+      }
+    }
   } else if (op.kind == OP_DENSE) {
     switch (op.k) {
       case 1: op_dense<1, BS>(lds, T, op, coef); break;
@@ -346,7 +430,7 @@ __device__ __forceinline__ uint64_t high_off(int kk, int T, const TilePos& pos) 
   return o;
 }
 
-template <int BS, int LB, bool WIDE>
+template <int BS, int LB, bool WIDE, bool MF>
 __global__ __launch_bounds__(BS) void tile_pass_kernel(v2d* state, int T, TilePos pos,
                                                       const qfx_tile_op* __restrict__ ops,
                                                       int nops, const v2d* __restrict__ coef,
@@ -385,7 +469,7 @@ __global__ __launch_bounds__(BS) void tile_pass_kernel(v2d* state, int T, TilePo
   }
   __syncthreads();
   for (int i = 0; i < nops; ++i) {
-    apply_op<BS, WIDE>(lds, T, ops[i], coef, srcs);
+    apply_op<BS, WIDE, MF>(lds, T, ops[i], coef, srcs);
     __syncthreads();
   }
   v2d* dst = state + base + toff;
@@ -412,7 +496,7 @@ __global__ __launch_bounds__(BS) void tile_pass_kernel(v2d* state, int T, TilePo
 // 2^T / BS <= kPrefetch amplitudes per lane.
 constexpr int kPrefetch = 16;
 
-template <int BS, int LB, bool WIDE>
+template <int BS, int LB, bool WIDE, bool MF>
 __global__ __launch_bounds__(BS) void tile_pipe_kernel(v2d* state, int T, TilePos pos,
                                                       const qfx_tile_op* __restrict__ ops,
                                                       int nops, const v2d* __restrict__ coef,
@@ -461,7 +545,7 @@ __global__ __launch_bounds__(BS) void tile_pipe_kernel(v2d* state, int T, TilePo
     }
     __syncthreads();
     for (int i = 0; i < nops; ++i) {
-      apply_op<BS, WIDE>(lds, T, ops[i], coef, srcs);
+      apply_op<BS, WIDE, MF>(lds, T, ops[i], coef, srcs);
       __syncthreads();
     }
     v2d* dst = state + tile_base(tile);
@@ -472,7 +556,7 @@ __global__ __launch_bounds__(BS) void tile_pipe_kernel(v2d* state, int T, TilePo
   }
 }
 
-template <int BS, int LB, bool WIDE>
+template <int BS, int LB, bool WIDE, bool MF>
 int launch_pass(hipStream_t s, v2d* state, int n, int T, const TilePos& pos,
                 const qfx_tile_op* ops, int nops, const v2d* coef, const int* srcs,
                 bool persistent) {
@@ -480,12 +564,12 @@ int launch_pass(hipStream_t s, v2d* state, int n, int T, const TilePos& pos,
   static bool attr_set = false;
   static int occ[kMaxTileBits + 1];
   if (!attr_set) {
-    int rc = qfx::check_hip(hipFuncSetAttribute((const void*)tile_pass_kernel<BS, LB, WIDE>,
+    int rc = qfx::check_hip(hipFuncSetAttribute((const void*)tile_pass_kernel<BS, LB, WIDE, MF>,
                                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                                 160 * 1024),
                             "hipFuncSetAttribute");
     if (rc) return rc;
-    rc = qfx::check_hip(hipFuncSetAttribute((const void*)tile_pipe_kernel<BS, LB, WIDE>,
+    rc = qfx::check_hip(hipFuncSetAttribute((const void*)tile_pipe_kernel<BS, LB, WIDE, MF>,
                                             hipFuncAttributeMaxDynamicSharedMemorySize,
                                             160 * 1024),
                         "hipFuncSetAttribute");
@@ -498,7 +582,7 @@ int launch_pass(hipStream_t s, v2d* state, int n, int T, const TilePos& pos,
     if (occ[T] == 0) {
       int per_cu = 0, dev = 0, cus = 0;
       int rc = qfx::check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                                  &per_cu, (const void*)tile_pipe_kernel<BS, LB, WIDE>, BS, lds),
+                                  &per_cu, (const void*)tile_pipe_kernel<BS, LB, WIDE, MF>, BS, lds),
                               "hipOccupancyMaxActiveBlocksPerMultiprocessor");
       if (rc) return rc;
       rc = qfx::check_hip(hipGetDevice(&dev), "hipGetDevice");
@@ -509,11 +593,11 @@ int launch_pass(hipStream_t s, v2d* state, int n, int T, const TilePos& pos,
       occ[T] = (per_cu > 0 ? per_cu : 1) * cus;
     }
     const uint64_t grid = tiles < (uint64_t)occ[T] ? tiles : (uint64_t)occ[T];
-    hipLaunchKernelGGL((tile_pipe_kernel<BS, LB, WIDE>), dim3((unsigned)grid), dim3(BS), lds, s,
+    hipLaunchKernelGGL((tile_pipe_kernel<BS, LB, WIDE, MF>), dim3((unsigned)grid), dim3(BS), lds, s,
                        state, T, pos, ops, nops, coef, srcs, (uint32_t)tiles);
     return qfx::check_launch("tile_pipe_kernel");
   }
-  hipLaunchKernelGGL((tile_pass_kernel<BS, LB, WIDE>), dim3((unsigned)tiles), dim3(BS), lds, s, state, T,
+  hipLaunchKernelGGL((tile_pass_kernel<BS, LB, WIDE, MF>), dim3((unsigned)tiles), dim3(BS), lds, s, state, T,
                      pos, ops, nops, coef, srcs);
   return qfx::check_launch("tile_pass_kernel");
 }
@@ -540,11 +624,37 @@ int check_pass(int n, int T, const int* tile_pos, int nops, int block, const cha
 
 extern "C" {
 
-int qfx_tile_check_ops(int T, int block, const qfx_tile_op* ops, int nops, int ncoef,
-                       const int* srcs, int nsrcs) {
-  const int kmax = block <= kWideOpMaxBlock ? 5 : 3;
+int qfx_tile_check_ops(int T, int block, int flags, const qfx_tile_op* ops, int nops,
+                       int ncoef, const int* srcs, int nsrcs) {
+  if (flags & ~(QFX_TILE_WIDE | QFX_TILE_PERSISTENT | QFX_TILE_MFMA))
+    return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: unknown flags");
+  if ((flags & QFX_TILE_WIDE) && block > kWideOpMaxBlock)
+    return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: 4-5 qubit ops need block <= 256");
+  if ((flags & QFX_TILE_MFMA) && block > kMfmaMaxBlock)
+    return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: matrix-core blocks need block <= 512");
+  if (!ops && nops > 0) return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: null ops");
+  const int kmax = (flags & QFX_TILE_WIDE) ? 5 : 3;
   for (int i = 0; i < nops; ++i) {
     const qfx_tile_op& o = ops[i];
+    if (o.kind == OP_MFMA) {
+      if (!(flags & QFX_TILE_MFMA))
+        return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: matrix-core block without QFX_TILE_MFMA");
+      if (o.k < 3 || o.k > 5 || T < o.k + 4)
+        return qfx::set_error(QFX_EINVAL,
+                              "qfx_tile_check_ops: matrix-core blocks need 3 <= k <= 5 and T >= k + 4");
+      if (o.coef < 0 || o.coef + 2 * (1 << (2 * o.k)) > ncoef)
+        return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: coefficients out of range");
+      for (int j = 0; j < o.k; ++j) {
+        if (o.t[j] < 0 || o.t[j] >= T)
+          return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: target outside the tile");
+        if (j > 0 && o.sorted[j] <= o.sorted[j - 1])
+          return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: sorted targets not ascending");
+        bool found = false;
+        for (int i2 = 0; i2 < o.k; ++i2) found |= o.sorted[i2] == o.t[j];
+        if (!found) return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: sorted != targets");
+      }
+      continue;
+    }
     if (o.kind == OP_RPHASE) {
       if (T < kRegBits) return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: tile too small");
       if (block > kRphaseMaxBlock)
@@ -582,7 +692,8 @@ int qfx_tile_check_ops(int T, int block, const qfx_tile_op* ops, int nops, int n
       continue;
     }
     if ((o.kind != OP_DENSE && o.kind != OP_SPARSE) || o.k < 1 || o.k > kmax || o.k > T)
-      return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: bad op kind or width for block");
+      return qfx::set_error(QFX_EINVAL,
+                            "qfx_tile_check_ops: bad op kind, or 4-5 qubit op without QFX_TILE_WIDE");
     const int D = 1 << o.k;
     const int need = o.kind == OP_DENSE ? D * D : D;
     if (o.coef < 0 || o.coef + need > ncoef)
@@ -618,8 +729,13 @@ int qfx_tile_pass(void* stream, void* state, int n, int tile_bits, const int* ti
   if (rc) return rc;
   const bool wide = (flags & QFX_TILE_WIDE) != 0;
   const bool pers = (flags & QFX_TILE_PERSISTENT) != 0;
+  const bool mf = (flags & QFX_TILE_MFMA) != 0;
+  if (flags & ~(QFX_TILE_WIDE | QFX_TILE_PERSISTENT | QFX_TILE_MFMA))
+    return qfx::set_error(QFX_EINVAL, "qfx_tile_pass: unknown flags");
   if (wide && block > kWideOpMaxBlock)
     return qfx::set_error(QFX_EINVAL, "qfx_tile_pass: 4-5 qubit ops need block <= 256");
+  if (mf && block > kMfmaMaxBlock)
+    return qfx::set_error(QFX_EINVAL, "qfx_tile_pass: matrix-core blocks need block <= 512");
   if (!state || (nops > 0 && (!ops || !coef)))
     return qfx::set_error(QFX_EINVAL, "qfx_tile_pass: null pointer");
   TilePos pos;
@@ -630,16 +746,18 @@ int qfx_tile_pass(void* stream, void* state, int n, int tile_bits, const int* ti
   auto* o = static_cast<const qfx_tile_op*>(ops);
   auto* c = static_cast<const v2d*>(coef);
   auto* sr = static_cast<const int*>(srcs);
+#define QFX_PASS(BS, LB, W, M) launch_pass<BS, LB, W, M>(s, st, n, tile_bits, pos, o, nops, c, sr, pers)
   switch (block) {
     case 64:
-      return wide ? launch_pass<64, 6, true>(s, st, n, tile_bits, pos, o, nops, c, sr, pers)
-                  : launch_pass<64, 6, false>(s, st, n, tile_bits, pos, o, nops, c, sr, pers);
+      if (mf) return wide ? QFX_PASS(64, 6, true, true) : QFX_PASS(64, 6, false, true);
+      return wide ? QFX_PASS(64, 6, true, false) : QFX_PASS(64, 6, false, false);
     case 256:
-      return wide ? launch_pass<256, 8, true>(s, st, n, tile_bits, pos, o, nops, c, sr, pers)
-                  : launch_pass<256, 8, false>(s, st, n, tile_bits, pos, o, nops, c, sr, pers);
-    case 512: return launch_pass<512, 9, false>(s, st, n, tile_bits, pos, o, nops, c, sr, pers);
-    default: return launch_pass<1024, 10, false>(s, st, n, tile_bits, pos, o, nops, c, sr, pers);
+      if (mf) return wide ? QFX_PASS(256, 8, true, true) : QFX_PASS(256, 8, false, true);
+      return wide ? QFX_PASS(256, 8, true, false) : QFX_PASS(256, 8, false, false);
+    case 512: return mf ? QFX_PASS(512, 9, false, true) : QFX_PASS(512, 9, false, false);
+    default: return QFX_PASS(1024, 10, false, false);
   }
+#undef QFX_PASS
 }
 
 }  // extern "C"

@@ -14,10 +14,15 @@ Planning (host, once per circuit):
      touch a qubit of a gate already deferred to a later pass (so it
      commutes with every gate it overtakes); otherwise it is deferred and
      its qubits are blocked for the rest of the pass.
-  2. ``fuse_ops``: within a pass, runs of consecutive gates whose union has
-     at most ``fuse_width`` (3..5) qubits are merged into one dense block
-     when a cost model (LDS sweeps + FP64 FMA work) says it is cheaper; the
-     block's matrix is the product of the gates' matrices embedded in the
+  2. Within a pass, by ``mode``:
+     'reg'   ``plan_phases`` groups gates into register phases of 4 tile
+             bits (one LDS sweep each, gates applied in registers);
+     'mfma'  ``plan_blocks`` merges all gates into dense blocks of 3 to
+             ``fuse_width`` (<= 5) qubits, applied by FP64 MFMA;
+     'lds'   ``fuse_ops`` merges runs of consecutive gates into dense blocks
+             of <= ``fuse_width`` qubits when a cost model (LDS sweeps + FP64
+             FMA work) prefers it; one LDS op each.
+     A block's matrix is the product of its gates' matrices embedded in the
      block (Circuit.asgate semantics, quantumflow/circuits.py:111-118).
   3. The op tables of all passes go to the device once; running the plan is
      one kernel launch per pass.
@@ -32,13 +37,15 @@ from . import _native
 from . import backend as bk
 from .states import State
 
-OP_DENSE, OP_SPARSE, OP_RPHASE = 0, 1, 2
+OP_DENSE, OP_SPARSE, OP_RPHASE, OP_MFMA = 0, 1, 2, 3
 SUB_D1, SUB_D2, SUB_DIAG, SUB_CX, SUB_SWAP = 1, 2, 4, 5, 6
 REG_BITS = 4
 MAX_TILE_BITS = 13
 MIN_TILE_BITS = 6
 TILE_WIDE = 1
 TILE_PERSISTENT = 2
+TILE_MFMA = 4
+MFMA_MIN_BITS = 3
 # position codes of the register-phase kernels (lexicographic)
 _D2_CODES = {p: i for i, p in enumerate([(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)])}
 _PAIRS = [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]
@@ -453,8 +460,80 @@ def _encode_phase(R, members, pos_index, T, coefs, subs):
     o.active = o.phase = 0
     return o
 
-class TilePlan:
-    """A gate list compiled into LDS-tiled passes on the device."""
+def plan_blocks(ops, width):
+    """Group a pass's ops (in order) into dense blocks of <= width qubits with
+    the commuting-overtake rule of plan_passes.  Returns [(bits, [ops])] with
+    bits in first-use order."""
+    pending = list(range(len(ops)))
+    blocks = []
+    while pending:
+        bits = list(ops[pending[0]].bits)
+        blocked = set()
+        taken, rest = [], []
+        for i in pending:
+            b = ops[i].bits
+            if set(b) & blocked:
+                blocked |= set(b)
+                rest.append(i)
+                continue
+            new = [x for x in b if x not in bits]
+            if len(bits) + len(new) <= width:
+                bits += new
+                taken.append(i)
+            else:
+                blocked |= set(b)
+                rest.append(i)
+        blocks.append((bits, [ops[i] for i in taken]))
+        pending = rest
+    return blocks
+
+
+def block_matrix(bits, members):
+    """Dense matrix of the ops applied in order on block qubits ``bits``
+    (block qubit 0 = most significant), Circuit.asgate semantics."""
+    nq = len(bits)
+    m = np.eye(2 ** nq, dtype=np.complex128)
+    for op in members:
+        m = embed(op.dense_matrix(), [bits.index(b) for b in op.bits], nq) @ m
+    return m
+
+
+def pack_mfma(matrix):
+    """Real form R = [[Re M, -Im M], [Im M, Re M]] of a 2^k x 2^k matrix,
+    packed in the operand order of the matrix-core blocks:
+    P[(rb * NK + ks) * 64 + l] = R[16 rb + l % 16][4 ks + l // 16]."""
+    m = np.asarray(matrix, dtype=np.complex128)
+    d2 = 2 * m.shape[0]
+    r = np.block([[m.real, -m.imag], [m.imag, m.real]])
+    nr, nk = d2 // 16, d2 // 4
+    lane = np.arange(64)
+    rows = 16 * np.arange(nr)[:, None, None] + (lane % 16)[None, None, :]
+    cols = 4 * np.arange(nk)[None, :, None] + (lane // 16)[None, None, :]
+    return np.ascontiguousarray(r[rows, cols].reshape(-1))
+
+
+def _encode_mfma(bits, matrix, pos_index, coefs):
+    o = _native.QfxTileOp()
+    k = len(bits)
+    t = [pos_index[b] for b in bits]
+    o.kind = OP_MFMA
+    o.k = k
+    st = sorted(t)
+    for j in range(5):
+        o.t[j] = t[j] if j < k else 0
+        o.sorted[j] = st[j] if j < k else 0
+    o.coef = len(coefs)
+    coefs.extend(pack_mfma(matrix).view(np.complex128))
+    o.src = o.active = o.phase = 0
+    return o
+
+
+class HostPlan:
+    """A gate list compiled into tiled passes on the host: the pass list,
+    the op records, coefficient and source tables, validated by the
+    library's qfx_tile_check_ops.  ``passes`` holds ('tile', (tile bits,
+    first op, op count, block, flags)) and ('gate', (bits, GateKind)) for
+    gates too wide to tile."""
 
     def __init__(self, gates, qubits, tile_bits=MAX_TILE_BITS, low_bits=3, block=256,
                  fuse_width=4, mode='reg', persistent=True):
@@ -467,9 +546,11 @@ class TilePlan:
         from .autotune import block_for
         self.block = block_for(T, block)
         self.mode = mode if T >= REG_BITS + 2 else 'lds'
-        # register phases run in blocks of <= 512 lanes; 4- and 5-qubit ops
-        # only in blocks of <= 256
-        if self.mode == 'reg' and self.block > 512:
+        if mode == 'mfma' and T < max(fuse_width, MFMA_MIN_BITS) + 4:
+            self.mode = 'lds'           # tile too small for 16-group columns
+        # register phases and matrix-core blocks run in blocks of <= 512
+        # lanes; 4- and 5-qubit ops only in blocks of <= 256
+        if self.mode in ('reg', 'mfma') and self.block > 512:
             self.block = 512
         self.fuse_width = fuse_width
         self.persistent = persistent
@@ -479,6 +560,7 @@ class TilePlan:
         self.passes = []           # (kind, payload)
         ops_host, coefs, srcs = [], [], []
         self.n_phases = 0
+        self.n_blocks = 0
         for pos, members in plan_passes(bits_list, n, T, low_bits, tileable):
             if pos is None:
                 g = gates[members[0]]
@@ -488,7 +570,19 @@ class TilePlan:
             ops = [Op(bits_list[i], gates[i].kind) for i in members]
             wide = any(len(op.bits) > 3 for op in ops)
             start = len(ops_host)
-            if self.mode == 'reg' and not wide:
+            if self.mode == 'mfma':
+                width = max(MFMA_MIN_BITS, min(fuse_width, 5))
+                for bits, group in plan_blocks(ops, width):
+                    m = block_matrix(bits, group)
+                    # pad narrow blocks with free tile bits (identity factors)
+                    pad = [b for b in pos if b not in bits][:max(0, MFMA_MIN_BITS - len(bits))]
+                    if pad:
+                        m = np.kron(m, np.eye(2 ** len(pad)))
+                        bits = bits + pad
+                    ops_host.append(_encode_mfma(bits, m, pos_index, coefs))
+                    self.n_blocks += 1
+                block, flags = self.block, TILE_MFMA
+            elif self.mode == 'reg' and not wide:
                 for item in plan_phases(ops):
                     if item[0] == 'phase':
                         ops_host.append(_encode_phase(item[1], item[2], pos_index, T, coefs, srcs))
@@ -510,16 +604,34 @@ class TilePlan:
             self.passes.append(('tile', (pos, start, len(ops_host) - start, block, flags)))
         self.n_passes = len(self.passes)
         self.n_ops = len(ops_host)
-        # validate on the host (each pass with its own block), then upload once
+        self.ops_host, self.coefs, self.srcs = ops_host, coefs, srcs
+        # validate every pass with its own block and flags
         lib = _native.load()
         src_arr = _native.int_array(srcs)
         for kind, payload in self.passes:
             if kind != 'tile':
                 continue
-            _, start, nops, block, _ = payload
+            _, start, nops, block, flags = payload
             arr = (_native.QfxTileOp * max(1, nops))(*ops_host[start:start + nops])
-            _native.check(lib.qfx_tile_check_ops(T, block, arr, nops, len(coefs), src_arr,
-                                                 len(srcs)), 'qfx_tile_check_ops')
+            _native.check(lib.qfx_tile_check_ops(T, block, flags, arr, nops, len(coefs),
+                                                 src_arr, len(srcs)), 'qfx_tile_check_ops')
+
+    def stats(self):
+        ng = len(self.gates)
+        return {'gates': ng, 'passes': self.n_passes, 'ops': self.n_ops,
+                'phases': self.n_phases, 'blocks': self.n_blocks, 'mode': self.mode,
+                'persistent': self.persistent,
+                'tile_bits': self.tile_bits, 'low_bits': self.low_bits,
+                'block': self.block, 'fuse_width': self.fuse_width}
+
+
+class TilePlan(HostPlan):
+    """A HostPlan uploaded to the device once; running it is one kernel
+    launch per pass."""
+
+    def __init__(self, gates, qubits, **cfg):
+        super().__init__(gates, qubits, **cfg)
+        ops_host, coefs, srcs = self.ops_host, self.coefs, self.srcs
         arr = (_native.QfxTileOp * max(1, self.n_ops))(*ops_host)
         raw = bytes(arr)
         dev = bk.DEVICE
@@ -562,13 +674,6 @@ class TilePlan:
         work = bk.as_device(ket.tensor).clone()
         self.run_inplace(work)
         return State(work.view([2] * self.n), self.qubits, ket.memory)
-
-    def stats(self):
-        ng = len(self.gates)
-        return {'gates': ng, 'passes': self.n_passes, 'ops': self.n_ops,
-                'phases': self.n_phases, 'mode': self.mode, 'persistent': self.persistent,
-                'tile_bits': self.tile_bits, 'low_bits': self.low_bits,
-                'block': self.block, 'fuse_width': self.fuse_width}
 
 
 def plan_for(gates, qubits, **kwargs):

@@ -83,13 +83,13 @@ def test_encoded_ops_validate():
             ops.append(fusion._encode_op(op, pidx, coefs, srcs))
     arr = (_native.QfxTileOp * len(ops))(*ops)
     lib = _native.load()
-    assert lib.qfx_tile_check_ops(T, 256, arr, len(ops), len(coefs), _native.int_array(srcs),
+    assert lib.qfx_tile_check_ops(T, 256, 0, arr, len(ops), len(coefs), _native.int_array(srcs),
                                   len(srcs)) == 0
     # a target outside the tile is rejected
     bad = (_native.QfxTileOp * 1)(ops[0])
     bad[0].t[0] = T
     bad[0].sorted[0] = T
-    assert lib.qfx_tile_check_ops(T, 256, bad, 1, len(coefs), _native.int_array(srcs),
+    assert lib.qfx_tile_check_ops(T, 256, 0, bad, 1, len(coefs), _native.int_array(srcs),
                                   len(srcs)) == _native.QFX_EINVAL
 
 
@@ -100,6 +100,21 @@ def test_gatekind_classification():
     assert bk.classify(O.gate_matrix('RZ', 0.3)).kind == 'diagonal'
     assert bk.classify(O.gate_matrix('H')).kind == 'dense'
     assert bk.classify(O.gate_matrix('CCNOT')).kind == 'monomial'
+
+
+def unpack_mfma(slots, k):
+    """Complex matrix from the packed real form, by the header's formula."""
+    p = np.asarray(slots, dtype=np.complex128).view(np.float64)
+    d2 = 2 << k
+    nr, nk = d2 // 16, d2 // 4
+    r = np.zeros((d2, d2))
+    for rb in range(nr):
+        for ks in range(nk):
+            for lane in range(64):
+                r[16 * rb + lane % 16, 4 * ks + lane // 16] = p[(rb * nk + ks) * 64 + lane]
+    d = 1 << k
+    assert np.array_equal(r[:d, :d], r[d:, d:]) and np.array_equal(r[:d, d:], -r[d:, :d])
+    return r[:d, :d] + 1j * r[d:, :d]
 
 
 def _emulate_ops(ket, n, pos, ops, coefs, srcs):
@@ -146,6 +161,8 @@ def _emulate_ops(ket, n, pos, ops, coefs, srcs):
             axes = [n - 1 - pos[o.t[j]] for j in range(k)]
             if o.kind == fusion.OP_DENSE:
                 m = coefs[o.coef:o.coef + 4 ** k].reshape(2 ** k, 2 ** k)
+            elif o.kind == fusion.OP_MFMA:
+                m = unpack_mfma(coefs[o.coef:o.coef + 2 * 4 ** k], k)
             else:
                 m = np.zeros((2 ** k, 2 ** k), dtype=complex)
                 for r in range(2 ** k):
@@ -174,7 +191,7 @@ def test_register_phase_encoding_replays(T, seed):
                 ops.append(fusion._encode_op(item[1], pidx, coefs, srcs))
         nphase += len(ops)
         arr = (_native.QfxTileOp * len(ops))(*ops)
-        assert _native.load().qfx_tile_check_ops(T, 256, arr, len(ops), len(coefs),
+        assert _native.load().qfx_tile_check_ops(T, 256, 0, arr, len(ops), len(coefs),
                                                  _native.int_array(srcs), len(srcs)) == 0
         ket = _emulate_ops(ket, n, pos, ops, coefs, srcs)
     assert nphase < len(gates)
@@ -217,7 +234,103 @@ def test_register_phases_permutation_heavy(seed):
             else:
                 ops.append(fusion._encode_op(item[1], pidx, coefs, srcs))
         arr = (_native.QfxTileOp * len(ops))(*ops)
-        assert _native.load().qfx_tile_check_ops(T, 256, arr, len(ops), len(coefs),
+        assert _native.load().qfx_tile_check_ops(T, 256, 0, arr, len(ops), len(coefs),
                                                  _native.int_array(srcs), len(srcs)) == 0
         ket = _emulate_ops(ket, n, pos, ops, coefs, srcs)
     assert np.max(np.abs(ket - ref)) < 1e-12
+
+
+def mfma_16x16x4(a_lanes, b_lanes, acc):
+    """v_mfma_f64_16x16x4f64 on 64 lanes by its operand layout: lane l holds
+    A[l % 16][l // 16], B[l // 16][l % 16] and D[4 j + l // 16][l % 16] in
+    register j."""
+    A = np.zeros((16, 4))
+    B = np.zeros((4, 16))
+    for l in range(64):
+        A[l % 16, l // 16] = a_lanes[l]
+        B[l // 16, l % 16] = b_lanes[l]
+    D = A @ B
+    out = acc.copy()
+    for l in range(64):
+        for j in range(4):
+            out[l, j] += D[4 * j + l // 16, l % 16]
+    return out
+
+
+@pytest.mark.parametrize("k", [3, 4, 5])
+def test_mfma_block_lane_program(k):
+    """The matrix-core op's lane program (which amplitudes a lane reads, the
+    packed operands it feeds, which result registers it writes back), run
+    on an emulated 16x16x4 instruction, computes M x for 16 groups."""
+    rng = np.random.default_rng(k)
+    d = 1 << k
+    m = rng.normal(size=(d, d)) + 1j * rng.normal(size=(d, d))
+    x = rng.normal(size=(16, d)) + 1j * rng.normal(size=(16, d))    # group, amplitude
+    p = fusion.pack_mfma(m)
+    na, nk, nr = d // 4, 2 * d // 4, 2 * d // 16
+    lanes = np.arange(64)
+    q, c = lanes // 16, lanes % 16
+    xs = [x[c, 4 * s + q] for s in range(na)]                  # x[s] per lane
+    b = [xs[ks].real if ks < na else xs[ks - na].imag for ks in range(nk)]
+    y = np.zeros((16, d), dtype=complex)
+    if k == 3:
+        acc = np.zeros((64, 4))
+        for ks in range(nk):
+            acc = mfma_16x16x4(p[ks * 64:(ks + 1) * 64], b[ks], acc)
+        for s in range(2):
+            y[c, 4 * s + q] = acc[:, s] + 1j * acc[:, s + 2]
+    else:
+        npair = nr // 2
+        for rp in range(npair):
+            re = np.zeros((64, 4))
+            im = np.zeros((64, 4))
+            for ks in range(nk):
+                re = mfma_16x16x4(p[(rp * nk + ks) * 64:][:64], b[ks], re)
+                im = mfma_16x16x4(p[((rp + npair) * nk + ks) * 64:][:64], b[ks], im)
+            for j in range(4):
+                y[c, 4 * (4 * rp + j) + q] = re[:, j] + 1j * im[:, j]
+    assert np.max(np.abs(y - x @ m.T)) < 1e-12
+    assert np.array_equal(unpack_mfma(p.view(np.complex128), k), m)
+
+
+@pytest.mark.parametrize("width,T", [(3, 8), (4, 9), (5, 10)])
+def test_mfma_plan_replays(width, T):
+    """mode='mfma' plans: every block holds 3..width qubits, the encoded
+    blocks validate, and replaying them gives the circuit's state."""
+    n = 11
+    desc = random_mixed_circuit(n, 10, seed=width)
+    gates = build_gates(desc)
+    ref = O.run_circuit(desc, qubits=tuple(range(n)))
+    plan = fusion.HostPlan(gates, tuple(range(n)), tile_bits=T, low_bits=3, block=256,
+                           fuse_width=width, mode='mfma')
+    assert plan.mode == 'mfma' and plan.n_blocks < len(gates)
+    ket = O.zero_state(n)
+    for kind, payload in plan.passes:
+        assert kind == 'tile'
+        pos, start, nops, block, flags = payload
+        assert flags & fusion.TILE_MFMA
+        ops = plan.ops_host[start:start + nops]
+        assert all(o.kind == fusion.OP_MFMA and 3 <= o.k <= width for o in ops)
+        ket = _emulate_ops(ket, n, pos, ops, plan.coefs, plan.srcs)
+    assert np.max(np.abs(ket - ref)) < 1e-12
+
+
+def test_mfma_validator_rules():
+    """Matrix-core blocks need QFX_TILE_MFMA, 3 <= k <= 5, T >= k + 4 and
+    their 2 * 4^k coefficient slots; 4-5 qubit dense ops need QFX_TILE_WIDE."""
+    lib = _native.load()
+    coefs = []
+    pidx = {b: b for b in range(9)}
+    o = fusion._encode_mfma([0, 1, 2, 3], np.eye(16), pidx, coefs)
+    arr = (_native.QfxTileOp * 1)(o)
+    srcs = _native.int_array([0])
+    M = fusion.TILE_MFMA
+    assert lib.qfx_tile_check_ops(9, 256, M, arr, 1, len(coefs), srcs, 1) == 0
+    assert lib.qfx_tile_check_ops(9, 256, 0, arr, 1, len(coefs), srcs, 1) == _native.QFX_EINVAL
+    assert lib.qfx_tile_check_ops(7, 256, M, arr, 1, len(coefs), srcs, 1) == _native.QFX_EINVAL
+    assert lib.qfx_tile_check_ops(9, 256, M, arr, 1, len(coefs) - 1, srcs, 1) == _native.QFX_EINVAL
+    assert lib.qfx_tile_check_ops(9, 1024, M, arr, 1, len(coefs), srcs, 1) == _native.QFX_EINVAL
+    dense = fusion._encode_op(fusion.Op([0, 1, 2, 3], None, np.eye(16)), pidx, coefs, [])
+    arr = (_native.QfxTileOp * 1)(dense)
+    assert lib.qfx_tile_check_ops(9, 256, 0, arr, 1, len(coefs), srcs, 1) == _native.QFX_EINVAL
+    assert lib.qfx_tile_check_ops(9, 256, fusion.TILE_WIDE, arr, 1, len(coefs), srcs, 1) == 0

@@ -130,6 +130,36 @@ def test_mixed_circuit_20q(qf, fused):
     assert np.max(np.abs(got - ref)) <= 1e-12
 
 
+@pytest.fixture(scope="module")
+def mixed20():
+    """The 28-qubit benchmark generator at 20 qubits (depth 40, seed 2024)
+    and the oracle's final state."""
+    from quantumflow_amd.workloads import random_mixed_circuit
+    desc = random_mixed_circuit(20, 40, seed=2024)
+    return desc, O.run_circuit(desc, qubits=tuple(range(20)))
+
+
+@pytest.mark.parametrize("mode,width", [('unfused', 0), ('reg', 1), ('lds', 3), ('mfma', 3),
+                                        ('mfma', 4), ('mfma', 5)])
+def test_benchmark_circuit_20q(qf, mixed20, mode, width):
+    """Amplitude error of every executor on the benchmark circuit at 20
+    qubits against numpy (<= 1e-12)."""
+    from quantumflow_amd.fusion import TilePlan
+    from quantumflow_amd.workloads import build_gates
+    desc, ref = mixed20
+    gates = build_gates(desc)
+    if mode == 'unfused':
+        got = qf.Circuit(gates).run(qf.zero_state(20), fused=False).vec.asarray()
+    else:
+        plan = TilePlan(gates, tuple(range(20)), tile_bits=12, low_bits=3, block=256,
+                        fuse_width=width, mode=mode)
+        got = plan.run(qf.zero_state(20)).vec.asarray()
+    err = float(np.max(np.abs(got - ref)))
+    print("20q benchmark circuit: %d gates, mode %s width %d, max |d amp| %.3g"
+          % (len(gates), mode, width, err), flush=True)
+    assert err <= 1e-12
+
+
 def test_w16_kat(qf):
     # examples/state_prep_w16.py via tests/test_examples.py:29-32
     from tests.test_oracle import w16_gates

@@ -181,7 +181,9 @@ def test_errors_map_to_reference_types(bk):
     (11, 3, 512, 4, 'lds'), (12, 5, 1024, 3, 'lds'), (13, 3, 256, 5, 'lds'),
     (13, 4, 512, 4, 'lds'), (6, 3, 64, 1, 'reg'), (8, 2, 256, 1, 'reg'),
     (11, 3, 256, 1, 'reg'), (12, 3, 256, 1, 'reg'), (12, 4, 512, 1, 'reg'),
-    (13, 3, 512, 1, 'reg')])
+    (13, 3, 512, 1, 'reg'), (9, 3, 256, 3, 'mfma'), (10, 3, 64, 4, 'mfma'),
+    (12, 3, 256, 5, 'mfma'), (12, 3, 512, 4, 'mfma'), (13, 3, 256, 3, 'mfma'),
+    (13, 4, 512, 5, 'mfma')])
 def test_tile_plan_configs(bk, T, L, block, width, mode, persistent):
     """Every tiled-executor configuration reproduces the oracle's state."""
     import quantumflow_amd as qf
@@ -212,8 +214,9 @@ def test_tile_plan_permutations_exact(bk):
     psi = rand_state(n, rng)
     ref = O.run_circuit(desc, qubits=tuple(range(n)), ket=psi)
     from quantumflow_amd.workloads import build_gates
-    for mode in ('lds', 'reg'):
+    for mode, width in (('lds', 1), ('reg', 1), ('mfma', 4)):
         plan = TilePlan(build_gates(desc), tuple(range(n)), tile_bits=10, low_bits=3,
-                        fuse_width=1, mode=mode)
+                        fuse_width=width, mode=mode)
         got = plan.run(qf.State(psi)).vec.asarray()
+        # permutation blocks on the matrix cores multiply by exact 0/1
         assert np.array_equal(got, ref), mode
