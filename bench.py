@@ -6,10 +6,15 @@ CNOT/CZ/ISWAP/SWAP 45%, CCNOT 5%, targets uniform over all qubits), state
 resident in HBM (4 GiB).  One step = the whole circuit applied once, in
 place, to the working state.  Prints one JSON line:
 
-  value      gates/s of the default executor (fused when available)
+  value      gates/s of the default executor (fused when available);
+             gates_per_s_pct_of_hbm_roofline prices every gate at one state
+             read + write (the unfused cost), passes_per_s_pct_of_hbm_roofline
+             every fused pass
   roofline   dominant kernel: algorithmic bytes per launch (32 * 2^28 B: one
              read + one write of the state) / its average duration, timed
              with HIP events on the launching stream inside the timed region
+  fused_mfma_blocks  the circuit fused into dense 3/4/5-qubit blocks on the
+             matrix cores (FP64 MFMA)
   cpu_baseline  the numpy oracle (np.einsum per gate, 1 thread) on a
              bounded prefix of the same circuit, on this host
 
@@ -50,6 +55,40 @@ def cpu_baseline(desc, n, ngates):
             "sample": "first %d gates of the %d-qubit depth-%d circuit, numpy einsum "
                       "per gate (single-threaded), host os.cpu_count()=%d, affinity=%d"
                       % (ngates, n, DEPTH, os.cpu_count(), len(os.sched_getaffinity(0)))}
+
+
+def fused_block_section(gates, qubits, work, widths=(3, 4, 5), reps=2):
+    """The same circuit with every pass fused into dense 3..K-qubit blocks
+    applied on the matrix cores (FP64 MFMA): gates/s, blocks/s, the tile
+    pass's HBM rate and the MFMA rate (real-form flops: 8 * 2^k per
+    amplitude and block)."""
+    import torch
+    from quantumflow_amd import fusion
+    n = len(qubits)
+    out = {}
+    for k in widths:
+        plan = fusion.TilePlan(gates, qubits, tile_bits=12, low_bits=3, block=256,
+                               fuse_width=k, mode="mfma", persistent=True)
+        if plan.mode != "mfma":
+            continue
+        plan.run_inplace(work)
+        prof = fusion.KernelTimer()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            plan.run_inplace(work, prof=prof)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / reps
+        name, avg_ms, _ = prof.dominant()
+        flops = sum(8.0 * 2 ** (n + o.k) for o in plan.ops_host if o.kind == fusion.OP_MFMA)
+        out["k%d" % k] = {
+            "gates_per_s": round(len(gates) / dt, 2), "ms_per_circuit": round(dt * 1e3, 3),
+            "blocks": plan.n_blocks, "passes": plan.n_passes,
+            "blocks_per_s": round(plan.n_blocks / dt, 1),
+            "tile_pass_avg_ms": round(avg_ms, 4),
+            "tile_pass_GBs": round(32.0 * 2 ** n / (avg_ms * 1e-3) / 1e9, 1),
+            "mfma_TFLOPs": round(flops / dt / 1e12, 2), "fp64_peak_TFLOPs": 78.6}
+    return out
 
 
 def qaoa_section(steps, plain_sample=24, grad_sample=4):
@@ -142,6 +181,7 @@ def main():
     ap.add_argument("--cpu-gates", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-qaoa", action="store_true")
+    ap.add_argument("--no-mfma", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -208,6 +248,9 @@ def main():
                    "executor": "fused" if plan is not None else "unfused",
                    "passes_per_step": (plan.n_passes if plan is not None else gates_per_step)},
         "gates_per_s_pct_of_hbm_roofline": round(100.0 * value * bytes_per_pass / (HBM_PEAK_GBS * 1e9), 2),
+        "passes_per_s_pct_of_hbm_roofline": round(
+            100.0 * value / gates_per_step * (plan.n_passes if plan is not None else gates_per_step)
+            * bytes_per_pass / (HBM_PEAK_GBS * 1e9), 2),
         "roofline": roof,
     }
     if plan is not None:
@@ -222,6 +265,8 @@ def main():
     out["unfused"] = {"gates_per_s": round(len(gates) / dtu, 2),
                       "dominant_kernel": un_name, "avg_ms": round(un_ms, 4),
                       "achieved_GBs": round(bytes_per_pass / (un_ms * 1e-3) / 1e9, 1)}
+    if not args.no_mfma:
+        out["fused_mfma_blocks"] = fused_block_section(gates, qubits, work)
     if not args.no_qaoa:
         del state
         out["qaoa"] = qaoa_section(max(2, args.steps))

@@ -61,6 +61,11 @@ __device__ __forceinline__ uint64_t insert_zeros_rt(uint64_t g, const int* sorte
 // HIP's union-based double2 can be demoted to scratch memory).
 typedef double v2d __attribute__((ext_vector_type(2)));
 
+// Streaming loads/stores of amplitudes that are touched once per pass:
+// nontemporal, so they do not displace reusable lines in the caches.
+__device__ __forceinline__ v2d ld_stream(const v2d* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ void st_stream(v2d* p, v2d v) { __builtin_nontemporal_store(v, p); }
+
 __device__ __forceinline__ void cmac(v2d& acc, const v2d a, const v2d b) {
   acc.x = fma(a.x, b.x, acc.x);
   acc.x = fma(-a.y, b.y, acc.x);

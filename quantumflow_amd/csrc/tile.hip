@@ -143,16 +143,21 @@ __device__ __forceinline__ void op_sparse(v2d* lds, int T, const qfx_tile_op& op
 // gates to them in registers, so the run costs one LDS sweep.
 // Sub-ops (4 ints each in `subs`): {type, code, coef index, mask}.
 //   SUB_D1  2x2 matrix on register bit `code`
+//   SUB_D1R 2x2 real matrix (real parts of the 4 coefficients; RY, H)
+//   SUB_D1X 2x2 matrix with real diagonal and imaginary off-diagonal
+//           (RX-like: real parts of m00, m11, imaginary parts of m01, m10)
 //   SUB_D2  4x4 matrix on register bits (a<b) = pair `code` (lexicographic),
 //           gate index (bit b, bit a)
-//   SUB_DIAG  16 phases (one per slot), applied where `mask` has a bit
+//   SUB_DIAG  16 phases (one per slot; exactly 1 where `mask` has no bit)
 //   SUB_CX  X on register bit t = code & 3 where every bit of the control
 //           mask code >> 2 is set: a pure register exchange (X, CNOT, CCNOT)
 //   SUB_SWAP  exchange of register bits (a<b) = pair `code & 7` where every
 //           bit of the control mask code >> 3 is set (SWAP, CSWAP)
 // Exchanges move amplitudes bit for bit; phases of monomial gates (ISWAP,
 // Y, ...) follow as a SUB_DIAG.
-enum SubKind : int { SUB_D1 = 1, SUB_D2 = 2, SUB_DIAG = 4, SUB_CX = 5, SUB_SWAP = 6 };
+enum SubKind : int {
+  SUB_D1 = 1, SUB_D2 = 2, SUB_DIAG = 4, SUB_CX = 5, SUB_SWAP = 6, SUB_D1R = 7, SUB_D1X = 8
+};
 constexpr int kRegBits = 4;
 constexpr int kRegs = 1 << kRegBits;
 
@@ -170,6 +175,31 @@ __device__ __forceinline__ void sub_d1(v2d (&v)[kRegs], const v2d* __restrict__ 
     cmac(b, m3, y);
     v[r] = a;
     v[r | (1 << A)] = b;
+  }
+}
+
+template <int A>
+__device__ __forceinline__ void sub_d1r(v2d (&v)[kRegs], const v2d* __restrict__ m) {
+  const double m0 = m[0].x, m1 = m[1].x, m2 = m[2].x, m3 = m[3].x;
+#pragma unroll
+  for (int r = 0; r < kRegs; ++r) {
+    if (r & (1 << A)) continue;
+    const v2d x = v[r], y = v[r | (1 << A)];
+    v[r] = v2d{fma(m0, x.x, m1 * y.x), fma(m0, x.y, m1 * y.y)};
+    v[r | (1 << A)] = v2d{fma(m2, x.x, m3 * y.x), fma(m2, x.y, m3 * y.y)};
+  }
+}
+
+template <int A>
+__device__ __forceinline__ void sub_d1x(v2d (&v)[kRegs], const v2d* __restrict__ m) {
+  // [[c0, i s0], [i s1, c1]]
+  const double c0 = m[0].x, s0 = m[1].y, s1 = m[2].y, c1 = m[3].x;
+#pragma unroll
+  for (int r = 0; r < kRegs; ++r) {
+    if (r & (1 << A)) continue;
+    const v2d x = v[r], y = v[r | (1 << A)];
+    v[r] = v2d{fma(c0, x.x, -s0 * y.y), fma(c0, x.y, s0 * y.x)};
+    v[r | (1 << A)] = v2d{fma(c1, y.x, -s1 * x.y), fma(c1, y.y, s1 * x.x)};
   }
 }
 
@@ -191,11 +221,11 @@ __device__ __forceinline__ void sub_d2(v2d (&v)[kRegs], const v2d* __restrict__ 
   }
 }
 
-__device__ __forceinline__ void sub_diag(v2d (&v)[kRegs], const v2d* __restrict__ ph,
-                                         unsigned mask) {
+// Every slot is multiplied (slots outside `mask` hold exactly 1, which
+// leaves an amplitude bit for bit unchanged): no per-slot branches.
+__device__ __forceinline__ void sub_diag(v2d (&v)[kRegs], const v2d* __restrict__ ph) {
 #pragma unroll
-  for (int r = 0; r < kRegs; ++r)
-    if (mask & (1u << r)) v[r] = cmul(ph[r], v[r]);
+  for (int r = 0; r < kRegs; ++r) v[r] = cmul(ph[r], v[r]);
 }
 
 // Exchanges with a runtime control mask: the pair structure (target bits)
@@ -249,7 +279,23 @@ __device__ __forceinline__ void apply_sub(v2d (&v)[kRegs], const int* __restrict
         case 5: sub_d2<2, 3>(v, m); break;
       }
       break;
-    case SUB_DIAG: sub_diag(v, m, (unsigned)sub[3]); break;
+    case SUB_D1R:
+      switch (code) {
+        case 0: sub_d1r<0>(v, m); break;
+        case 1: sub_d1r<1>(v, m); break;
+        case 2: sub_d1r<2>(v, m); break;
+        case 3: sub_d1r<3>(v, m); break;
+      }
+      break;
+    case SUB_D1X:
+      switch (code) {
+        case 0: sub_d1x<0>(v, m); break;
+        case 1: sub_d1x<1>(v, m); break;
+        case 2: sub_d1x<2>(v, m); break;
+        case 3: sub_d1x<3>(v, m); break;
+      }
+      break;
+    case SUB_DIAG: sub_diag(v, m); break;
     case SUB_CX: {
       const unsigned cm = (unsigned)code >> 2;
       switch (code & 3) {
@@ -460,7 +506,7 @@ __global__ __launch_bounds__(BS) void tile_pass_kernel(v2d* state, int T, TilePo
     for (int u = 0; u < kLoadBatch; ++u) {
       const int kk = k0 + u;
       if (kk < per) {
-        v[u] = src[high_off<LB>(kk, T, pos)];
+        v[u] = ld_stream(src + high_off<LB>(kk, T, pos));
       }
     }
 #pragma unroll
@@ -482,7 +528,7 @@ __global__ __launch_bounds__(BS) void tile_pass_kernel(v2d* state, int T, TilePo
     for (int u = 0; u < kLoadBatch; ++u) {
       const int kk = k0 + u;
       if (kk < per) {
-        dst[high_off<LB>(kk, T, pos)] = v[u];
+        st_stream(dst + high_off<LB>(kk, T, pos), v[u]);
       }
     }
   }
@@ -492,8 +538,11 @@ __global__ __launch_bounds__(BS) void tile_pass_kernel(v2d* state, int T, TilePo
 // Persistent, software-pipelined variant: each workgroup walks tiles
 // blockIdx.x, blockIdx.x + gridDim.x, ...; while the current tile is
 // processed in LDS, the next tile's amplitudes are already in flight into
-// registers (pf), so HBM traffic overlaps the LDS work.  Needs
-// 2^T / BS <= kPrefetch amplitudes per lane.
+// registers (pf), so HBM traffic overlaps the LDS work.  Needs exactly
+// kPrefetch = 2^T / BS amplitudes per lane (T = LB + 4): with that count
+// fixed at compile time the loads and stores carry no guards, and the wait
+// before a prefetched amplitude goes to LDS covers only its own load, not
+// the previous tile's stores.
 constexpr int kPrefetch = 16;
 
 template <int BS, int LB, bool WIDE, bool MF>
@@ -507,10 +556,10 @@ __global__ __launch_bounds__(BS) void tile_pipe_kernel(v2d* state, int T, TilePo
   uint64_t toff = 0;
 #pragma unroll
   for (int j = 0; j < LB; ++j) toff |= (uint64_t)((tid >> j) & 1) << pos.s[j];
-  const int per = 1 << (T - LB);
-  uint64_t hoff[kPrefetch];
-#pragma unroll
-  for (int u = 0; u < kPrefetch; ++u) hoff[u] = high_off<LB>(u, T, pos);
+  // the four tile bits above the lane bits: amplitude u of a lane sits at
+  // h(u) = sum of hb[j] over the set bits j of u
+  const uint64_t hb0 = (uint64_t)1 << pos.s[LB], hb1 = (uint64_t)1 << pos.s[LB + 1];
+  const uint64_t hb2 = (uint64_t)1 << pos.s[LB + 2], hb3 = (uint64_t)1 << pos.s[LB + 3];
 
   auto tile_base = [&](uint32_t tile) {
     uint64_t base = tile;
@@ -523,35 +572,46 @@ __global__ __launch_bounds__(BS) void tile_pipe_kernel(v2d* state, int T, TilePo
     }
     return base + toff;
   };
+  auto load_tile = [&](v2d (&pf)[kPrefetch], uint64_t base) {
+    const v2d* p0 = state + base;
+    const v2d* p1 = p0 + hb1;
+#pragma unroll
+    for (int u = 0; u < kPrefetch; u += 4) {
+      const uint64_t o = ((u & 4) ? hb2 : 0) + ((u & 8) ? hb3 : 0);
+      pf[u] = ld_stream(p0 + o);
+      pf[u + 1] = ld_stream(p0 + o + hb0);
+      pf[u + 2] = ld_stream(p1 + o);
+      pf[u + 3] = ld_stream(p1 + o + hb0);
+    }
+  };
 
   v2d pf[kPrefetch];
-  uint32_t tile = blockIdx.x;
-  if (tile < ntiles) {
-    const v2d* src = state + tile_base(tile);
-#pragma unroll
-    for (int u = 0; u < kPrefetch; ++u)
-      if (u < per) pf[u] = src[hoff[u]];
-  }
+  uint32_t tile = blockIdx.x;       // the grid never exceeds ntiles
+  uint64_t base = tile_base(tile);
+  load_tile(pf, base);
   for (; tile < ntiles; tile += gridDim.x) {
 #pragma unroll
-    for (int u = 0; u < kPrefetch; ++u)
-      if (u < per) lds[sw(tid + u * BS)] = pf[u];
-    const uint32_t next = tile + gridDim.x;
-    if (next < ntiles) {
-      const v2d* src = state + tile_base(next);
-#pragma unroll
-      for (int u = 0; u < kPrefetch; ++u)
-        if (u < per) pf[u] = src[hoff[u]];
-    }
+    for (int u = 0; u < kPrefetch; ++u) lds[sw(tid + u * BS)] = pf[u];
+    // the last tile re-reads itself instead of branching around the loads
+    const uint32_t next = tile + gridDim.x < ntiles ? tile + gridDim.x : tile;
+    const uint64_t nbase = tile_base(next);
+    load_tile(pf, nbase);
     __syncthreads();
     for (int i = 0; i < nops; ++i) {
       apply_op<BS, WIDE, MF>(lds, T, ops[i], coef, srcs);
       __syncthreads();
     }
-    v2d* dst = state + tile_base(tile);
+    v2d* q0 = state + base;
+    v2d* q1 = q0 + hb1;
 #pragma unroll
-    for (int u = 0; u < kPrefetch; ++u)
-      if (u < per) dst[hoff[u]] = lds[sw(tid + u * BS)];
+    for (int u = 0; u < kPrefetch; u += 4) {
+      const uint64_t o = ((u & 4) ? hb2 : 0) + ((u & 8) ? hb3 : 0);
+      st_stream(q0 + o, lds[sw(tid + u * BS)]);
+      st_stream(q0 + o + hb0, lds[sw(tid + (u + 1) * BS)]);
+      st_stream(q1 + o, lds[sw(tid + (u + 2) * BS)]);
+      st_stream(q1 + o + hb0, lds[sw(tid + (u + 3) * BS)]);
+    }
+    base = nbase;
     __syncthreads();
   }
 }
@@ -578,7 +638,7 @@ int launch_pass(hipStream_t s, v2d* state, int n, int T, const TilePos& pos,
     attr_set = true;
   }
   const uint64_t tiles = (uint64_t)1 << (n - T);
-  if (persistent && BS <= 512 && (T - LB) <= 4) {
+  if (persistent && BS <= 512 && T - LB == 4) {
     if (occ[T] == 0) {
       int per_cu = 0, dev = 0, cus = 0;
       int rc = qfx::check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(
@@ -674,10 +734,10 @@ int qfx_tile_check_ops(int T, int block, int flags, const qfx_tile_op* ops, int 
         const int* sub = srcs + o.src + 4 * q;
         // codes per type: D1 4, D2 6, DIAG 1, CX 4 targets x 16 masks,
         // SWAP 6 pairs x 16 masks; control masks may not touch the targets
-        static const int ncode[7] = {0, 4, 6, 0, 1, 64, 128};
-        static const int need[7] = {0, 4, 16, 0, kRegs, 0, 0};
+        static const int ncode[9] = {0, 4, 6, 0, 1, 64, 128, 4, 4};
+        static const int need[9] = {0, 4, 16, 0, kRegs, 0, 0, 4, 4};
         const int ty = sub[0], cd = sub[1];
-        if (ty < SUB_D1 || ty > SUB_SWAP || ty == 3 || cd < 0 || cd >= ncode[ty])
+        if (ty < SUB_D1 || ty > SUB_D1X || ty == 3 || cd < 0 || cd >= ncode[ty])
           return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: bad sub-op");
         if (ty == SUB_CX && (((cd >> 2) >> (cd & 3)) & 1))
           return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: CX control on its target");

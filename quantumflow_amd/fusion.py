@@ -38,7 +38,7 @@ from . import backend as bk
 from .states import State
 
 OP_DENSE, OP_SPARSE, OP_RPHASE, OP_MFMA = 0, 1, 2, 3
-SUB_D1, SUB_D2, SUB_DIAG, SUB_CX, SUB_SWAP = 1, 2, 4, 5, 6
+SUB_D1, SUB_D2, SUB_DIAG, SUB_CX, SUB_SWAP, SUB_D1R, SUB_D1X = 1, 2, 4, 5, 6, 7, 8
 REG_BITS = 4
 MAX_TILE_BITS = 13
 MIN_TILE_BITS = 6
@@ -378,6 +378,18 @@ def _slot_phases(diag, p):
     return ph
 
 
+def d1_variant(m):
+    """Cheapest 1-qubit sub-op that applies 2x2 matrix m exactly: real (RY,
+    H: half the FMAs), real diagonal with imaginary off-diagonal (RX: half
+    the FMAs) or general."""
+    m = np.asarray(m, dtype=np.complex128)
+    if not np.any(m.imag):
+        return SUB_D1R
+    if not np.any(m.real[[0, 1], [1, 0]]) and not np.any(m.imag[[0, 1], [0, 1]]):
+        return SUB_D1X
+    return SUB_D1
+
+
 def _encode_phase(R, members, pos_index, T, coefs, subs):
     """qfx_tile_op of a register phase over tile bits R plus its sub-ops."""
     tl = sorted(pos_index[b] for b in R)
@@ -441,6 +453,8 @@ def _encode_phase(R, members, pos_index, T, coefs, subs):
             out.append([SUB_D2, _D2_CODES[ps], m])
     nsub = 0
     for typ, code, data in out:
+        if typ == SUB_D1:
+            typ = d1_variant(data)
         if typ == SUB_DIAG:
             mask = 0
             for r in range(2 ** REG_BITS):

@@ -278,7 +278,7 @@ def run_pipeline(p=4, ntrain=20000, neval=1000, n_lo=8, n_hi=20, opt_steps=200, 
     ramp with the batched engine, train the proposer on (features,
     optimised angles), then on neval fresh graphs compare proposer vs
     linear-ramp starts: starting ratio <C>/maxcut, final ratio after
-    opt_steps optimizer steps, and steps to reach 0.95 x the linear-ramp
+    opt_steps optimizer steps, and steps to reach 0.95, 0.99 and 0.999 x the linear-ramp
     run's final ratio (maxcut = exact brute force)."""
     import time
     log = (lambda *a: print(*a, flush=True)) if verbose else (lambda *a: None)
@@ -312,13 +312,15 @@ def run_pipeline(p=4, ntrain=20000, neval=1000, n_lo=8, n_hi=20, opt_steps=200, 
         traces[name] = tr
         res[name] = {'start_ratio_mean': float(np.mean(tr[:, 0])),
                      'final_ratio_mean': float(np.mean(fin))}
-    target = 0.95 * traces['linear_ramp'][:, -1]
-    for name, tr in traces.items():
-        hit = tr >= target[:, None]
-        steps = np.where(hit.any(axis=1), hit.argmax(axis=1), tr.shape[1])
-        res[name]['steps_to_0.95_of_ramp_optimum_mean'] = float(np.mean(steps))
-        res[name]['steps_to_0.95_of_ramp_optimum_median'] = float(np.median(steps))
-        res[name]['reached_fraction'] = float(np.mean(hit.any(axis=1)))
+    for frac in (0.95, 0.99, 0.999):
+        target = frac * traces['linear_ramp'][:, -1]
+        for name, tr in traces.items():
+            hit = tr >= target[:, None]
+            steps = np.where(hit.any(axis=1), hit.argmax(axis=1), tr.shape[1])
+            key = 'steps_to_%g_of_ramp_optimum' % frac
+            res[name][key + '_mean'] = float(np.mean(steps))
+            res[name][key + '_median'] = float(np.median(steps))
+            res[name][key + '_reached_fraction'] = float(np.mean(hit.any(axis=1)))
     res['proposer_wins_start_ratio_fraction'] = float(np.mean(traces['proposer'][:, 0] >
                                                               traces['linear_ramp'][:, 0]))
     log(res)

@@ -146,12 +146,18 @@ def _emulate_ops(ket, n, pos, ops, coefs, srcs):
                         perm[dst, r] = 1
                     u = perm @ u
                     continue
-                if typ == fusion.SUB_D1:
+                if typ in (fusion.SUB_D1, fusion.SUB_D1R, fusion.SUB_D1X):
                     bits = [code]
                 else:
                     bits = list(d2[code])
                 k = len(bits)
                 m = coefs[c0:c0 + 4 ** k].reshape(2 ** k, 2 ** k)
+                # the specialised kernels read only part of each coefficient
+                if typ == fusion.SUB_D1R:
+                    m = m.real.astype(complex)
+                elif typ == fusion.SUB_D1X:
+                    m = np.array([[m[0, 0].real, 1j * m[0, 1].imag],
+                                  [1j * m[1, 0].imag, m[1, 1].real]])
                 axes = [3 - b for b in reversed(bits)]      # gate MSB <-> highest bit
                 u = O.full_operator(m, 4, axes) @ u
             axes = [n - 1 - gb[j] for j in (3, 2, 1, 0)]
@@ -334,3 +340,12 @@ def test_mfma_validator_rules():
     arr = (_native.QfxTileOp * 1)(dense)
     assert lib.qfx_tile_check_ops(9, 256, 0, arr, 1, len(coefs), srcs, 1) == _native.QFX_EINVAL
     assert lib.qfx_tile_check_ops(9, 256, fusion.TILE_WIDE, arr, 1, len(coefs), srcs, 1) == 0
+
+
+def test_d1_variants():
+    from quantumflow_amd import stdgates as sg
+    assert fusion.d1_variant(sg.RY(0.3).kind.matrix) == fusion.SUB_D1R
+    assert fusion.d1_variant(sg.H().kind.matrix) == fusion.SUB_D1R
+    assert fusion.d1_variant(sg.RX(0.3).kind.matrix) == fusion.SUB_D1X
+    assert fusion.d1_variant(sg.RX(0.3).kind.matrix @ sg.RY(0.2).kind.matrix) == fusion.SUB_D1
+    assert fusion.d1_variant(sg.T().kind.matrix) == fusion.SUB_D1
