@@ -158,9 +158,11 @@ typedef struct {
  * register bits in registers and applies its sub-ops there.  Sub-op records
  * are 4 ints {type, position code, coef index, mask}:
  *   1  2x2 matrix (4 coefs) on register bit `code`;
- *   2  4x4 matrix (16 coefs) on register bits (a<b) = pair `code` in
- *      lexicographic order, gate index (bit b, bit a);
- *   4  16 phases, one per register slot, applied where `mask` has a bit;
+ *   7  real 2x2 matrix (real parts of the 4 coefs are read);
+ *   8  2x2 matrix with real diagonal and imaginary off-diagonal (real parts
+ *      of coefs 0 and 3, imaginary parts of coefs 1 and 2 are read);
+ *   4  16 phases, one per register slot (exactly 1 where `mask` has no
+ *      bit);
  *   5  X on register bit code & 3 where every bit of the control mask
  *      code >> 2 is set (X, CNOT, CCNOT as register exchanges);
  *   6  exchange of register bits (a<b) = pair code & 7 where every bit of
@@ -181,8 +183,8 @@ typedef struct {
 
 /* Host-side validation of an op list against tile width T, the block size
  * and flags it will run with (ops on 4-5 qubits need QFX_TILE_WIDE and
- * block <= 256; matrix-core blocks need QFX_TILE_MFMA), ncoef coefficients
- * and nsrcs source rows (host arrays).  The device entry point trusts its op
+ * block <= 256; matrix-core blocks need QFX_TILE_MFMA; other passes at most
+ * 512 coefficients), ncoef coefficients and nsrcs source rows (host arrays).  The device entry point trusts its op
  * list; the Python planner checks every plan with this before upload. */
 int qfx_tile_check_ops(int T, int block, int flags, const qfx_tile_op* ops /*host*/,
                        int nops, int ncoef, const int* srcs /*host*/,
@@ -190,10 +192,14 @@ int qfx_tile_check_ops(int T, int block, int flags, const qfx_tile_op* ops /*hos
 
 /* One pass: 6 <= tile_bits <= 13, tile_bits <= n, block in {64, 256, 512,
  * 1024} with 2^tile_bits >= block; flags as accepted by qfx_tile_check_ops
- * for this op list.  ops/coef/srcs in device memory. */
+ * for this op list.  ops/coef/srcs in device memory; coef holds the op
+ * list's ncoef coefficients (indices in the ops are relative to coef).  A
+ * pass without QFX_TILE_WIDE or QFX_TILE_MFMA takes at most 512
+ * coefficients: it stages them in LDS. */
 int qfx_tile_pass(void* stream, void* state, int n, int tile_bits,
                   const int* tile_pos /*host*/, const void* ops, int nops,
-                  const void* coef, const void* srcs, int block, int flags);
+                  const void* coef, int ncoef, const void* srcs, int block,
+                  int flags);
 
 /* ---- QAOA MaxCut (quantumflow/qaoa.py) --------------------------------- */
 

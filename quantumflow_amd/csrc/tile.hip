@@ -24,6 +24,12 @@ constexpr int kMaxTileBits = 13;
 constexpr int kWideOpMaxBlock = 256;  // largest block running 4/5-qubit ops
 constexpr int kRphaseMaxBlock = 512;  // largest block running register phases
 constexpr int kMfmaMaxBlock = 512;    // largest block running matrix-core blocks
+// Passes without 4-5 qubit ops or matrix-core blocks copy their coefficient
+// table (at most kMaxStaged entries) into LDS behind the tile once per
+// workgroup: uniform coefficient reads then come from LDS broadcasts instead
+// of scalar registers, which a pass's 2x2/4x4 matrices and 16-slot phase
+// vectors would otherwise exhaust.  8 KiB keeps two 2^12-amplitude tiles per CU.
+constexpr int kMaxStaged = 512;
 
 struct TilePos {
   int s[QFX_MAX_QUBITS];  // tile bit positions, ascending
@@ -64,9 +70,13 @@ __device__ __forceinline__ unsigned loc_off_lsb(const int* t, unsigned r) {
 template <int K>
 constexpr int op_groups() { return K == 1 ? 4 : (K == 2 ? 2 : 1); }
 
+// Coefficient pointers of the LDS ops are deliberately not __restrict__:
+// the coefficients may then not be hoisted above the op's own LDS stores,
+// so a matrix is re-read per group (cheap uniform reads) instead of pinning
+// up to 4^k complex values in registers for the whole op.
 template <int K, int BS>
 __device__ __forceinline__ void op_dense(v2d* lds, int T, const qfx_tile_op& op,
-                         const v2d* __restrict__ coef) {
+                         const v2d* coef) {
   constexpr int D = 1 << K;
   constexpr int G = op_groups<K>();
   const v2d* m = coef + op.coef;
@@ -94,6 +104,8 @@ __device__ __forceinline__ void op_dense(v2d* lds, int T, const qfx_tile_op& op,
 #pragma unroll
           for (int c = 0; c < D; ++c) cmac(acc, m[r * D + c], v[u][c]);
           lds[sw(lb[u] + loc_off<K>(op.t, r))] = acc;
+          // one matrix row in registers at a time
+          __builtin_amdgcn_sched_barrier(0);
         }
       }
     }
@@ -102,7 +114,7 @@ __device__ __forceinline__ void op_dense(v2d* lds, int T, const qfx_tile_op& op,
 
 template <int K, int BS>
 __device__ __forceinline__ void op_sparse(v2d* lds, int T, const qfx_tile_op& op,
-                          const v2d* __restrict__ coef, const int* __restrict__ srcs) {
+                          const v2d* coef, const int* __restrict__ srcs) {
   constexpr int D = 1 << K;
   constexpr int G = op_groups<K>();
   const v2d* ph = coef + op.coef;
@@ -146,17 +158,17 @@ __device__ __forceinline__ void op_sparse(v2d* lds, int T, const qfx_tile_op& op
 //   SUB_D1R 2x2 real matrix (real parts of the 4 coefficients; RY, H)
 //   SUB_D1X 2x2 matrix with real diagonal and imaginary off-diagonal
 //           (RX-like: real parts of m00, m11, imaginary parts of m01, m10)
-//   SUB_D2  4x4 matrix on register bits (a<b) = pair `code` (lexicographic),
-//           gate index (bit b, bit a)
 //   SUB_DIAG  16 phases (one per slot; exactly 1 where `mask` has no bit)
 //   SUB_CX  X on register bit t = code & 3 where every bit of the control
 //           mask code >> 2 is set: a pure register exchange (X, CNOT, CCNOT)
 //   SUB_SWAP  exchange of register bits (a<b) = pair `code & 7` where every
 //           bit of the control mask code >> 3 is set (SWAP, CSWAP)
 // Exchanges move amplitudes bit for bit; phases of monomial gates (ISWAP,
-// Y, ...) follow as a SUB_DIAG.
+// Y, ...) follow as a SUB_DIAG.  Dense 2-qubit gates run as LDS ops: a 4x4
+// matrix next to 16 register amplitudes would raise the register allocation
+// of the whole kernel past two waves per SIMD.
 enum SubKind : int {
-  SUB_D1 = 1, SUB_D2 = 2, SUB_DIAG = 4, SUB_CX = 5, SUB_SWAP = 6, SUB_D1R = 7, SUB_D1X = 8
+  SUB_D1 = 1, SUB_DIAG = 4, SUB_CX = 5, SUB_SWAP = 6, SUB_D1R = 7, SUB_D1X = 8
 };
 constexpr int kRegBits = 4;
 constexpr int kRegs = 1 << kRegBits;
@@ -203,29 +215,15 @@ __device__ __forceinline__ void sub_d1x(v2d (&v)[kRegs], const v2d* __restrict__
   }
 }
 
-template <int A, int B>
-__device__ __forceinline__ void sub_d2(v2d (&v)[kRegs], const v2d* __restrict__ m) {
-#pragma unroll
-  for (int r = 0; r < kRegs; ++r) {
-    if (r & ((1 << A) | (1 << B))) continue;
-    v2d x[4];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) x[c] = v[r | ((c >> 1) << B) | ((c & 1) << A)];
-#pragma unroll
-    for (int o = 0; o < 4; ++o) {
-      v2d acc = {0.0, 0.0};
-#pragma unroll
-      for (int c = 0; c < 4; ++c) cmac(acc, m[o * 4 + c], x[c]);
-      v[r | ((o >> 1) << B) | ((o & 1) << A)] = acc;
-    }
-  }
-}
-
 // Every slot is multiplied (slots outside `mask` hold exactly 1, which
 // leaves an amplitude bit for bit unchanged): no per-slot branches.
 __device__ __forceinline__ void sub_diag(v2d (&v)[kRegs], const v2d* __restrict__ ph) {
 #pragma unroll
-  for (int r = 0; r < kRegs; ++r) v[r] = cmul(ph[r], v[r]);
+  for (int r0 = 0; r0 < kRegs; r0 += 4) {
+#pragma unroll
+    for (int r = r0; r < r0 + 4; ++r) v[r] = cmul(ph[r], v[r]);
+    __builtin_amdgcn_sched_barrier(0);   // four phases in registers at a time
+  }
 }
 
 // Exchanges with a runtime control mask: the pair structure (target bits)
@@ -257,7 +255,7 @@ __device__ __forceinline__ void sub_swap(v2d (&v)[kRegs], unsigned cm) {
 }
 
 __device__ __forceinline__ void apply_sub(v2d (&v)[kRegs], const int* __restrict__ sub,
-                                          const v2d* __restrict__ coef) {
+                                          const v2d* coef) {
   const int type = sub[0], code = sub[1];
   const v2d* m = coef + sub[2];
   switch (type) {
@@ -267,16 +265,6 @@ __device__ __forceinline__ void apply_sub(v2d (&v)[kRegs], const int* __restrict
         case 1: sub_d1<1>(v, m); break;
         case 2: sub_d1<2>(v, m); break;
         case 3: sub_d1<3>(v, m); break;
-      }
-      break;
-    case SUB_D2:
-      switch (code) {
-        case 0: sub_d2<0, 1>(v, m); break;
-        case 1: sub_d2<0, 2>(v, m); break;
-        case 2: sub_d2<0, 3>(v, m); break;
-        case 3: sub_d2<1, 2>(v, m); break;
-        case 4: sub_d2<1, 3>(v, m); break;
-        case 5: sub_d2<2, 3>(v, m); break;
       }
       break;
     case SUB_D1R:
@@ -323,26 +311,34 @@ __device__ __forceinline__ void apply_sub(v2d (&v)[kRegs], const int* __restrict
 
 template <int BS>
 __device__ __forceinline__ void op_rphase(v2d* lds, int T, const qfx_tile_op& op,
-                                          const v2d* __restrict__ coef,
+                                          const v2d* coef,
                                           const int* __restrict__ subs) {
   const int ngroups = 1 << (T - kRegBits);
   const int* sub0 = subs + op.src;
   const int nsub = op.coef;
-  // sw is XOR-linear on disjoint bit sets: sw(lb | off) = sw(lb) ^ sw(off)
-  unsigned soff[kRegs];
-#pragma unroll
-  for (int r = 0; r < kRegs; ++r) soff[r] = sw(loc_off_lsb<kRegBits>(op.t, r));
+  // sw is XOR-linear: the slot of register r is sw(lb) ^ (XOR of w[j] over
+  // the set bits j of r), w[j] = sw(1 << t[j]), built up as a XOR tree
+  const unsigned w0 = sw(1u << op.t[0]), w1 = sw(1u << op.t[1]);
+  const unsigned w2 = sw(1u << op.t[2]), w3 = sw(1u << op.t[3]);
   for (int g = threadIdx.x; g < ngroups; g += BS) {
     unsigned lb = (unsigned)g;
 #pragma unroll
     for (int j = 0; j < kRegBits; ++j) lb = ins0(lb, op.sorted[j]);
-    const unsigned slb = sw(lb);
+    unsigned a[kRegs];
+    a[0] = sw(lb);
+    a[1] = a[0] ^ w0;
+    a[2] = a[0] ^ w1;
+    a[3] = a[1] ^ w1;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) a[4 + r] = a[r] ^ w2;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) a[8 + r] = a[r] ^ w3;
     v2d v[kRegs];
 #pragma unroll
-    for (int r = 0; r < kRegs; ++r) v[r] = lds[slb ^ soff[r]];
+    for (int r = 0; r < kRegs; ++r) v[r] = lds[a[r]];
     for (int i = 0; i < nsub; ++i) apply_sub(v, sub0 + 4 * i, coef);
 #pragma unroll
-    for (int r = 0; r < kRegs; ++r) lds[slb ^ soff[r]] = v[r];
+    for (int r = 0; r < kRegs; ++r) lds[a[r]] = v[r];
   }
 }
 
@@ -421,9 +417,13 @@ __device__ __forceinline__ void op_mfma(v2d* lds, int T, const qfx_tile_op& op,
   }
 }
 
+// `coef` holds the op list's coefficients (staged in LDS by the kernels that
+// stage them); matrix-core blocks read their packed operands from `coef_g`,
+// the same table in global memory (each lane loads its own operand).
 template <int BS, bool WIDE, bool MF>
 __device__ __forceinline__ void apply_op(v2d* lds, int T, const qfx_tile_op& op,
-                                         const v2d* __restrict__ coef,
+                                         const v2d* coef,
+                                         const v2d* __restrict__ coef_g,
                                          const int* __restrict__ srcs) {
   if (op.kind == OP_RPHASE) {
     // 16 register amplitudes per lane exceed what 1024-lane blocks leave
@@ -431,9 +431,9 @@ __device__ __forceinline__ void apply_op(v2d* lds, int T, const qfx_tile_op& op,
   } else if (op.kind == OP_MFMA) {
     if constexpr (MF && BS <= kMfmaMaxBlock) {
       switch (op.k) {
-        case 3: op_mfma<3, BS>(lds, T, op, coef); break;
-        case 4: op_mfma<4, BS>(lds, T, op, coef); break;
-        case 5: op_mfma<5, BS>(lds, T, op, coef); break;
+        case 3: op_mfma<3, BS>(lds, T, op, coef_g); break;
+        case 4: op_mfma<4, BS>(lds, T, op, coef_g); break;
+        case 5: op_mfma<5, BS>(lds, T, op, coef_g); break;
       }
     }
   } else if (op.kind == OP_DENSE) {
@@ -476,13 +476,22 @@ __device__ __forceinline__ uint64_t high_off(int kk, int T, const TilePos& pos) 
   return o;
 }
 
+template <int BS>
+__device__ __forceinline__ v2d* stage_coef(v2d* lds, int T, const v2d* __restrict__ coef, int ncoef) {
+  v2d* cf = lds + (1 << T);
+  for (int i = threadIdx.x; i < ncoef; i += BS) cf[i] = coef[i];
+  return cf;   // read after the kernel's first barrier
+}
+
 template <int BS, int LB, bool WIDE, bool MF>
 __global__ __launch_bounds__(BS) void tile_pass_kernel(v2d* state, int T, TilePos pos,
                                                       const qfx_tile_op* __restrict__ ops,
                                                       int nops, const v2d* __restrict__ coef,
-                                                      const int* __restrict__ srcs) {
+                                                      int ncoef, const int* __restrict__ srcs) {
   extern __shared__ v2d lds[];
   const int tid = threadIdx.x;
+  constexpr bool kStage = !WIDE && !MF;
+  v2d* cf = kStage ? stage_coef<BS>(lds, T, coef, ncoef) : nullptr;
   // tile base: block index bits spread over the non-tile positions
   // (loops over tile bits run to a compile-time bound so pos.s is indexed
   // with constants and stays in kernel-argument registers)
@@ -515,7 +524,8 @@ __global__ __launch_bounds__(BS) void tile_pass_kernel(v2d* state, int T, TilePo
   }
   __syncthreads();
   for (int i = 0; i < nops; ++i) {
-    apply_op<BS, WIDE, MF>(lds, T, ops[i], coef, srcs);
+    if constexpr (kStage) apply_op<BS, WIDE, MF>(lds, T, ops[i], cf, coef, srcs);
+    else apply_op<BS, WIDE, MF>(lds, T, ops[i], coef, coef, srcs);
     __syncthreads();
   }
   v2d* dst = state + base + toff;
@@ -549,10 +559,12 @@ template <int BS, int LB, bool WIDE, bool MF>
 __global__ __launch_bounds__(BS) void tile_pipe_kernel(v2d* state, int T, TilePos pos,
                                                       const qfx_tile_op* __restrict__ ops,
                                                       int nops, const v2d* __restrict__ coef,
-                                                      const int* __restrict__ srcs,
+                                                      int ncoef, const int* __restrict__ srcs,
                                                       uint32_t ntiles) {
   extern __shared__ v2d lds[];
   const int tid = threadIdx.x;
+  constexpr bool kStage = !WIDE && !MF;
+  v2d* cf = kStage ? stage_coef<BS>(lds, T, coef, ncoef) : nullptr;
   uint64_t toff = 0;
 #pragma unroll
   for (int j = 0; j < LB; ++j) toff |= (uint64_t)((tid >> j) & 1) << pos.s[j];
@@ -598,7 +610,8 @@ __global__ __launch_bounds__(BS) void tile_pipe_kernel(v2d* state, int T, TilePo
     load_tile(pf, nbase);
     __syncthreads();
     for (int i = 0; i < nops; ++i) {
-      apply_op<BS, WIDE, MF>(lds, T, ops[i], coef, srcs);
+      if constexpr (kStage) apply_op<BS, WIDE, MF>(lds, T, ops[i], cf, coef, srcs);
+      else apply_op<BS, WIDE, MF>(lds, T, ops[i], coef, coef, srcs);
       __syncthreads();
     }
     v2d* q0 = state + base;
@@ -618,9 +631,12 @@ __global__ __launch_bounds__(BS) void tile_pipe_kernel(v2d* state, int T, TilePo
 
 template <int BS, int LB, bool WIDE, bool MF>
 int launch_pass(hipStream_t s, v2d* state, int n, int T, const TilePos& pos,
-                const qfx_tile_op* ops, int nops, const v2d* coef, const int* srcs,
+                const qfx_tile_op* ops, int nops, const v2d* coef, int ncoef, const int* srcs,
                 bool persistent) {
-  const size_t lds = sizeof(v2d) << T;
+  constexpr bool kStage = !WIDE && !MF;
+  const size_t lds = sizeof(v2d) * ((size_t(1) << T) + (kStage ? ncoef : 0));
+  // occupancy (persistent grid size) for the largest staged table
+  const size_t lds_max = sizeof(v2d) * ((size_t(1) << T) + (kStage ? kMaxStaged : 0));
   static bool attr_set = false;
   static int occ[kMaxTileBits + 1];
   if (!attr_set) {
@@ -642,7 +658,7 @@ int launch_pass(hipStream_t s, v2d* state, int n, int T, const TilePos& pos,
     if (occ[T] == 0) {
       int per_cu = 0, dev = 0, cus = 0;
       int rc = qfx::check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                                  &per_cu, (const void*)tile_pipe_kernel<BS, LB, WIDE, MF>, BS, lds),
+                                  &per_cu, (const void*)tile_pipe_kernel<BS, LB, WIDE, MF>, BS, lds_max),
                               "hipOccupancyMaxActiveBlocksPerMultiprocessor");
       if (rc) return rc;
       rc = qfx::check_hip(hipGetDevice(&dev), "hipGetDevice");
@@ -654,11 +670,11 @@ int launch_pass(hipStream_t s, v2d* state, int n, int T, const TilePos& pos,
     }
     const uint64_t grid = tiles < (uint64_t)occ[T] ? tiles : (uint64_t)occ[T];
     hipLaunchKernelGGL((tile_pipe_kernel<BS, LB, WIDE, MF>), dim3((unsigned)grid), dim3(BS), lds, s,
-                       state, T, pos, ops, nops, coef, srcs, (uint32_t)tiles);
+                       state, T, pos, ops, nops, coef, ncoef, srcs, (uint32_t)tiles);
     return qfx::check_launch("tile_pipe_kernel");
   }
   hipLaunchKernelGGL((tile_pass_kernel<BS, LB, WIDE, MF>), dim3((unsigned)tiles), dim3(BS), lds, s, state, T,
-                     pos, ops, nops, coef, srcs);
+                     pos, ops, nops, coef, ncoef, srcs);
   return qfx::check_launch("tile_pass_kernel");
 }
 
@@ -693,6 +709,9 @@ int qfx_tile_check_ops(int T, int block, int flags, const qfx_tile_op* ops, int 
   if ((flags & QFX_TILE_MFMA) && block > kMfmaMaxBlock)
     return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: matrix-core blocks need block <= 512");
   if (!ops && nops > 0) return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: null ops");
+  if (!(flags & (QFX_TILE_WIDE | QFX_TILE_MFMA)) && ncoef > kMaxStaged)
+    return qfx::set_error(QFX_EINVAL,
+                          "qfx_tile_check_ops: more than 512 coefficients in a pass without wide ops");
   const int kmax = (flags & QFX_TILE_WIDE) ? 5 : 3;
   for (int i = 0; i < nops; ++i) {
     const qfx_tile_op& o = ops[i];
@@ -732,12 +751,12 @@ int qfx_tile_check_ops(int T, int block, int flags, const qfx_tile_op* ops, int 
         return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: sub-ops out of range");
       for (int q = 0; q < o.coef; ++q) {
         const int* sub = srcs + o.src + 4 * q;
-        // codes per type: D1 4, D2 6, DIAG 1, CX 4 targets x 16 masks,
+        // codes per type: D1/D1R/D1X 4, DIAG 1, CX 4 targets x 16 masks,
         // SWAP 6 pairs x 16 masks; control masks may not touch the targets
-        static const int ncode[9] = {0, 4, 6, 0, 1, 64, 128, 4, 4};
-        static const int need[9] = {0, 4, 16, 0, kRegs, 0, 0, 4, 4};
+        static const int ncode[9] = {0, 4, 0, 0, 1, 64, 128, 4, 4};
+        static const int need[9] = {0, 4, 0, 0, kRegs, 0, 0, 4, 4};
         const int ty = sub[0], cd = sub[1];
-        if (ty < SUB_D1 || ty > SUB_D1X || ty == 3 || cd < 0 || cd >= ncode[ty])
+        if (ty < SUB_D1 || ty > SUB_D1X || ncode[ty] == 0 || cd < 0 || cd >= ncode[ty])
           return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: bad sub-op");
         if (ty == SUB_CX && (((cd >> 2) >> (cd & 3)) & 1))
           return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: CX control on its target");
@@ -783,8 +802,8 @@ int qfx_tile_check_ops(int T, int block, int flags, const qfx_tile_op* ops, int 
 }
 
 int qfx_tile_pass(void* stream, void* state, int n, int tile_bits, const int* tile_pos,
-                  const void* ops, int nops, const void* coef, const void* srcs, int block,
-                  int flags) {
+                  const void* ops, int nops, const void* coef, int ncoef, const void* srcs,
+                  int block, int flags) {
   int rc = check_pass(n, tile_bits, tile_pos, nops, block, "qfx_tile_pass");
   if (rc) return rc;
   const bool wide = (flags & QFX_TILE_WIDE) != 0;
@@ -796,8 +815,11 @@ int qfx_tile_pass(void* stream, void* state, int n, int tile_bits, const int* ti
     return qfx::set_error(QFX_EINVAL, "qfx_tile_pass: 4-5 qubit ops need block <= 256");
   if (mf && block > kMfmaMaxBlock)
     return qfx::set_error(QFX_EINVAL, "qfx_tile_pass: matrix-core blocks need block <= 512");
-  if (!state || (nops > 0 && (!ops || !coef)))
+  if (!state || (nops > 0 && (!ops || !coef)) || (ncoef > 0 && !coef))
     return qfx::set_error(QFX_EINVAL, "qfx_tile_pass: null pointer");
+  if (ncoef < 0 || (!wide && !mf && ncoef > kMaxStaged))
+    return qfx::set_error(QFX_ERANGE,
+                          "qfx_tile_pass: more than 512 coefficients in a pass without wide ops");
   TilePos pos;
   std::memset(&pos, 0, sizeof(pos));
   for (int j = 0; j < tile_bits; ++j) pos.s[j] = tile_pos[j];
@@ -806,7 +828,8 @@ int qfx_tile_pass(void* stream, void* state, int n, int tile_bits, const int* ti
   auto* o = static_cast<const qfx_tile_op*>(ops);
   auto* c = static_cast<const v2d*>(coef);
   auto* sr = static_cast<const int*>(srcs);
-#define QFX_PASS(BS, LB, W, M) launch_pass<BS, LB, W, M>(s, st, n, tile_bits, pos, o, nops, c, sr, pers)
+#define QFX_PASS(BS, LB, W, M) \
+  launch_pass<BS, LB, W, M>(s, st, n, tile_bits, pos, o, nops, c, ncoef, sr, pers)
   switch (block) {
     case 64:
       if (mf) return wide ? QFX_PASS(64, 6, true, true) : QFX_PASS(64, 6, false, true);

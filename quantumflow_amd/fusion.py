@@ -38,7 +38,7 @@ from . import backend as bk
 from .states import State
 
 OP_DENSE, OP_SPARSE, OP_RPHASE, OP_MFMA = 0, 1, 2, 3
-SUB_D1, SUB_D2, SUB_DIAG, SUB_CX, SUB_SWAP, SUB_D1R, SUB_D1X = 1, 2, 4, 5, 6, 7, 8
+SUB_D1, SUB_DIAG, SUB_CX, SUB_SWAP, SUB_D1R, SUB_D1X = 1, 4, 5, 6, 7, 8
 REG_BITS = 4
 MAX_TILE_BITS = 13
 MIN_TILE_BITS = 6
@@ -47,7 +47,6 @@ TILE_PERSISTENT = 2
 TILE_MFMA = 4
 MFMA_MIN_BITS = 3
 # position codes of the register-phase kernels (lexicographic)
-_D2_CODES = {p: i for i, p in enumerate([(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)])}
 _PAIRS = [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]
 
 
@@ -312,13 +311,14 @@ def exchange_pattern(kind):
 def phaseable(op):
     """Can the op run inside a register phase?"""
     if op.matrix is not None:
-        return len(op.bits) <= 2
+        return len(op.bits) == 1
     kd = op.kind
     if kd.kind == bk.GateKind.DIAGONAL:
         return kd.k <= REG_BITS
     if kd.kind == bk.GateKind.MONOMIAL and exchange_pattern(kd) is not None:
         return kd.k <= REG_BITS
-    return kd.k <= 2
+    # dense 2-qubit gates run as LDS ops (no 4x4 matrices in registers)
+    return kd.k == 1
 
 
 def plan_phases(ops, width=REG_BITS):
@@ -354,15 +354,6 @@ def plan_phases(ops, width=REG_BITS):
         items.append(('phase', R, [ops[i] for i in taken]))
         pending = rest
     return items
-
-
-def _reorder_qubits(matrix, perm):
-    """Matrix of the same gate with its qubits listed in order perm (new
-    qubit j = old qubit perm[j])."""
-    k = len(perm)
-    t = np.asarray(matrix, dtype=np.complex128).reshape([2] * (2 * k))
-    axes = list(perm) + [k + p for p in perm]
-    return np.transpose(t, axes).reshape(2 ** k, 2 ** k)
 
 
 def _slot_phases(diag, p):
@@ -440,17 +431,11 @@ def _encode_phase(R, members, pos_index, T, coefs, subs):
             if kd.phase is not None:
                 push_diag(_slot_phases(kd.phase, p))
             continue
-        m = op.dense_matrix()
-        order = sorted(range(k), key=lambda j: -p[j])       # gate MSB <-> highest bit
-        m = _reorder_qubits(m, order)
-        ps = tuple(sorted(p))
-        if k == 1:
-            if out and out[-1][0] == SUB_D1 and out[-1][1] == ps[0]:
-                out[-1][2] = m @ out[-1][2]                 # same bit: one 2x2
-                continue
-            out.append([SUB_D1, ps[0], m])
-        else:
-            out.append([SUB_D2, _D2_CODES[ps], m])
+        m = np.asarray(op.dense_matrix(), dtype=np.complex128)     # 1-qubit
+        if out and out[-1][0] == SUB_D1 and out[-1][1] == p[0]:
+            out[-1][2] = m @ out[-1][2]                     # same bit: one 2x2
+            continue
+        out.append([SUB_D1, p[0], m])
     nsub = 0
     for typ, code, data in out:
         if typ == SUB_D1:
@@ -542,12 +527,35 @@ def _encode_mfma(bits, matrix, pos_index, coefs):
     return o
 
 
+MAX_STAGED = 512          # coefficients a pass without wide ops stages in LDS
+_COEF_SUBS = (SUB_D1, SUB_DIAG, SUB_D1R, SUB_D1X)
+
+
+def _op_coef_start(o, srcs):
+    """First coefficient index an encoded op reads (None if it reads none)."""
+    if o.kind == OP_RPHASE:
+        idx = [srcs[o.src + 4 * q + 2] for q in range(o.coef)
+               if srcs[o.src + 4 * q] in _COEF_SUBS]
+        return min(idx) if idx else None
+    return o.coef
+
+
+def _rebase_coef(o, srcs, off):
+    if o.kind == OP_RPHASE:
+        for q in range(o.coef):
+            if srcs[o.src + 4 * q] in _COEF_SUBS:
+                srcs[o.src + 4 * q + 2] -= off
+    else:
+        o.coef -= off
+
+
 class HostPlan:
     """A gate list compiled into tiled passes on the host: the pass list,
     the op records, coefficient and source tables, validated by the
     library's qfx_tile_check_ops.  ``passes`` holds ('tile', (tile bits,
-    first op, op count, block, flags)) and ('gate', (bits, GateKind)) for
-    gates too wide to tile."""
+    first op, op count, block, flags, first coefficient, coefficient count))
+    and ('gate', (bits, GateKind)) for gates too wide to tile; op records
+    index their pass's coefficient table."""
 
     def __init__(self, gates, qubits, tile_bits=MAX_TILE_BITS, low_bits=3, block=256,
                  fuse_width=4, mode='reg', persistent=True):
@@ -584,6 +592,7 @@ class HostPlan:
             ops = [Op(bits_list[i], gates[i].kind) for i in members]
             wide = any(len(op.bits) > 3 for op in ops)
             start = len(ops_host)
+            c_lo = len(coefs)
             if self.mode == 'mfma':
                 width = max(MFMA_MIN_BITS, min(fuse_width, 5))
                 for bits, group in plan_blocks(ops, width):
@@ -615,7 +624,7 @@ class HostPlan:
                     block = min(block, 256)
             if persistent:
                 flags |= TILE_PERSISTENT
-            self.passes.append(('tile', (pos, start, len(ops_host) - start, block, flags)))
+            self._add_tile_passes(pos, ops_host, start, coefs, c_lo, srcs, block, flags)
         self.n_passes = len(self.passes)
         self.n_ops = len(ops_host)
         self.ops_host, self.coefs, self.srcs = ops_host, coefs, srcs
@@ -625,10 +634,35 @@ class HostPlan:
         for kind, payload in self.passes:
             if kind != 'tile':
                 continue
-            _, start, nops, block, flags = payload
+            _, start, nops, block, flags, c_lo, ncoef = payload
             arr = (_native.QfxTileOp * max(1, nops))(*ops_host[start:start + nops])
-            _native.check(lib.qfx_tile_check_ops(T, block, flags, arr, nops, len(coefs),
+            _native.check(lib.qfx_tile_check_ops(T, block, flags, arr, nops, ncoef,
                                                  src_arr, len(srcs)), 'qfx_tile_check_ops')
+
+    def _add_tile_passes(self, pos, ops_host, start, coefs, c_lo, srcs, block, flags):
+        """Record the pass of ops_host[start:] with coefficient indices made
+        relative to the pass's table; a pass that would stage more than
+        MAX_STAGED coefficients is cut into consecutive passes on the same
+        tile bits."""
+        ops = ops_host[start:]
+        if not ops:
+            return
+        bounds = [_op_coef_start(o, srcs) for o in ops] + [len(coefs)]
+        # an op without coefficients starts where the next op starts
+        for i in range(len(ops) - 1, -1, -1):
+            if bounds[i] is None:
+                bounds[i] = bounds[i + 1]
+        staged = not flags & (TILE_WIDE | TILE_MFMA)
+        a = 0
+        while a < len(ops):
+            b = a + 1 if staged else len(ops)
+            while b < len(ops) and bounds[b + 1] - bounds[a] <= MAX_STAGED:
+                b += 1
+            for o in ops[a:b]:
+                _rebase_coef(o, srcs, bounds[a])
+            self.passes.append(('tile', (pos, start + a, b - a, block, flags, bounds[a],
+                                         bounds[b] - bounds[a])))
+            a = b
 
     def stats(self):
         ng = len(self.gates)
@@ -664,7 +698,7 @@ class TilePlan(HostPlan):
         lib = _native.load()
         wp = _native.ctypes.c_void_p(work.data_ptr())
         base_ops = self._ops.data_ptr()
-        coef = _native.ctypes.c_void_p(self._coef.data_ptr())
+        coef = self._coef.data_ptr()
         srcs = _native.ctypes.c_void_p(self._srcs.data_ptr())
         for (kind, payload), posarr in zip(self.passes, self._pos_arrays):
             if kind == 'gate':
@@ -672,11 +706,12 @@ class TilePlan(HostPlan):
                 tok = prof.start(kernel_class(gk)) if prof is not None else None
                 bk.apply_gate(work, work, n, bits, gk, inplace_sparse=True)
             else:
-                pos, start, nops, block, flags = payload
+                pos, start, nops, block, flags, c_lo, ncoef = payload
                 tok = prof.start('tile_pass') if prof is not None else None
                 rc = lib.qfx_tile_pass(s, wp, n, posarr[1], posarr[0],
                                        _native.ctypes.c_void_p(base_ops + start * self._opsize),
-                                       nops, coef, srcs, block, flags)
+                                       nops, _native.ctypes.c_void_p(coef + 16 * c_lo), ncoef,
+                                       srcs, block, flags)
                 _native.check(rc, 'qfx_tile_pass')
             if tok is not None:
                 prof.stop(tok)

@@ -310,14 +310,17 @@ def test_mfma_plan_replays(width, T):
     plan = fusion.HostPlan(gates, tuple(range(n)), tile_bits=T, low_bits=3, block=256,
                            fuse_width=width, mode='mfma')
     assert plan.mode == 'mfma' and plan.n_blocks < len(gates)
+    # matrix-core passes read their coefficients from global memory: never cut
+    nbits = [tuple(n - 1 - q for q in g.qubits) for g in gates]
+    assert plan.n_passes == len(fusion.plan_passes(nbits, n, T, 3))
     ket = O.zero_state(n)
     for kind, payload in plan.passes:
         assert kind == 'tile'
-        pos, start, nops, block, flags = payload
+        pos, start, nops, block, flags, c_lo, ncoef = payload
         assert flags & fusion.TILE_MFMA
         ops = plan.ops_host[start:start + nops]
         assert all(o.kind == fusion.OP_MFMA and 3 <= o.k <= width for o in ops)
-        ket = _emulate_ops(ket, n, pos, ops, plan.coefs, plan.srcs)
+        ket = _emulate_ops(ket, n, pos, ops, plan.coefs[c_lo:c_lo + ncoef], plan.srcs)
     assert np.max(np.abs(ket - ref)) < 1e-12
 
 
@@ -349,3 +352,44 @@ def test_d1_variants():
     assert fusion.d1_variant(sg.RX(0.3).kind.matrix) == fusion.SUB_D1X
     assert fusion.d1_variant(sg.RX(0.3).kind.matrix @ sg.RY(0.2).kind.matrix) == fusion.SUB_D1
     assert fusion.d1_variant(sg.T().kind.matrix) == fusion.SUB_D1
+
+
+@pytest.mark.parametrize("mode,width", [('reg', 1), ('lds', 3)])
+def test_host_plan_pass_tables_replay(mode, width):
+    """Each pass indexes its own coefficient table (what the kernel stages in
+    LDS); passes over the staging limit are cut; replaying every pass with
+    its table gives the circuit's state."""
+    n = 12
+    desc = random_mixed_circuit(n, 14, seed=11)
+    gates = build_gates(desc)
+    ref = O.run_circuit(desc, qubits=tuple(range(n)))
+    plan = fusion.HostPlan(gates, tuple(range(n)), tile_bits=9, low_bits=3, block=256,
+                           fuse_width=width, mode=mode)
+    ket = O.zero_state(n)
+    for kind, payload in plan.passes:
+        pos, start, nops, block, flags, c_lo, ncoef = payload
+        assert ncoef <= fusion.MAX_STAGED or flags & (fusion.TILE_WIDE | fusion.TILE_MFMA)
+        ket = _emulate_ops(ket, n, pos, plan.ops_host[start:start + nops],
+                           plan.coefs[c_lo:c_lo + ncoef], plan.srcs)
+    assert np.max(np.abs(ket - ref)) < 1e-12
+
+
+def test_pass_split_at_staging_limit(monkeypatch):
+    """A pass whose coefficients exceed the staging limit is cut into
+    consecutive passes on the same tile bits."""
+    monkeypatch.setattr(fusion, 'MAX_STAGED', 40)
+    n = 10
+    desc = random_mixed_circuit(n, 8, seed=2)
+    gates = build_gates(desc)
+    ref = O.run_circuit(desc, qubits=tuple(range(n)))
+    plan = fusion.HostPlan(gates, tuple(range(n)), tile_bits=8, low_bits=3, block=256,
+                           fuse_width=1, mode='reg')
+    unsplit = fusion.plan_passes([tuple(n - 1 - q for q in g.qubits) for g in gates], n, 8, 3)
+    assert plan.n_passes > len(unsplit)
+    ket = O.zero_state(n)
+    for kind, payload in plan.passes:
+        pos, start, nops, block, flags, c_lo, ncoef = payload
+        assert ncoef <= 40 or nops == 1
+        ket = _emulate_ops(ket, n, pos, plan.ops_host[start:start + nops],
+                           plan.coefs[c_lo:c_lo + ncoef], plan.srcs)
+    assert np.max(np.abs(ket - ref)) < 1e-12
