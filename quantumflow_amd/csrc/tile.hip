@@ -173,12 +173,19 @@ enum SubKind : int {
 constexpr int kRegBits = 4;
 constexpr int kRegs = 1 << kRegBits;
 
+// 1-qubit sub-ops walk the 8 register pairs (r, r | 1 << A); a scheduling
+// fence after the first four keeps half of the results in flight at a time
+// (all eight would push the kernel past 256 VGPRs, i.e. below two waves per
+// SIMD).
+template <int A>
+constexpr int pair_lo(int p) { return ((p >> A) << (A + 1)) | (p & ((1 << A) - 1)); }
+
 template <int A>
 __device__ __forceinline__ void sub_d1(v2d (&v)[kRegs], const v2d* __restrict__ m) {
   const v2d m0 = m[0], m1 = m[1], m2 = m[2], m3 = m[3];
 #pragma unroll
-  for (int r = 0; r < kRegs; ++r) {
-    if (r & (1 << A)) continue;
+  for (int p = 0; p < kRegs / 2; ++p) {
+    const int r = pair_lo<A>(p);
     const v2d x = v[r], y = v[r | (1 << A)];
     v2d a = {0.0, 0.0}, b = {0.0, 0.0};
     cmac(a, m0, x);
@@ -187,6 +194,7 @@ __device__ __forceinline__ void sub_d1(v2d (&v)[kRegs], const v2d* __restrict__ 
     cmac(b, m3, y);
     v[r] = a;
     v[r | (1 << A)] = b;
+    if (p == kRegs / 4 - 1) __builtin_amdgcn_sched_barrier(0);
   }
 }
 
@@ -194,11 +202,12 @@ template <int A>
 __device__ __forceinline__ void sub_d1r(v2d (&v)[kRegs], const v2d* __restrict__ m) {
   const double m0 = m[0].x, m1 = m[1].x, m2 = m[2].x, m3 = m[3].x;
 #pragma unroll
-  for (int r = 0; r < kRegs; ++r) {
-    if (r & (1 << A)) continue;
+  for (int p = 0; p < kRegs / 2; ++p) {
+    const int r = pair_lo<A>(p);
     const v2d x = v[r], y = v[r | (1 << A)];
     v[r] = v2d{fma(m0, x.x, m1 * y.x), fma(m0, x.y, m1 * y.y)};
     v[r | (1 << A)] = v2d{fma(m2, x.x, m3 * y.x), fma(m2, x.y, m3 * y.y)};
+    if (p == kRegs / 4 - 1) __builtin_amdgcn_sched_barrier(0);
   }
 }
 
@@ -207,11 +216,12 @@ __device__ __forceinline__ void sub_d1x(v2d (&v)[kRegs], const v2d* __restrict__
   // [[c0, i s0], [i s1, c1]]
   const double c0 = m[0].x, s0 = m[1].y, s1 = m[2].y, c1 = m[3].x;
 #pragma unroll
-  for (int r = 0; r < kRegs; ++r) {
-    if (r & (1 << A)) continue;
+  for (int p = 0; p < kRegs / 2; ++p) {
+    const int r = pair_lo<A>(p);
     const v2d x = v[r], y = v[r | (1 << A)];
     v[r] = v2d{fma(c0, x.x, -s0 * y.y), fma(c0, x.y, s0 * y.x)};
     v[r | (1 << A)] = v2d{fma(c1, y.x, -s1 * x.y), fma(c1, y.y, s1 * x.x)};
+    if (p == kRegs / 4 - 1) __builtin_amdgcn_sched_barrier(0);
   }
 }
 
@@ -254,6 +264,10 @@ __device__ __forceinline__ void sub_swap(v2d (&v)[kRegs], unsigned cm) {
   }
 }
 
+// Sub-op records are validated on the host (qfx_tile_check_ops), so every
+// type and code below is one of the listed cases; the unreachable defaults
+// let the dispatch drop its do-nothing path, which would otherwise copy all
+// 16 register amplitudes on every sub-op.
 __device__ __forceinline__ void apply_sub(v2d (&v)[kRegs], const int* __restrict__ sub,
                                           const v2d* coef) {
   const int type = sub[0], code = sub[1];
@@ -265,6 +279,7 @@ __device__ __forceinline__ void apply_sub(v2d (&v)[kRegs], const int* __restrict
         case 1: sub_d1<1>(v, m); break;
         case 2: sub_d1<2>(v, m); break;
         case 3: sub_d1<3>(v, m); break;
+        default: __builtin_unreachable();
       }
       break;
     case SUB_D1R:
@@ -273,6 +288,7 @@ __device__ __forceinline__ void apply_sub(v2d (&v)[kRegs], const int* __restrict
         case 1: sub_d1r<1>(v, m); break;
         case 2: sub_d1r<2>(v, m); break;
         case 3: sub_d1r<3>(v, m); break;
+        default: __builtin_unreachable();
       }
       break;
     case SUB_D1X:
@@ -281,6 +297,7 @@ __device__ __forceinline__ void apply_sub(v2d (&v)[kRegs], const int* __restrict
         case 1: sub_d1x<1>(v, m); break;
         case 2: sub_d1x<2>(v, m); break;
         case 3: sub_d1x<3>(v, m); break;
+        default: __builtin_unreachable();
       }
       break;
     case SUB_DIAG: sub_diag(v, m); break;
@@ -291,6 +308,7 @@ __device__ __forceinline__ void apply_sub(v2d (&v)[kRegs], const int* __restrict
         case 1: sub_cx<1>(v, cm); break;
         case 2: sub_cx<2>(v, cm); break;
         case 3: sub_cx<3>(v, cm); break;
+        default: __builtin_unreachable();
       }
       break;
     }
@@ -303,9 +321,11 @@ __device__ __forceinline__ void apply_sub(v2d (&v)[kRegs], const int* __restrict
         case 3: sub_swap<1, 2>(v, cm); break;
         case 4: sub_swap<1, 3>(v, cm); break;
         case 5: sub_swap<2, 3>(v, cm); break;
+        default: __builtin_unreachable();
       }
       break;
     }
+    default: __builtin_unreachable();
   }
 }
 

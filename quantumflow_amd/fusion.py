@@ -111,10 +111,61 @@ def apply_gates_inplace(gates, qubits, work, prof=None):
 # planning
 # ---------------------------------------------------------------------------
 
-def plan_passes(bits_list, n, T, L, tileable=None):
+def _absorb(bits_list, pending, S, ok=None):
+    """Items (indices into bits_list, in order) a set of bits S takes: an item
+    joins if it is allowed, its bits lie in S and it touches no bit of an
+    item left behind, so every item it overtakes acts on other bits and
+    commutes with it."""
+    blocked = set()
+    taken = []
+    for i in pending:
+        b = bits_list[i]
+        if blocked.intersection(b) or not S.issuperset(b) or (ok is not None and not ok[i]):
+            blocked.update(b)
+            continue
+        taken.append(i)
+    return taken
+
+
+def _grow(bits_list, pending, S, width, ok=None, cap=40):
+    """Grow the bit set S to at most ``width`` bits, one candidate at a time:
+    the candidates are the bit sets of the first ``cap`` distinct items not
+    yet taken (in circuit order), and the one that takes the most further
+    items per added bit wins.  Returns (S, items taken)."""
+    cur = _absorb(bits_list, pending, S, ok)
+    while len(S) < width:
+        taken = set(cur)
+        best = None
+        seen = set()
+        for i in pending:
+            if i in taken or (ok is not None and not ok[i]):
+                continue
+            new = frozenset(bits_list[i]) - S
+            if not new or new in seen or len(S) + len(new) > width:
+                continue
+            seen.add(new)
+            got = _absorb(bits_list, pending, S | new, ok)
+            gain = (len(got) - len(cur)) / len(new)
+            if best is None or gain > best[0]:
+                best = (gain, new, got)
+            if len(seen) >= cap:
+                break
+        if best is None or best[0] <= 0:
+            break
+        S = S | best[1]
+        cur = best[2]
+    return S, cur
+
+
+def plan_passes(bits_list, n, T, L, tileable=None, strategy='gain'):
     """Group gates (given by their index-bit tuples, in circuit order) into
     passes.  Returns a list of (sorted tile bits, [gate indices]); a gate
-    that is not tileable forms a pass of its own with tile bits None."""
+    that is not tileable forms a pass of its own with tile bits None.
+
+    strategy 'gain' grows each pass's tile set by ``_grow`` (the bits that
+    take the most gates per added bit); 'greedy' takes gates first come,
+    first served (28-qubit depth-40 benchmark circuit at T=12: 40 passes
+    against 51)."""
     pending = list(range(len(bits_list)))
     passes = []
     low = set(range(min(L, n)))
@@ -124,30 +175,30 @@ def plan_passes(bits_list, n, T, L, tileable=None):
             passes.append((None, [first]))
             pending = pending[1:]
             continue
-        S = set(low)
-        blocked = set()
-        taken, rest = [], []
-        for gi in pending:
-            bs = set(bits_list[gi])
-            if (bs & blocked) or (tileable is not None and not tileable[gi]):
-                blocked |= bs
-                rest.append(gi)
-                continue
-            u = S | bs
-            if len(u) <= T:
-                S = u
-                taken.append(gi)
-            else:
-                blocked |= bs
-                rest.append(gi)
+        if strategy == 'gain':
+            S, _ = _grow(bits_list, pending, set(low), T, tileable)
+        else:
+            S = set(low)
+            blocked = set()
+            for gi in pending:
+                bs = set(bits_list[gi])
+                if (bs & blocked) or (tileable is not None and not tileable[gi]):
+                    blocked |= bs
+                    continue
+                if len(S | bs) <= T:
+                    S |= bs
+                else:
+                    blocked |= bs
         # fill the tile with the lowest unused bits: longer contiguous runs
         b = 0
         while len(S) < T:
             if b not in S:
                 S.add(b)
             b += 1
+        taken = _absorb(bits_list, pending, S, tileable)
         passes.append((sorted(S), taken))
-        pending = rest
+        done = set(taken)
+        pending = [g for g in pending if g not in done]
     return passes
 
 
@@ -323,36 +374,23 @@ def phaseable(op):
 
 def plan_phases(ops, width=REG_BITS):
     """Group a pass's ops (in order) into register phases of <= width bits
-    and standalone LDS ops.  Same rule as plan_passes: an op joins the phase
-    if its bits fit and it does not touch a bit of an op already deferred, so
-    every op it overtakes acts on other qubits and commutes with it.  Returns
-    [('phase', bits, ops) | ('lds', op)]."""
+    and standalone LDS ops, with the commuting-overtake rule of
+    plan_passes and its bit-set growth.  Returns [('phase', bits, ops) |
+    ('lds', op)]."""
+    bits_list = [tuple(op.bits) for op in ops]
+    ok = [phaseable(op) for op in ops]
     pending = list(range(len(ops)))
     items = []
     while pending:
-        first = ops[pending[0]]
-        if not phaseable(first):
-            items.append(('lds', first))
+        first = pending[0]
+        if not ok[first]:
+            items.append(('lds', ops[first]))
             pending = pending[1:]
             continue
-        R = set(first.bits)
-        blocked = set()
-        taken, rest = [], []
-        for i in pending:
-            b = set(ops[i].bits)
-            if (b & blocked) or not phaseable(ops[i]):
-                blocked |= b
-                rest.append(i)
-                continue
-            u = R | b
-            if len(u) <= width:
-                R = u
-                taken.append(i)
-            else:
-                blocked |= b
-                rest.append(i)
+        R, taken = _grow(bits_list, pending, set(), width, ok)
         items.append(('phase', R, [ops[i] for i in taken]))
-        pending = rest
+        done = set(taken)
+        pending = [i for i in pending if i not in done]
     return items
 
 
@@ -460,30 +498,20 @@ def _encode_phase(R, members, pos_index, T, coefs, subs):
     return o
 
 def plan_blocks(ops, width):
-    """Group a pass's ops (in order) into dense blocks of <= width qubits with
-    the commuting-overtake rule of plan_passes.  Returns [(bits, [ops])] with
-    bits in first-use order."""
+    """Group a pass's ops (in order) into dense blocks of <= width qubits,
+    with the commuting-overtake rule and bit-set growth of plan_passes.
+    Returns [(bits, [ops])], bits in first-use order."""
+    bits_list = [tuple(op.bits) for op in ops]
     pending = list(range(len(ops)))
     blocks = []
     while pending:
-        bits = list(ops[pending[0]].bits)
-        blocked = set()
-        taken, rest = [], []
-        for i in pending:
-            b = ops[i].bits
-            if set(b) & blocked:
-                blocked |= set(b)
-                rest.append(i)
-                continue
-            new = [x for x in b if x not in bits]
-            if len(bits) + len(new) <= width:
-                bits += new
-                taken.append(i)
-            else:
-                blocked |= set(b)
-                rest.append(i)
+        _, taken = _grow(bits_list, pending, set(), width)
+        bits = []
+        for i in taken:
+            bits += [b for b in bits_list[i] if b not in bits]
         blocks.append((bits, [ops[i] for i in taken]))
-        pending = rest
+        done = set(taken)
+        pending = [i for i in pending if i not in done]
     return blocks
 
 
