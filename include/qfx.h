@@ -156,7 +156,8 @@ typedef struct {
 
 /* A register phase keeps, per lane, the 16 amplitudes spanned by its four
  * register bits in registers and applies its sub-ops there.  Sub-op records
- * are 4 ints {type, position code, coef index, mask}:
+ * are 4 ints {type, position code, coef index, mask}, starting at a multiple
+ * of 4 ints in srcs:
  *   1  2x2 matrix (4 coefs) on register bit `code`;
  *   7  real 2x2 matrix (real parts of the 4 coefs are read);
  *   8  2x2 matrix with real diagonal and imaginary off-diagonal (real parts

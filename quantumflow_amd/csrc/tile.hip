@@ -268,10 +268,9 @@ __device__ __forceinline__ void sub_swap(v2d (&v)[kRegs], unsigned cm) {
 // type and code below is one of the listed cases; the unreachable defaults
 // let the dispatch drop its do-nothing path, which would otherwise copy all
 // 16 register amplitudes on every sub-op.
-__device__ __forceinline__ void apply_sub(v2d (&v)[kRegs], const int* __restrict__ sub,
-                                          const v2d* coef) {
-  const int type = sub[0], code = sub[1];
-  const v2d* m = coef + sub[2];
+__device__ __forceinline__ void apply_sub(v2d (&v)[kRegs], const int4 sub, const v2d* coef) {
+  const int type = sub.x, code = sub.y;
+  const v2d* m = coef + sub.z;
   switch (type) {
     case SUB_D1:
       switch (code) {
@@ -356,7 +355,14 @@ __device__ __forceinline__ void op_rphase(v2d* lds, int T, const qfx_tile_op& op
     v2d v[kRegs];
 #pragma unroll
     for (int r = 0; r < kRegs; ++r) v[r] = lds[a[r]];
-    for (int i = 0; i < nsub; ++i) apply_sub(v, sub0 + 4 * i, coef);
+    // the next sub-op record is fetched while the current one runs
+    const int4* rec = reinterpret_cast<const int4*>(sub0);
+    int4 next = rec[0];
+    for (int i = 0; i < nsub; ++i) {
+      const int4 cur = next;
+      next = rec[i + 1 < nsub ? i + 1 : i];
+      apply_sub(v, cur, coef);
+    }
 #pragma unroll
     for (int r = 0; r < kRegs; ++r) lds[a[r]] = v[r];
   }
@@ -767,7 +773,7 @@ int qfx_tile_check_ops(int T, int block, int flags, const qfx_tile_op* ops, int 
         for (int i2 = 0; i2 < kRegBits; ++i2) found |= o.sorted[i2] == o.t[j];
         if (!found) return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: sorted != register bits");
       }
-      if (o.coef < 0 || o.src < 0 || o.src + 4 * o.coef > nsrcs)
+      if (o.coef < 0 || o.src < 0 || o.src % 4 != 0 || o.src + 4 * o.coef > nsrcs)
         return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: sub-ops out of range");
       for (int q = 0; q < o.coef; ++q) {
         const int* sub = srcs + o.src + 4 * q;

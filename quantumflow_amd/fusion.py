@@ -437,6 +437,8 @@ def _encode_phase(R, members, pos_index, T, coefs, subs):
     st = sorted(regs)
     for j in range(5):
         o.sorted[j] = st[j] if j < REG_BITS else 0
+    while len(subs) % 4:                    # records are read as aligned int4
+        subs.append(0)
     o.src = len(subs)
     out = []                                # pending sub-ops: [type, code, matrix/phases]
 
