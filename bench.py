@@ -57,6 +57,24 @@ def cpu_baseline(desc, n, ngates):
                       % (ngates, n, DEPTH, os.cpu_count(), len(os.sched_getaffinity(0)))}
 
 
+def measured_traffic(plan, n):
+    """HBM bytes per tile-pass launch from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE runs (scaled by a plain copy of known size), when
+    they were taken on this workload and plan; else None."""
+    path = os.path.join(HERE, "profiles", "tile_pass_traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if plan is None or (n, DEPTH, SEED) != (rec.get("qubits"), rec.get("depth"), rec.get("seed")):
+        return None
+    st = plan.stats()
+    if any(st.get(k) != v for k, v in rec.get("plan", {}).items()):
+        return None
+    return rec.get("traffic_bytes_per_launch")
+
+
 def fused_block_section(gates, qubits, work, widths=(3, 4, 5), reps=2):
     """The same circuit with every pass fused into dense 3..K-qubit blocks
     applied on the matrix cores (FP64 MFMA): gates/s, blocks/s, the tile
@@ -182,6 +200,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-qaoa", action="store_true")
     ap.add_argument("--no-mfma", action="store_true")
+    ap.add_argument("--no-unfused", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one plain device copy of the state (4 GiB read + 4 GiB "
+                         "written at 28 qubits) to scale HBM byte counters against")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -234,7 +256,8 @@ def main():
     kname, avg_ms, launches = prof.dominant()
     achieved = bytes_per_pass / (avg_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic(plan, n),
             "kernel": kname, "avg_ms": round(avg_ms, 4), "launches_timed": launches,
             "bytes_per_launch": bytes_per_pass}
 
@@ -255,16 +278,22 @@ def main():
     }
     if plan is not None:
         out["config"].update(plan.stats())
-    unf = fusion.KernelTimer()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    fusion.apply_gates_inplace(gates, qubits, work, unf)
-    torch.cuda.synchronize()
-    dtu = time.perf_counter() - t0
-    un_name, un_ms, _ = unf.dominant()
-    out["unfused"] = {"gates_per_s": round(len(gates) / dtu, 2),
-                      "dominant_kernel": un_name, "avg_ms": round(un_ms, 4),
-                      "achieved_GBs": round(bytes_per_pass / (un_ms * 1e-3) / 1e9, 1)}
+    if not args.no_unfused:
+        unf = fusion.KernelTimer()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fusion.apply_gates_inplace(gates, qubits, work, unf)
+        torch.cuda.synchronize()
+        dtu = time.perf_counter() - t0
+        un_name, un_ms, _ = unf.dominant()
+        out["unfused"] = {"gates_per_s": round(len(gates) / dtu, 2),
+                          "dominant_kernel": un_name, "avg_ms": round(un_ms, 4),
+                          "achieved_GBs": round(bytes_per_pass / (un_ms * 1e-3) / 1e9, 1)}
+    if args.copy_calibration:
+        spare = torch.empty_like(work)
+        spare.copy_(work)
+        torch.cuda.synchronize()
+        del spare
     if not args.no_mfma:
         out["fused_mfma_blocks"] = fused_block_section(gates, qubits, work)
     if not args.no_qaoa:
