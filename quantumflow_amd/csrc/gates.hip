@@ -6,6 +6,8 @@
 // amplitudes that differ only in the target bits) to one work item, so the
 // 16-byte complex128 loads of consecutive lanes are consecutive in memory
 // whenever the lowest index bits are not targets, and out == in is safe.
+// Amplitudes are streamed with nontemporal loads and stores (each is
+// touched once per gate).
 #include <cmath>
 #include <cstring>
 
@@ -39,7 +41,7 @@ __global__ __launch_bounds__(kBlock) void dense_kernel(double2* out, const doubl
     base[u] = insert_zeros<K>(g, a.sorted);
     if (g < ngroups) {
 #pragma unroll
-      for (int r = 0; r < D; ++r) v[u][r] = in[base[u] + a.off[r]];
+      for (int r = 0; r < D; ++r) v[u][r] = ld_stream(in + base[u] + a.off[r]);
     }
   }
 #pragma unroll
@@ -51,7 +53,7 @@ __global__ __launch_bounds__(kBlock) void dense_kernel(double2* out, const doubl
         double2 acc = make_double2(0.0, 0.0);
 #pragma unroll
         for (int c = 0; c < D; ++c) cmac(acc, a.m[r * D + c], v[u][c]);
-        out[base[u] + a.off[r]] = acc;
+        st_stream(out + base[u] + a.off[r], acc);
       }
     }
   }
@@ -81,7 +83,7 @@ __global__ __launch_bounds__(kBlock) void sparse_kernel(double2* out, const doub
     if (g < ngroups) {
 #pragma unroll
       for (int r = 0; r < D; ++r)
-        if (a.active & (1u << r)) v[u][r] = in[base[u] + a.srcoff[r]];
+        if (a.active & (1u << r)) v[u][r] = ld_stream(in + base[u] + a.srcoff[r]);
     }
   }
 #pragma unroll
@@ -92,7 +94,7 @@ __global__ __launch_bounds__(kBlock) void sparse_kernel(double2* out, const doub
       for (int r = 0; r < D; ++r) {
         if (a.active & (1u << r)) {
           const double2 x = (a.has_phase & (1u << r)) ? cmul(a.ph[r], v[u][r]) : v[u][r];
-          out[base[u] + a.off[r]] = x;
+          st_stream(out + base[u] + a.off[r], x);
         }
       }
     }

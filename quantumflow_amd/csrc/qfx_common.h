@@ -65,6 +65,13 @@ typedef double v2d __attribute__((ext_vector_type(2)));
 // nontemporal, so they do not displace reusable lines in the caches.
 __device__ __forceinline__ v2d ld_stream(const v2d* p) { return __builtin_nontemporal_load(p); }
 __device__ __forceinline__ void st_stream(v2d* p, v2d v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ double2 ld_stream(const double2* p) {
+  const v2d v = __builtin_nontemporal_load(reinterpret_cast<const v2d*>(p));
+  return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ void st_stream(double2* p, double2 v) {
+  __builtin_nontemporal_store(v2d{v.x, v.y}, reinterpret_cast<v2d*>(p));
+}
 
 __device__ __forceinline__ void cmac(v2d& acc, const v2d a, const v2d b) {
   acc.x = fma(a.x, b.x, acc.x);

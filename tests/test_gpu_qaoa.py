@@ -204,3 +204,37 @@ def test_plain_path_matches_engine(E):
         assert abs(v1 - vb[i].item()) <= 1e-10
         assert abs(v2.item() - vb[i].item()) <= 1e-10
         assert np.max(np.abs(g2.cpu().numpy() - gb[i].cpu().numpy())) <= 1e-10
+
+
+def test_engine_maxcut_on_standard_graph_sets(E):
+    """Exact MaxCut from the engine's cut tables equals the brute-force
+    fixture for all 100 standard graphs of every size 6..16."""
+    import json, os
+    from quantumflow_amd.datasets import load_stdgraphs
+    with open(os.path.join(os.path.dirname(__file__), "golden", "stdgraph_maxcut.json")) as f:
+        want = json.load(f)["sizes"]
+    graphs, best = [], []
+    for n in range(6, 17):
+        graphs += load_stdgraphs(n)
+        best += want[str(n)]["maxcut"]
+    print("maxcut batch of", len(graphs), "standard graphs", flush=True)
+    batch = E.QAOABatch(graphs, 1, mode=E.MODE_FORWARD)
+    assert np.array_equal(batch.maxcut, np.asarray(best, dtype=float))
+
+
+def test_engine_values_on_standard_graphs(E):
+    """QAOA values on standard 8- and 12-node graphs equal the gate-by-gate
+    oracle (qubo_circuit + Circuit.run) to 1e-10."""
+    from quantumflow_amd.datasets import load_stdgraphs
+    graphs = load_stdgraphs(8)[:6] + load_stdgraphs(12)[:6]
+    rng = np.random.default_rng(7)
+    p = 2
+    beta = rng.uniform(0, np.pi / 2, size=(len(graphs), p))
+    gamma = rng.uniform(0, np.pi, size=(len(graphs), p))
+    batch = E.QAOABatch(graphs, p, mode=E.MODE_FORWARD)
+    val, _ = batch.evaluate(beta, gamma)
+    batch.check()
+    val = val.cpu().numpy()
+    for i, g in enumerate(graphs):
+        ref = O.qaoa_expectation(g.number_of_nodes(), list(g.edges()), beta[i], gamma[i])
+        assert abs(val[i] - ref) <= 1e-10
