@@ -159,16 +159,19 @@ __device__ __forceinline__ void op_sparse(v2d* lds, int T, const qfx_tile_op& op
 //   SUB_D1X 2x2 matrix with real diagonal and imaginary off-diagonal
 //           (RX-like: real parts of m00, m11, imaginary parts of m01, m10)
 //   SUB_DIAG  16 phases (one per slot; exactly 1 where `mask` has no bit)
-//   SUB_CX  X on register bit t = code & 3 where every bit of the control
-//           mask code >> 2 is set: a pure register exchange (X, CNOT, CCNOT)
-//   SUB_SWAP  exchange of register bits (a<b) = pair `code & 7` where every
-//           bit of the control mask code >> 3 is set (SWAP, CSWAP)
-// Exchanges move amplitudes bit for bit; phases of monomial gates (ISWAP,
-// Y, ...) follow as a SUB_DIAG.  Dense 2-qubit gates run as LDS ops: a 4x4
+//   SUB_CX  X on register bit t = code & 3 for the slots whose bits under
+//           the control mask (code >> 2) & 15 equal (code >> 6) & 15: a pure
+//           register exchange (CNOT, CCNOT; controlled SWAP as three)
+// Uncontrolled exchanges (SWAP, X, the permutation of ISWAP and Y) never
+// reach the kernel: the host relabels which logical slot each register
+// holds, and the phase's store map (op.active: logical bit of each register
+// bit, two bits each; op.phase: flipped logical bits) writes register r to
+// its slot.  Exchanges move amplitudes bit for bit; phases of monomial gates
+// follow as a SUB_DIAG.  Dense 2-qubit gates run as LDS ops: a 4x4
 // matrix next to 16 register amplitudes would raise the register allocation
 // of the whole kernel past two waves per SIMD.
 enum SubKind : int {
-  SUB_D1 = 1, SUB_DIAG = 4, SUB_CX = 5, SUB_SWAP = 6, SUB_D1R = 7, SUB_D1X = 8
+  SUB_D1 = 1, SUB_DIAG = 4, SUB_CX = 5, SUB_D1R = 7, SUB_D1X = 8
 };
 constexpr int kRegBits = 4;
 constexpr int kRegs = 1 << kRegBits;
@@ -236,31 +239,19 @@ __device__ __forceinline__ void sub_diag(v2d (&v)[kRegs], const v2d* __restrict_
   }
 }
 
-// Exchanges with a runtime control mask: the pair structure (target bits)
-// is compile time, the control test a per-slot select, so no register is
-// ever indexed at run time.
+// Controlled exchange with runtime control bits: the pair structure (target
+// bit) is compile time, the control test a per-slot select, so no register
+// is ever indexed at run time.  Slot r's pair swaps when its control bits
+// (mask cm) equal cv.
 template <int T>
-__device__ __forceinline__ void sub_cx(v2d (&v)[kRegs], unsigned cm) {
+__device__ __forceinline__ void sub_cx(v2d (&v)[kRegs], unsigned cm, unsigned cv) {
 #pragma unroll
   for (int r = 0; r < kRegs; ++r) {
     if (r & (1 << T)) continue;
-    const bool on = ((unsigned)r & cm) == cm;
+    const bool on = ((unsigned)r & cm) == cv;
     const v2d x = v[r], y = v[r | (1 << T)];
     v[r] = on ? y : x;
     v[r | (1 << T)] = on ? x : y;
-  }
-}
-
-template <int A, int B>
-__device__ __forceinline__ void sub_swap(v2d (&v)[kRegs], unsigned cm) {
-#pragma unroll
-  for (int r = 0; r < kRegs; ++r) {
-    if (!(r & (1 << A)) || (r & (1 << B))) continue;
-    constexpr int d = (1 << A) ^ (1 << B);
-    const bool on = ((unsigned)r & cm) == cm;
-    const v2d x = v[r], y = v[r ^ d];
-    v[r] = on ? y : x;
-    v[r ^ d] = on ? x : y;
   }
 }
 
@@ -301,25 +292,12 @@ __device__ __forceinline__ void apply_sub(v2d (&v)[kRegs], const int4 sub, const
       break;
     case SUB_DIAG: sub_diag(v, m); break;
     case SUB_CX: {
-      const unsigned cm = (unsigned)code >> 2;
+      const unsigned cm = ((unsigned)code >> 2) & 15u, cv = ((unsigned)code >> 6) & 15u;
       switch (code & 3) {
-        case 0: sub_cx<0>(v, cm); break;
-        case 1: sub_cx<1>(v, cm); break;
-        case 2: sub_cx<2>(v, cm); break;
-        case 3: sub_cx<3>(v, cm); break;
-        default: __builtin_unreachable();
-      }
-      break;
-    }
-    case SUB_SWAP: {
-      const unsigned cm = (unsigned)code >> 3;
-      switch (code & 7) {
-        case 0: sub_swap<0, 1>(v, cm); break;
-        case 1: sub_swap<0, 2>(v, cm); break;
-        case 2: sub_swap<0, 3>(v, cm); break;
-        case 3: sub_swap<1, 2>(v, cm); break;
-        case 4: sub_swap<1, 3>(v, cm); break;
-        case 5: sub_swap<2, 3>(v, cm); break;
+        case 0: sub_cx<0>(v, cm, cv); break;
+        case 1: sub_cx<1>(v, cm, cv); break;
+        case 2: sub_cx<2>(v, cm, cv); break;
+        case 3: sub_cx<3>(v, cm, cv); break;
         default: __builtin_unreachable();
       }
       break;
@@ -339,6 +317,12 @@ __device__ __forceinline__ void op_rphase(v2d* lds, int T, const qfx_tile_op& op
   // the set bits j of r), w[j] = sw(1 << t[j]), built up as a XOR tree
   const unsigned w0 = sw(1u << op.t[0]), w1 = sw(1u << op.t[1]);
   const unsigned w2 = sw(1u << op.t[2]), w3 = sw(1u << op.t[3]);
+  auto wl = [&](unsigned lbit) { return lbit == 0 ? w0 : lbit == 1 ? w1 : lbit == 2 ? w2 : w3; };
+  const unsigned perm = op.active, fl = op.phase;
+  const unsigned s0 = wl(perm & 3u), s1 = wl((perm >> 2) & 3u);
+  const unsigned s2 = wl((perm >> 4) & 3u), s3 = wl((perm >> 6) & 3u);
+  const unsigned cflip = ((fl & 1u) ? w0 : 0u) ^ ((fl & 2u) ? w1 : 0u) ^ ((fl & 4u) ? w2 : 0u) ^
+                         ((fl & 8u) ? w3 : 0u);
   for (int g = threadIdx.x; g < ngroups; g += BS) {
     unsigned lb = (unsigned)g;
 #pragma unroll
@@ -363,6 +347,16 @@ __device__ __forceinline__ void op_rphase(v2d* lds, int T, const qfx_tile_op& op
       next = rec[i + 1 < nsub ? i + 1 : i];
       apply_sub(v, cur, coef);
     }
+    // store map: register bit j holds logical bit (active >> 2j) & 3, and
+    // the logical bits in `phase` are flipped
+    a[0] = a[0] ^ cflip;
+    a[1] = a[0] ^ s0;
+    a[2] = a[0] ^ s1;
+    a[3] = a[1] ^ s1;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) a[4 + r] = a[r] ^ s2;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) a[8 + r] = a[r] ^ s3;
 #pragma unroll
     for (int r = 0; r < kRegs; ++r) lds[a[r]] = v[r];
   }
@@ -773,23 +767,25 @@ int qfx_tile_check_ops(int T, int block, int flags, const qfx_tile_op* ops, int 
         for (int i2 = 0; i2 < kRegBits; ++i2) found |= o.sorted[i2] == o.t[j];
         if (!found) return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: sorted != register bits");
       }
+      unsigned seen = 0;
+      for (int j = 0; j < kRegBits; ++j) seen |= 1u << ((o.active >> (2 * j)) & 3u);
+      if ((o.active >> (2 * kRegBits)) != 0 || seen != 15u || o.phase >= 16u)
+        return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: bad register-phase store map");
       if (o.coef < 0 || o.src < 0 || o.src % 4 != 0 || o.src + 4 * o.coef > nsrcs)
         return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: sub-ops out of range");
       for (int q = 0; q < o.coef; ++q) {
         const int* sub = srcs + o.src + 4 * q;
-        // codes per type: D1/D1R/D1X 4, DIAG 1, CX 4 targets x 16 masks,
-        // SWAP 6 pairs x 16 masks; control masks may not touch the targets
-        static const int ncode[9] = {0, 4, 0, 0, 1, 64, 128, 4, 4};
+        // codes per type: D1/D1R/D1X 4, DIAG 1, CX 4 targets x 16 control
+        // masks x 16 control values; control masks may not touch the target
+        static const int ncode[9] = {0, 4, 0, 0, 1, 1024, 0, 4, 4};
         static const int need[9] = {0, 4, 0, 0, kRegs, 0, 0, 4, 4};
         const int ty = sub[0], cd = sub[1];
         if (ty < SUB_D1 || ty > SUB_D1X || ncode[ty] == 0 || cd < 0 || cd >= ncode[ty])
           return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: bad sub-op");
-        if (ty == SUB_CX && (((cd >> 2) >> (cd & 3)) & 1))
-          return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: CX control on its target");
-        if (ty == SUB_SWAP) {
-          static const int pa[6] = {0, 0, 0, 1, 1, 2}, pb[6] = {1, 2, 3, 2, 3, 3};
-          if ((cd & 7) > 5 || (((cd >> 3) >> pa[cd & 7]) & 1) || (((cd >> 3) >> pb[cd & 7]) & 1))
-            return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: bad SWAP sub-op");
+        if (ty == SUB_CX) {
+          const int cm = (cd >> 2) & 15, cv = (cd >> 6) & 15;
+          if (((cm >> (cd & 3)) & 1) || (cv & ~cm))
+            return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: bad CX control bits");
         }
         if (need[ty] && (sub[2] < 0 || sub[2] + need[ty] > ncoef))
           return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: sub-op coefficients out of range");

@@ -38,7 +38,7 @@ from . import backend as bk
 from .states import State
 
 OP_DENSE, OP_SPARSE, OP_RPHASE, OP_MFMA = 0, 1, 2, 3
-SUB_D1, SUB_DIAG, SUB_CX, SUB_SWAP, SUB_D1R, SUB_D1X = 1, 4, 5, 6, 7, 8
+SUB_D1, SUB_DIAG, SUB_CX, SUB_D1R, SUB_D1X = 1, 4, 5, 7, 8
 REG_BITS = 4
 MAX_TILE_BITS = 13
 MIN_TILE_BITS = 6
@@ -46,8 +46,6 @@ TILE_WIDE = 1
 TILE_PERSISTENT = 2
 TILE_MFMA = 4
 MFMA_MIN_BITS = 3
-# position codes of the register-phase kernels (lexicographic)
-_PAIRS = [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]
 
 
 class KernelTimer:
@@ -420,14 +418,23 @@ def d1_variant(m):
 
 
 def _encode_phase(R, members, pos_index, T, coefs, subs):
-    """qfx_tile_op of a register phase over tile bits R plus its sub-ops."""
+    """qfx_tile_op of a register phase over tile bits R plus its sub-ops.
+
+    Logical bit A of the phase is tile bit regs[A]; register i starts out
+    holding logical index i.  Uncontrolled exchanges (SWAP, X and the
+    permutation part of ISWAP, Y) are not executed: they relabel which
+    logical index each register holds, phi(i) = P(i) xor c with P a bit
+    permutation (q[A] = register bit of logical bit A), and later sub-ops
+    are translated through phi.  The op's store map writes register i to
+    the slot of logical phi(i): ``active`` holds the logical bit of each
+    register bit (2 bits per register bit), ``phase`` the flip mask c."""
     tl = sorted(pos_index[b] for b in R)
     for j in range(T):                      # pad to 4 register bits
         if len(tl) >= REG_BITS:
             break
         if j not in tl:
             tl.append(j)
-    regs = tl[:REG_BITS]                    # register bit j <-> tile bit regs[j]
+    regs = tl[:REG_BITS]                    # logical bit A <-> tile bit regs[A]
     reg_of = {t: j for j, t in enumerate(regs)}
     o = _native.QfxTileOp()
     o.kind = OP_RPHASE
@@ -441,16 +448,48 @@ def _encode_phase(R, members, pos_index, T, coefs, subs):
         subs.append(0)
     o.src = len(subs)
     out = []                                # pending sub-ops: [type, code, matrix/phases]
+    q = list(range(REG_BITS))               # register bit holding logical bit A
+    flip = [0]                              # logical bits flipped (c)
 
-    def push_diag(ph):
-        if out and out[-1][0] == SUB_DIAG:
-            out[-1][2] = out[-1][2] * ph
-        else:
-            out.append([SUB_DIAG, 0, ph])
+    def phi(i):
+        x = 0
+        for a in range(REG_BITS):
+            x |= ((i >> q[a]) & 1) << a
+        return x ^ flip[0]
+
+    # Diagonal factors wait in `pending` (register space) and are emitted as
+    # one SUB_DIAG only before a 1-qubit sub-op on a bit they depend on, or
+    # at the end: an exchange E moves past a pending diagonal D as
+    # E D = D' E with D'[i] = D[E(i)].
+    pending = [None]
+    regs16 = np.arange(2 ** REG_BITS)
+
+    def push_diag(ph_log):
+        ph = np.array([ph_log[phi(i)] for i in range(2 ** REG_BITS)])
+        pending[0] = ph if pending[0] is None else pending[0] * ph
+
+    def flush(bit=None):
+        d = pending[0]
+        if d is None or (bit is not None and np.array_equal(d, d[regs16 ^ (1 << bit)])):
+            return
+        out.append([SUB_DIAG, 0, d])
+        pending[0] = None
+
+    def push_cx(t, ctrls):
+        if not ctrls:                       # X: relabel only
+            flip[0] ^= 1 << t
+            return
+        cm = cv = 0
+        for b in ctrls:
+            cm |= 1 << q[b]
+            cv |= (1 ^ ((flip[0] >> b) & 1)) << q[b]
+        if pending[0] is not None:
+            e = np.where((regs16 & cm) == cv, regs16 ^ (1 << q[t]), regs16)
+            pending[0] = pending[0][e]
+        out.append([SUB_CX, q[t] | (cm << 2) | (cv << 6), None])
 
     for op in members:
-        p = [reg_of[pos_index[b]] for b in op.bits]        # register bit per gate qubit
-        k = len(p)
+        p = [reg_of[pos_index[b]] for b in op.bits]        # logical bit per gate qubit
         kd = op.kind
         if op.matrix is None and kd.kind == bk.GateKind.DIAGONAL:
             push_diag(_slot_phases(kd.diag, p))
@@ -458,24 +497,31 @@ def _encode_phase(R, members, pos_index, T, coefs, subs):
         pat = exchange_pattern(kd) if op.matrix is None else None
         if pat is not None:
             if pat[0] == 'cx':
-                cm = 0
-                for c in pat[2]:
-                    cm |= 1 << p[c]
-                out.append([SUB_CX, p[pat[1]] | (cm << 2), None])
+                push_cx(p[pat[1]], [p[c] for c in pat[2]])
             else:
-                a, b = sorted((p[pat[1]], p[pat[2]]))
-                cm = 0
-                for c in pat[3]:
-                    cm |= 1 << p[c]
-                out.append([SUB_SWAP, _PAIRS.index((a, b)) | (cm << 3), None])
+                a, b = p[pat[1]], p[pat[2]]
+                ctrls = [p[c] for c in pat[3]]
+                if not ctrls:               # SWAP: relabel only
+                    q[a], q[b] = q[b], q[a]
+                    if ((flip[0] >> a) ^ (flip[0] >> b)) & 1:
+                        flip[0] ^= (1 << a) | (1 << b)
+                else:                       # controlled SWAP as three exchanges
+                    push_cx(a, [b])
+                    push_cx(b, ctrls + [a])
+                    push_cx(a, [b])
             if kd.phase is not None:
                 push_diag(_slot_phases(kd.phase, p))
             continue
         m = np.asarray(op.dense_matrix(), dtype=np.complex128)     # 1-qubit
-        if out and out[-1][0] == SUB_D1 and out[-1][1] == p[0]:
+        if (flip[0] >> p[0]) & 1:          # register pair holds (1, 0): X m X
+            m = m[::-1, ::-1].copy()
+        j = q[p[0]]
+        flush(j)
+        if out and out[-1][0] == SUB_D1 and out[-1][1] == j:
             out[-1][2] = m @ out[-1][2]                     # same bit: one 2x2
             continue
-        out.append([SUB_D1, p[0], m])
+        out.append([SUB_D1, j, m])
+    flush()
     nsub = 0
     for typ, code, data in out:
         if typ == SUB_D1:
@@ -489,15 +535,18 @@ def _encode_phase(R, members, pos_index, T, coefs, subs):
                 continue
             subs.extend([SUB_DIAG, 0, len(coefs), mask])
             coefs.extend(data)
-        elif typ in (SUB_CX, SUB_SWAP):
+        elif typ == SUB_CX:
             subs.extend([typ, code, 0, 0])
         else:
             subs.extend([typ, code, len(coefs), 0])
             coefs.extend(np.asarray(data, dtype=np.complex128).reshape(-1))
         nsub += 1
     o.coef = nsub
-    o.active = o.phase = 0
+    logical_of = [q.index(j) for j in range(REG_BITS)]     # logical bit of register bit j
+    o.active = sum(a << (2 * j) for j, a in enumerate(logical_of))
+    o.phase = flip[0]
     return o
+
 
 def plan_blocks(ops, width):
     """Group a pass's ops (in order) into dense blocks of <= width qubits,

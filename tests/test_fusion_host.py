@@ -132,17 +132,11 @@ def _emulate_ops(ket, n, pos, ops, coefs, srcs):
                     ph = np.where([(mask >> r) & 1 for r in range(16)], coefs[c0:c0 + 16], 1.0)
                     u = np.diag(ph) @ u
                     continue
-                if typ in (fusion.SUB_CX, fusion.SUB_SWAP):
+                if typ == fusion.SUB_CX:
                     perm = np.zeros((16, 16))
+                    t, cm, cv = code & 3, (code >> 2) & 15, (code >> 6) & 15
                     for r in range(16):
-                        if typ == fusion.SUB_CX:
-                            t, cm = code & 3, code >> 2
-                            dst = r ^ (1 << t) if (r & cm) == cm else r
-                        else:
-                            a, b = d2[code & 7]
-                            cm = code >> 3
-                            on = (r & cm) == cm and ((r >> a) & 1) != ((r >> b) & 1)
-                            dst = r ^ (1 << a) ^ (1 << b) if on else r
+                        dst = r ^ (1 << t) if (r & cm) == cv else r
                         perm[dst, r] = 1
                     u = perm @ u
                     continue
@@ -160,6 +154,14 @@ def _emulate_ops(ket, n, pos, ops, coefs, srcs):
                                   [1j * m[1, 0].imag, m[1, 1].real]])
                 axes = [3 - b for b in reversed(bits)]      # gate MSB <-> highest bit
                 u = O.full_operator(m, 4, axes) @ u
+            # store map: register i goes to the slot of logical index phi(i)
+            store = np.zeros((16, 16))
+            for r in range(16):
+                x = 0
+                for j in range(4):
+                    x |= ((r >> j) & 1) << ((o.active >> (2 * j)) & 3)
+                store[x ^ o.phase, r] = 1
+            u = store @ u
             axes = [n - 1 - gb[j] for j in (3, 2, 1, 0)]
             ket = O.tensormul(u.reshape([2] * 8), ket, axes)
         else:
@@ -214,6 +216,36 @@ def test_exchange_patterns():
     assert fusion.exchange_pattern(sg.ISWAP().kind) == ('swap', 0, 1, [])
     assert fusion.exchange_pattern(sg.Y().kind) == ('cx', 0, [])
     assert fusion.exchange_pattern(sg.H().kind) is None
+
+
+def test_uncontrolled_exchanges_cost_no_sub_ops():
+    """SWAP, X and ISWAP's permutation become relabelings: a phase of only
+    SWAP/X/ISWAP gates holds at most the merged ISWAP phases."""
+    n, T = 8, 8
+    desc = [('SWAP', (), (0, 1)), ('X', (), (2,)), ('ISWAP', (), (1, 2)), ('SWAP', (), (2, 3)),
+            ('X', (), (0,)), ('ISWAP', (), (0, 3))]
+    gates = build_gates(desc)
+    plan = fusion.HostPlan(gates, tuple(range(n)), tile_bits=T, low_bits=3, block=256,
+                           fuse_width=1, mode='reg')
+    ref = O.run_circuit(desc, qubits=tuple(range(n)))
+    ket = O.zero_state(n)
+    rng = np.random.default_rng(3)
+    psi = rng.normal(size=2 ** n) + 1j * rng.normal(size=2 ** n)
+    psi = (psi / np.linalg.norm(psi)).reshape([2] * n)
+    ref = O.run_circuit(desc, qubits=tuple(range(n)), ket=psi)
+    ket = psi
+    nsub = 0
+    for kind, payload in plan.passes:
+        pos, start, nops, block, flags, c_lo, ncoef = payload
+        ops = plan.ops_host[start:start + nops]
+        for o in ops:
+            nsub += o.coef
+            for q in range(o.coef):
+                assert plan.srcs[o.src + 4 * q] == fusion.SUB_DIAG
+        ket = _emulate_ops(ket, n, pos, ops, plan.coefs[c_lo:c_lo + ncoef], plan.srcs)
+    assert nsub <= 2
+    assert np.array_equal(np.abs(ket), np.abs(ref))
+    assert np.max(np.abs(ket - ref)) < 1e-15
 
 
 @pytest.mark.parametrize("seed", [5, 6])
