@@ -164,10 +164,12 @@ __device__ __forceinline__ void op_sparse(v2d* lds, int T, const qfx_tile_op& op
 //           register exchange (CNOT, CCNOT; controlled SWAP as three)
 // Uncontrolled exchanges (SWAP, X, the permutation of ISWAP and Y) never
 // reach the kernel: the host relabels which logical slot each register
-// holds, and the phase's store map (op.active: logical bit of each register
-// bit, two bits each; op.phase: flipped logical bits) writes register r to
-// its slot.  Exchanges move amplitudes bit for bit; phases of monomial gates
-// follow as a SUB_DIAG.  Dense 2-qubit gates run as LDS ops: a 4x4
+// holds, and a leading run of SWAP/X/CNOT gates is folded into the load.
+// The phase's load map (op.phase) and store map (op.active) are affine maps
+// of the 4-bit logical index, matrix | offset << 16 with column j the image
+// of bit j: register r loads from and stores to the slots of their images.
+// Exchanges move amplitudes bit for bit; phases of monomial gates follow as
+// a SUB_DIAG.  Dense 2-qubit gates run as LDS ops: a 4x4
 // matrix next to 16 register amplitudes would raise the register allocation
 // of the whole kernel past two waves per SIMD.
 enum SubKind : int {
@@ -317,25 +319,32 @@ __device__ __forceinline__ void op_rphase(v2d* lds, int T, const qfx_tile_op& op
   // the set bits j of r), w[j] = sw(1 << t[j]), built up as a XOR tree
   const unsigned w0 = sw(1u << op.t[0]), w1 = sw(1u << op.t[1]);
   const unsigned w2 = sw(1u << op.t[2]), w3 = sw(1u << op.t[3]);
-  auto wl = [&](unsigned lbit) { return lbit == 0 ? w0 : lbit == 1 ? w1 : lbit == 2 ? w2 : w3; };
-  const unsigned perm = op.active, fl = op.phase;
-  const unsigned s0 = wl(perm & 3u), s1 = wl((perm >> 2) & 3u);
-  const unsigned s2 = wl((perm >> 4) & 3u), s3 = wl((perm >> 6) & 3u);
-  const unsigned cflip = ((fl & 1u) ? w0 : 0u) ^ ((fl & 2u) ? w1 : 0u) ^ ((fl & 4u) ? w2 : 0u) ^
-                         ((fl & 8u) ? w3 : 0u);
+  // LDS offset of logical index x: XOR of w_k over the set bits k of x
+  auto off = [&](unsigned x) {
+    return ((x & 1u) ? w0 : 0u) ^ ((x & 2u) ? w1 : 0u) ^ ((x & 4u) ? w2 : 0u) ^
+           ((x & 8u) ? w3 : 0u);
+  };
+  // affine register maps (matrix | offset << 16, column j = image of bit j):
+  // load map op.phase, store map op.active
+  const unsigned lm = op.phase, sm = op.active;
+  const unsigned l0 = off(lm & 15u), l1 = off((lm >> 4) & 15u), l2 = off((lm >> 8) & 15u),
+                 l3 = off((lm >> 12) & 15u), lc = off(lm >> 16);
+  const unsigned s0 = off(sm & 15u), s1 = off((sm >> 4) & 15u), s2 = off((sm >> 8) & 15u),
+                 s3 = off((sm >> 12) & 15u), sc = off(sm >> 16);
   for (int g = threadIdx.x; g < ngroups; g += BS) {
     unsigned lb = (unsigned)g;
 #pragma unroll
     for (int j = 0; j < kRegBits; ++j) lb = ins0(lb, op.sorted[j]);
+    const unsigned slb = sw(lb);
     unsigned a[kRegs];
-    a[0] = sw(lb);
-    a[1] = a[0] ^ w0;
-    a[2] = a[0] ^ w1;
-    a[3] = a[1] ^ w1;
+    a[0] = slb ^ lc;
+    a[1] = a[0] ^ l0;
+    a[2] = a[0] ^ l1;
+    a[3] = a[1] ^ l1;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) a[4 + r] = a[r] ^ w2;
+    for (int r = 0; r < 4; ++r) a[4 + r] = a[r] ^ l2;
 #pragma unroll
-    for (int r = 0; r < 8; ++r) a[8 + r] = a[r] ^ w3;
+    for (int r = 0; r < 8; ++r) a[8 + r] = a[r] ^ l3;
     v2d v[kRegs];
 #pragma unroll
     for (int r = 0; r < kRegs; ++r) v[r] = lds[a[r]];
@@ -347,9 +356,7 @@ __device__ __forceinline__ void op_rphase(v2d* lds, int T, const qfx_tile_op& op
       next = rec[i + 1 < nsub ? i + 1 : i];
       apply_sub(v, cur, coef);
     }
-    // store map: register bit j holds logical bit (active >> 2j) & 3, and
-    // the logical bits in `phase` are flipped
-    a[0] = a[0] ^ cflip;
+    a[0] = slb ^ sc;
     a[1] = a[0] ^ s0;
     a[2] = a[0] ^ s1;
     a[3] = a[1] ^ s1;
@@ -767,10 +774,19 @@ int qfx_tile_check_ops(int T, int block, int flags, const qfx_tile_op* ops, int 
         for (int i2 = 0; i2 < kRegBits; ++i2) found |= o.sorted[i2] == o.t[j];
         if (!found) return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: sorted != register bits");
       }
-      unsigned seen = 0;
-      for (int j = 0; j < kRegBits; ++j) seen |= 1u << ((o.active >> (2 * j)) & 3u);
-      if ((o.active >> (2 * kRegBits)) != 0 || seen != 15u || o.phase >= 16u)
-        return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: bad register-phase store map");
+      // load and store maps: affine, invertible (4 independent columns)
+      for (const uint32_t m : {o.phase, o.active}) {
+        unsigned span = 1u;                   // bit x set: x is in the column span
+        for (int j = 0; j < kRegBits; ++j) {
+          const unsigned col = (m >> (4 * j)) & 15u;
+          unsigned next = span;
+          for (unsigned x = 0; x < 16; ++x)
+            if ((span >> x) & 1u) next |= 1u << (x ^ col);
+          span = next;
+        }
+        if ((m >> 20) != 0 || span != 0xFFFFu)
+          return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: bad register-phase load/store map");
+      }
       if (o.coef < 0 || o.src < 0 || o.src % 4 != 0 || o.src + 4 * o.coef > nsrcs)
         return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: sub-ops out of range");
       for (int q = 0; q < o.coef; ++q) {

@@ -426,8 +426,10 @@ def _encode_phase(R, members, pos_index, T, coefs, subs):
     logical index each register holds, phi(i) = P(i) xor c with P a bit
     permutation (q[A] = register bit of logical bit A), and later sub-ops
     are translated through phi.  The op's store map writes register i to
-    the slot of logical phi(i): ``active`` holds the logical bit of each
-    register bit (2 bits per register bit), ``phase`` the flip mask c."""
+    the slot of logical phi(i), and a leading run of SWAP/X/CNOT gates and
+    diagonals is folded into the load map.  Both maps are affine on the
+    4-bit logical index and are stored as matrix | offset << 16 (column j
+    = image of bit j): ``active`` the store map, ``phase`` the load map."""
     tl = sorted(pos_index[b] for b in R)
     for j in range(T):                      # pad to 4 register bits
         if len(tl) >= REG_BITS:
@@ -488,7 +490,42 @@ def _encode_phase(R, members, pos_index, T, coefs, subs):
             pending[0] = pending[0][e]
         out.append([SUB_CX, q[t] | (cm << 2) | (cv << 6), None])
 
+    # Leading exchanges with at most one control (SWAP, X, CNOT: affine maps
+    # of the logical index) and diagonals are folded into the load map:
+    # register i loads logical slot E^-1(i), so the registers start out
+    # holding E psi.  Diagonals met on the way are carried past E.
+    k0 = 0
     for op in members:
+        p = [reg_of[pos_index[b]] for b in op.bits]
+        kd = op.kind
+        if op.matrix is None and kd.kind == bk.GateKind.DIAGONAL:
+            k0 += 1
+            continue
+        pat = exchange_pattern(kd) if op.matrix is None else None
+        if pat is None or len(pat[-1]) > 1 or (pat[0] == 'swap' and pat[-1]):
+            break
+        k0 += 1
+    e_map = list(range(2 ** REG_BITS))      # E as a table, logical -> logical
+    lead_diag = np.ones(2 ** REG_BITS, dtype=np.complex128)
+    for op in members[:k0]:
+        p = [reg_of[pos_index[b]] for b in op.bits]
+        kd = op.kind
+        if kd.kind == bk.GateKind.DIAGONAL:
+            # D applied after E so far: multiply by D at the current index
+            lead_diag = lead_diag * _slot_phases(kd.diag, p)
+            continue
+        g = _gate_perm(kd, p)
+        e_map = [g[x] for x in e_map]       # x -> g(E(x))
+        # a diagonal before g becomes D'(y) = D(g^-1 y) after it
+        ginv = np.argsort(g)
+        lead_diag = lead_diag[ginv]
+        if kd.phase is not None:
+            lead_diag = lead_diag * _slot_phases(kd.phase, p)
+    load_inv = np.argsort(e_map)            # register i loads logical E^-1(i)
+    if np.any(lead_diag != 1.0):
+        push_diag(lead_diag)
+
+    for op in members[k0:]:
         p = [reg_of[pos_index[b]] for b in op.bits]        # logical bit per gate qubit
         kd = op.kind
         if op.matrix is None and kd.kind == bk.GateKind.DIAGONAL:
@@ -542,10 +579,45 @@ def _encode_phase(R, members, pos_index, T, coefs, subs):
             coefs.extend(np.asarray(data, dtype=np.complex128).reshape(-1))
         nsub += 1
     o.coef = nsub
-    logical_of = [q.index(j) for j in range(REG_BITS)]     # logical bit of register bit j
-    o.active = sum(a << (2 * j) for j, a in enumerate(logical_of))
-    o.phase = flip[0]
+    o.active = _affine_code([phi(i) for i in range(2 ** REG_BITS)])   # store map
+    o.phase = _affine_code(list(load_inv))                            # load map
     return o
+
+
+def _gate_perm(kind, p):
+    """Logical-index permutation (table over the 4 phase bits) of a monomial
+    gate whose qubit j is logical bit p[j]: out index = table[in index]."""
+    k = len(p)
+    table = []
+    for x in range(2 ** REG_BITS):
+        gi = 0
+        for j in range(k):
+            gi |= ((x >> p[j]) & 1) << (k - 1 - j)
+        # the gate maps input row src[r] to output row r
+        r = int(np.nonzero(kind.src == gi)[0][0])
+        y = x
+        for j in range(k):
+            y = (y & ~(1 << p[j])) | (((r >> (k - 1 - j)) & 1) << p[j])
+        table.append(y)
+    return np.asarray(table)
+
+
+def _affine_code(table):
+    """Encode an affine map of 4-bit indices (given as a table) as
+    matrix | offset << 16: column j (bits 4j..4j+3) = image of bit j xor the
+    image of 0.  Raises if the table is not affine and invertible."""
+    c = int(table[0])
+    cols = [int(table[1 << j]) ^ c for j in range(REG_BITS)]
+    for x in range(2 ** REG_BITS):
+        y = c
+        for j in range(REG_BITS):
+            if (x >> j) & 1:
+                y ^= cols[j]
+        if y != table[x]:
+            raise ValueError('register map is not affine')
+    if len(set(int(v) for v in table)) != 2 ** REG_BITS:
+        raise ValueError('register map is not invertible')
+    return sum(col << (4 * j) for j, col in enumerate(cols)) | (c << 16)
 
 
 def plan_blocks(ops, width):

@@ -154,14 +154,20 @@ def _emulate_ops(ket, n, pos, ops, coefs, srcs):
                                   [1j * m[1, 0].imag, m[1, 1].real]])
                 axes = [3 - b for b in reversed(bits)]      # gate MSB <-> highest bit
                 u = O.full_operator(m, 4, axes) @ u
-            # store map: register i goes to the slot of logical index phi(i)
+            # load map: register r starts with logical slot L(r); store map:
+            # register r goes to logical slot S(r)
+            def affine(m, r):
+                y = m >> 16
+                for j in range(4):
+                    if (r >> j) & 1:
+                        y ^= (m >> (4 * j)) & 15
+                return y
+            load = np.zeros((16, 16))
             store = np.zeros((16, 16))
             for r in range(16):
-                x = 0
-                for j in range(4):
-                    x |= ((r >> j) & 1) << ((o.active >> (2 * j)) & 3)
-                store[x ^ o.phase, r] = 1
-            u = store @ u
+                load[r, affine(o.phase, r)] = 1
+                store[affine(o.active, r), r] = 1
+            u = store @ u @ load
             axes = [n - 1 - gb[j] for j in (3, 2, 1, 0)]
             ket = O.tensormul(u.reshape([2] * 8), ket, axes)
         else:
