@@ -196,11 +196,14 @@ int qfx_tile_check_ops(int T, int block, int flags, const qfx_tile_op* ops /*hos
  * for this op list.  ops/coef/srcs in device memory; coef holds the op
  * list's ncoef coefficients (indices in the ops are relative to coef).  A
  * pass without QFX_TILE_WIDE or QFX_TILE_MFMA takes at most 512
- * coefficients: it stages them in LDS. */
+ * coefficients: it stages them in LDS.  Tile bit j is loaded from index bit
+ * tile_pos[j] and stored to index bit store_pos[j] (a permutation of
+ * tile_pos; NULL: the same), so a pass can end with a relabelling of its
+ * qubits (the SWAP gates it absorbed). */
 int qfx_tile_pass(void* stream, void* state, int n, int tile_bits,
-                  const int* tile_pos /*host*/, const void* ops, int nops,
-                  const void* coef, int ncoef, const void* srcs, int block,
-                  int flags);
+                  const int* tile_pos /*host*/, const int* store_pos /*host or NULL*/,
+                  const void* ops, int nops, const void* coef, int ncoef,
+                  const void* srcs, int block, int flags);
 
 /* ---- QAOA MaxCut (quantumflow/qaoa.py) --------------------------------- */
 

@@ -75,7 +75,7 @@ SIGNATURES = {
     'qfx_mlp_adam': (_i, [_vp, _vp, _vp, _vp, _vp, _i64, ctypes.c_float, ctypes.c_float,
                           ctypes.c_float, ctypes.c_float, _i, ctypes.c_float]),
     'qfx_tile_check_ops': (_i, [_i, _i, _i, _vp, _i, _i, _ip, _i]),
-    'qfx_tile_pass': (_i, [_vp, _vp, _i, _i, _ip, _vp, _i, _vp, _i, _vp, _i, _i]),
+    'qfx_tile_pass': (_i, [_vp, _vp, _i, _i, _ip, _ip, _vp, _i, _vp, _i, _vp, _i, _i]),
 }
 
 _lib = None

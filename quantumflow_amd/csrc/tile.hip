@@ -32,7 +32,8 @@ constexpr int kMfmaMaxBlock = 512;    // largest block running matrix-core block
 constexpr int kMaxStaged = 512;
 
 struct TilePos {
-  int s[QFX_MAX_QUBITS];  // tile bit positions, ascending
+  int s[QFX_MAX_QUBITS];  // tile bit positions, ascending (where tile bit j loads from)
+  int d[kMaxTileBits];    // where tile bit j is stored (a permutation of s[0..T-1])
 };
 
 __device__ __forceinline__ unsigned ins0(unsigned g, int b) {
@@ -495,11 +496,11 @@ __device__ __forceinline__ void apply_op(v2d* lds, int T, const qfx_tile_op& op,
 // Global offset of the tile bits above the lane bits: bit j of kk sits at
 // index bit pos.s[LB + j].
 template <int LB>
-__device__ __forceinline__ uint64_t high_off(int kk, int T, const TilePos& pos) {
+__device__ __forceinline__ uint64_t high_off(int kk, int T, const int* bitpos) {
   uint64_t o = 0;
 #pragma unroll
   for (int j = 0; j < kMaxTileBits - LB; ++j)
-    if (j < T - LB) o |= (uint64_t)((kk >> j) & 1) << pos.s[LB + j];
+    if (j < T - LB) o |= (uint64_t)((kk >> j) & 1) << bitpos[LB + j];
   return o;
 }
 
@@ -530,9 +531,12 @@ __global__ __launch_bounds__(BS) void tile_pass_kernel(v2d* state, int T, TilePo
       base = ((base >> b) << (b + 1)) | (base & (((uint64_t)1 << b) - 1));
     }
   }
-  uint64_t toff = 0;
+  uint64_t toff = 0, soff = 0;
 #pragma unroll
-  for (int j = 0; j < LB; ++j) toff |= (uint64_t)((tid >> j) & 1) << pos.s[j];
+  for (int j = 0; j < LB; ++j) {
+    toff |= (uint64_t)((tid >> j) & 1) << pos.s[j];
+    soff |= (uint64_t)((tid >> j) & 1) << pos.d[j];
+  }
   const int per = 1 << (T - LB);   // amplitudes per lane
   const v2d* src = state + base + toff;
   // issue kLoadBatch global loads before the first LDS write
@@ -542,7 +546,7 @@ __global__ __launch_bounds__(BS) void tile_pass_kernel(v2d* state, int T, TilePo
     for (int u = 0; u < kLoadBatch; ++u) {
       const int kk = k0 + u;
       if (kk < per) {
-        v[u] = ld_stream(src + high_off<LB>(kk, T, pos));
+        v[u] = ld_stream(src + high_off<LB>(kk, T, pos.s));
       }
     }
 #pragma unroll
@@ -555,7 +559,7 @@ __global__ __launch_bounds__(BS) void tile_pass_kernel(v2d* state, int T, TilePo
     else apply_op<BS, WIDE, MF>(lds, T, ops[i], coef, coef, srcs);
     __syncthreads();
   }
-  v2d* dst = state + base + toff;
+  v2d* dst = state + base + soff;
   for (int k0 = 0; k0 < per; k0 += kLoadBatch) {
     v2d v[kLoadBatch];
 #pragma unroll
@@ -565,7 +569,7 @@ __global__ __launch_bounds__(BS) void tile_pass_kernel(v2d* state, int T, TilePo
     for (int u = 0; u < kLoadBatch; ++u) {
       const int kk = k0 + u;
       if (kk < per) {
-        st_stream(dst + high_off<LB>(kk, T, pos), v[u]);
+        st_stream(dst + high_off<LB>(kk, T, pos.d), v[u]);
       }
     }
   }
@@ -592,13 +596,18 @@ __global__ __launch_bounds__(BS) void tile_pipe_kernel(v2d* state, int T, TilePo
   const int tid = threadIdx.x;
   constexpr bool kStage = !WIDE && !MF;
   v2d* cf = kStage ? stage_coef<BS>(lds, T, coef, ncoef) : nullptr;
-  uint64_t toff = 0;
+  uint64_t toff = 0, soff = 0;
 #pragma unroll
-  for (int j = 0; j < LB; ++j) toff |= (uint64_t)((tid >> j) & 1) << pos.s[j];
+  for (int j = 0; j < LB; ++j) {
+    toff |= (uint64_t)((tid >> j) & 1) << pos.s[j];
+    soff |= (uint64_t)((tid >> j) & 1) << pos.d[j];
+  }
   // the four tile bits above the lane bits: amplitude u of a lane sits at
-  // h(u) = sum of hb[j] over the set bits j of u
+  // h(u) = sum of hb[j] over the set bits j of u (stores: pos.d, sb[j])
   const uint64_t hb0 = (uint64_t)1 << pos.s[LB], hb1 = (uint64_t)1 << pos.s[LB + 1];
   const uint64_t hb2 = (uint64_t)1 << pos.s[LB + 2], hb3 = (uint64_t)1 << pos.s[LB + 3];
+  const uint64_t sb0 = (uint64_t)1 << pos.d[LB], sb1 = (uint64_t)1 << pos.d[LB + 1];
+  const uint64_t sb2 = (uint64_t)1 << pos.d[LB + 2], sb3 = (uint64_t)1 << pos.d[LB + 3];
 
   auto tile_base = [&](uint32_t tile) {
     uint64_t base = tile;
@@ -609,10 +618,10 @@ __global__ __launch_bounds__(BS) void tile_pipe_kernel(v2d* state, int T, TilePo
         base = ((base >> b) << (b + 1)) | (base & (((uint64_t)1 << b) - 1));
       }
     }
-    return base + toff;
+    return base;
   };
   auto load_tile = [&](v2d (&pf)[kPrefetch], uint64_t base) {
-    const v2d* p0 = state + base;
+    const v2d* p0 = state + base + toff;
     const v2d* p1 = p0 + hb1;
 #pragma unroll
     for (int u = 0; u < kPrefetch; u += 4) {
@@ -641,15 +650,15 @@ __global__ __launch_bounds__(BS) void tile_pipe_kernel(v2d* state, int T, TilePo
       else apply_op<BS, WIDE, MF>(lds, T, ops[i], coef, coef, srcs);
       __syncthreads();
     }
-    v2d* q0 = state + base;
-    v2d* q1 = q0 + hb1;
+    v2d* q0 = state + base + soff;
+    v2d* q1 = q0 + sb1;
 #pragma unroll
     for (int u = 0; u < kPrefetch; u += 4) {
-      const uint64_t o = ((u & 4) ? hb2 : 0) + ((u & 8) ? hb3 : 0);
+      const uint64_t o = ((u & 4) ? sb2 : 0) + ((u & 8) ? sb3 : 0);
       st_stream(q0 + o, lds[sw(tid + u * BS)]);
-      st_stream(q0 + o + hb0, lds[sw(tid + (u + 1) * BS)]);
+      st_stream(q0 + o + sb0, lds[sw(tid + (u + 1) * BS)]);
       st_stream(q1 + o, lds[sw(tid + (u + 2) * BS)]);
-      st_stream(q1 + o + hb0, lds[sw(tid + (u + 3) * BS)]);
+      st_stream(q1 + o + sb0, lds[sw(tid + (u + 3) * BS)]);
     }
     base = nbase;
     __syncthreads();
@@ -840,10 +849,19 @@ int qfx_tile_check_ops(int T, int block, int flags, const qfx_tile_op* ops, int 
 }
 
 int qfx_tile_pass(void* stream, void* state, int n, int tile_bits, const int* tile_pos,
-                  const void* ops, int nops, const void* coef, int ncoef, const void* srcs,
-                  int block, int flags) {
+                  const int* store_pos, const void* ops, int nops, const void* coef, int ncoef,
+                  const void* srcs, int block, int flags) {
   int rc = check_pass(n, tile_bits, tile_pos, nops, block, "qfx_tile_pass");
   if (rc) return rc;
+  if (store_pos) {
+    for (int j = 0; j < tile_bits; ++j) {
+      int hits = 0;
+      for (int i = 0; i < tile_bits; ++i) hits += store_pos[i] == tile_pos[j];
+      if (hits != 1)
+        return qfx::set_error(QFX_EINVAL,
+                              "qfx_tile_pass: store positions must permute the tile positions");
+    }
+  }
   const bool wide = (flags & QFX_TILE_WIDE) != 0;
   const bool pers = (flags & QFX_TILE_PERSISTENT) != 0;
   const bool mf = (flags & QFX_TILE_MFMA) != 0;
@@ -860,7 +878,10 @@ int qfx_tile_pass(void* stream, void* state, int n, int tile_bits, const int* ti
                           "qfx_tile_pass: more than 512 coefficients in a pass without wide ops");
   TilePos pos;
   std::memset(&pos, 0, sizeof(pos));
-  for (int j = 0; j < tile_bits; ++j) pos.s[j] = tile_pos[j];
+  for (int j = 0; j < tile_bits; ++j) {
+    pos.s[j] = tile_pos[j];
+    pos.d[j] = store_pos ? store_pos[j] : tile_pos[j];
+  }
   hipStream_t s = qfx::as_stream(stream);
   auto* st = static_cast<v2d*>(state);
   auto* o = static_cast<const qfx_tile_op*>(ops);

@@ -620,6 +620,35 @@ def _affine_code(table):
     return sum(col << (4 * j) for j, col in enumerate(cols)) | (c << 16)
 
 
+def relabel_swaps(ops, pos, low_bits):
+    """Absorb a pass's uncontrolled SWAP-type gates (SWAP, ISWAP, PSWAP) by
+    relabelling which tile bit holds which qubit: later ops are rewritten to
+    the new bits, the gate's phases stay as a diagonal, and the pass stores
+    tile bit j to index bit store_pos[j].  Swaps touching the lowest
+    ``low_bits`` index bits are kept (relabelling them would break the
+    coalesced runs of the store).  Returns (ops, store_pos or None)."""
+    where = {b: b for b in pos}             # qubit's index bit -> tile bit holding it
+    out = []
+    moved = False
+    for op in ops:
+        bits = [where[b] for b in op.bits]
+        pat = exchange_pattern(op.kind) if op.matrix is None else None
+        if (pat is not None and pat[0] == 'swap' and not pat[3]
+                and min(bits[pat[1]], bits[pat[2]]) >= low_bits):
+            x, y = op.bits[pat[1]], op.bits[pat[2]]
+            where[x], where[y] = where[y], where[x]
+            moved = True
+            if op.kind.phase is not None:   # phases on output rows, at the new bits
+                out.append(Op([where[b] for b in op.bits],
+                              bk.classify(np.diag(op.kind.phase))))
+            continue
+        out.append(Op(bits, op.kind, op.matrix, op.ngates))
+    if not moved:
+        return out, None
+    holder = {t: b for b, t in where.items()}  # tile bit -> qubit index bit it holds
+    return out, [holder[b] for b in pos]
+
+
 def plan_blocks(ops, width):
     """Group a pass's ops (in order) into dense blocks of <= width qubits,
     with the commuting-overtake rule and bit-set growth of plan_passes.
@@ -744,6 +773,7 @@ class HostPlan:
             wide = any(len(op.bits) > 3 for op in ops)
             start = len(ops_host)
             c_lo = len(coefs)
+            store_pos = None
             if self.mode == 'mfma':
                 width = max(MFMA_MIN_BITS, min(fuse_width, 5))
                 for bits, group in plan_blocks(ops, width):
@@ -757,6 +787,7 @@ class HostPlan:
                     self.n_blocks += 1
                 block, flags = self.block, TILE_MFMA
             elif self.mode == 'reg' and not wide:
+                ops, store_pos = relabel_swaps(ops, pos, low_bits)
                 for item in plan_phases(ops):
                     if item[0] == 'phase':
                         ops_host.append(_encode_phase(item[1], item[2], pos_index, T, coefs, srcs))
@@ -775,7 +806,8 @@ class HostPlan:
                     block = min(block, 256)
             if persistent:
                 flags |= TILE_PERSISTENT
-            self._add_tile_passes(pos, ops_host, start, coefs, c_lo, srcs, block, flags)
+            self._add_tile_passes(pos, ops_host, start, coefs, c_lo, srcs, block, flags,
+                                  store_pos)
         self.n_passes = len(self.passes)
         self.n_ops = len(ops_host)
         self.ops_host, self.coefs, self.srcs = ops_host, coefs, srcs
@@ -785,16 +817,17 @@ class HostPlan:
         for kind, payload in self.passes:
             if kind != 'tile':
                 continue
-            _, start, nops, block, flags, c_lo, ncoef = payload
+            _, start, nops, block, flags, c_lo, ncoef, _ = payload
             arr = (_native.QfxTileOp * max(1, nops))(*ops_host[start:start + nops])
             _native.check(lib.qfx_tile_check_ops(T, block, flags, arr, nops, ncoef,
                                                  src_arr, len(srcs)), 'qfx_tile_check_ops')
 
-    def _add_tile_passes(self, pos, ops_host, start, coefs, c_lo, srcs, block, flags):
+    def _add_tile_passes(self, pos, ops_host, start, coefs, c_lo, srcs, block, flags,
+                         store_pos=None):
         """Record the pass of ops_host[start:] with coefficient indices made
         relative to the pass's table; a pass that would stage more than
         MAX_STAGED coefficients is cut into consecutive passes on the same
-        tile bits."""
+        tile bits (only the last one stores through store_pos)."""
         ops = ops_host[start:]
         if not ops:
             return
@@ -811,8 +844,9 @@ class HostPlan:
                 b += 1
             for o in ops[a:b]:
                 _rebase_coef(o, srcs, bounds[a])
+            spos = store_pos if b == len(ops) else None
             self.passes.append(('tile', (pos, start + a, b - a, block, flags, bounds[a],
-                                         bounds[b] - bounds[a])))
+                                         bounds[b] - bounds[a], spos)))
             a = b
 
     def stats(self):
@@ -839,8 +873,9 @@ class TilePlan(HostPlan):
                                                  dtype=np.complex128)).to(dev)
         self._srcs = torch.tensor(srcs if srcs else [0], dtype=torch.int32, device=dev)
         self._opsize = _native.ctypes.sizeof(_native.QfxTileOp)
-        self._pos_arrays = [(_native.int_array(p[1][0]), len(p[1][0])) if p[0] == 'tile'
-                            else None for p in self.passes]
+        self._pos_arrays = [(_native.int_array(p[1][0]), len(p[1][0]),
+                             _native.int_array(p[1][7]) if p[1][7] is not None else None)
+                            if p[0] == 'tile' else None for p in self.passes]
 
     def run_inplace(self, work, prof=None):
         """Run every pass in place on the flat complex128 device state."""
@@ -857,9 +892,9 @@ class TilePlan(HostPlan):
                 tok = prof.start(kernel_class(gk)) if prof is not None else None
                 bk.apply_gate(work, work, n, bits, gk, inplace_sparse=True)
             else:
-                pos, start, nops, block, flags, c_lo, ncoef = payload
+                pos, start, nops, block, flags, c_lo, ncoef, _ = payload
                 tok = prof.start('tile_pass') if prof is not None else None
-                rc = lib.qfx_tile_pass(s, wp, n, posarr[1], posarr[0],
+                rc = lib.qfx_tile_pass(s, wp, n, posarr[1], posarr[0], posarr[2],
                                        _native.ctypes.c_void_p(base_ops + start * self._opsize),
                                        nops, _native.ctypes.c_void_p(coef + 16 * c_lo), ncoef,
                                        srcs, block, flags)

@@ -117,6 +117,17 @@ def unpack_mfma(slots, k):
     return r[:d, :d] + 1j * r[d:, :d]
 
 
+def apply_store(ket, n, pos, spos):
+    """A pass's store relabelling: tile bit pos[j] is written to index bit
+    spos[j] (None: unchanged)."""
+    if spos is None:
+        return ket
+    axes = list(range(n))
+    for p, q in zip(pos, spos):
+        axes[n - 1 - q] = n - 1 - p
+    return np.transpose(ket, axes)
+
+
 def _emulate_ops(ket, n, pos, ops, coefs, srcs):
     """Apply encoded qfx_tile_op records to a numpy state, as the tile kernel
     defines them (register phases: register bit j <-> tile bit t[j])."""
@@ -242,13 +253,14 @@ def test_uncontrolled_exchanges_cost_no_sub_ops():
     ket = psi
     nsub = 0
     for kind, payload in plan.passes:
-        pos, start, nops, block, flags, c_lo, ncoef = payload
+        pos, start, nops, block, flags, c_lo, ncoef, spos = payload
         ops = plan.ops_host[start:start + nops]
         for o in ops:
             nsub += o.coef
             for q in range(o.coef):
                 assert plan.srcs[o.src + 4 * q] == fusion.SUB_DIAG
         ket = _emulate_ops(ket, n, pos, ops, plan.coefs[c_lo:c_lo + ncoef], plan.srcs)
+        ket = apply_store(ket, n, pos, spos)
     assert nsub <= 2
     assert np.array_equal(np.abs(ket), np.abs(ref))
     assert np.max(np.abs(ket - ref)) < 1e-15
@@ -354,11 +366,12 @@ def test_mfma_plan_replays(width, T):
     ket = O.zero_state(n)
     for kind, payload in plan.passes:
         assert kind == 'tile'
-        pos, start, nops, block, flags, c_lo, ncoef = payload
+        pos, start, nops, block, flags, c_lo, ncoef, spos = payload
         assert flags & fusion.TILE_MFMA
         ops = plan.ops_host[start:start + nops]
         assert all(o.kind == fusion.OP_MFMA and 3 <= o.k <= width for o in ops)
         ket = _emulate_ops(ket, n, pos, ops, plan.coefs[c_lo:c_lo + ncoef], plan.srcs)
+        ket = apply_store(ket, n, pos, spos)
     assert np.max(np.abs(ket - ref)) < 1e-12
 
 
@@ -405,10 +418,11 @@ def test_host_plan_pass_tables_replay(mode, width):
                            fuse_width=width, mode=mode)
     ket = O.zero_state(n)
     for kind, payload in plan.passes:
-        pos, start, nops, block, flags, c_lo, ncoef = payload
+        pos, start, nops, block, flags, c_lo, ncoef, spos = payload
         assert ncoef <= fusion.MAX_STAGED or flags & (fusion.TILE_WIDE | fusion.TILE_MFMA)
         ket = _emulate_ops(ket, n, pos, plan.ops_host[start:start + nops],
                            plan.coefs[c_lo:c_lo + ncoef], plan.srcs)
+        ket = apply_store(ket, n, pos, spos)
     assert np.max(np.abs(ket - ref)) < 1e-12
 
 
@@ -426,8 +440,39 @@ def test_pass_split_at_staging_limit(monkeypatch):
     assert plan.n_passes > len(unsplit)
     ket = O.zero_state(n)
     for kind, payload in plan.passes:
-        pos, start, nops, block, flags, c_lo, ncoef = payload
+        pos, start, nops, block, flags, c_lo, ncoef, spos = payload
         assert ncoef <= 40 or nops == 1
         ket = _emulate_ops(ket, n, pos, plan.ops_host[start:start + nops],
                            plan.coefs[c_lo:c_lo + ncoef], plan.srcs)
+        ket = apply_store(ket, n, pos, spos)
+    assert np.max(np.abs(ket - ref)) < 1e-12
+
+
+def test_pass_level_swap_relabel():
+    """SWAP/ISWAP gates in a pass become a relabelling of tile bits plus the
+    ISWAP phases; the pass stores through a permutation; replaying with the
+    store permutation gives the circuit's state exactly."""
+    n, T = 10, 8
+    rng = np.random.default_rng(12)
+    names = [('SWAP', 2), ('ISWAP', 2), ('RX', 1), ('CNOT', 2), ('CZ', 2), ('H', 1)]
+    desc = []
+    for _ in range(40):
+        nm, k = names[int(rng.integers(len(names)))]
+        params = (float(rng.uniform(0, 6)),) if nm == 'RX' else ()
+        desc.append((nm, params, tuple(int(q) for q in rng.permutation(n)[:k])))
+    gates = build_gates(desc)
+    psi = rng.normal(size=2 ** n) + 1j * rng.normal(size=2 ** n)
+    psi = (psi / np.linalg.norm(psi)).reshape([2] * n)
+    ref = O.run_circuit(desc, qubits=tuple(range(n)), ket=psi)
+    plan = fusion.HostPlan(gates, tuple(range(n)), tile_bits=T, low_bits=3, block=256,
+                           fuse_width=1, mode='reg')
+    assert any(p[1][7] is not None for p in plan.passes)
+    ket = psi
+    for kind, payload in plan.passes:
+        pos, start, nops, block, flags, c_lo, ncoef, spos = payload
+        if spos is not None:
+            assert sorted(spos) == sorted(pos) and all(a == b for a, b in zip(pos, spos) if a < 3)
+        ket = _emulate_ops(ket, n, pos, plan.ops_host[start:start + nops],
+                           plan.coefs[c_lo:c_lo + ncoef], plan.srcs)
+        ket = apply_store(ket, n, pos, spos)
     assert np.max(np.abs(ket - ref)) < 1e-12
