@@ -181,6 +181,9 @@ typedef struct {
                                   the next tile during the current one (used
                                   when 2^tile_bits / block <= 16) */
 #define QFX_TILE_MFMA 4   /* flags: the pass holds matrix-core blocks */
+#define QFX_TILE_REG_IO 8 /* flags: persistent pass whose last op is a register
+                             phase storing straight to HBM; needs
+                             2^tile_bits == 16 * block, block <= 512 */
 
 /* Host-side validation of an op list against tile width T, the block size
  * and flags it will run with (ops on 4-5 qubits need QFX_TILE_WIDE and

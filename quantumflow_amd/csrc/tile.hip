@@ -33,7 +33,7 @@ constexpr int kMaxStaged = 512;
 
 struct TilePos {
   int s[QFX_MAX_QUBITS];  // tile bit positions, ascending (where tile bit j loads from)
-  int d[kMaxTileBits];    // where tile bit j is stored (a permutation of s[0..T-1])
+  int d[QFX_MAX_QUBITS];  // where tile bit j is stored (a permutation of s[0..T-1])
 };
 
 __device__ __forceinline__ unsigned ins0(unsigned g, int b) {
@@ -309,64 +309,94 @@ __device__ __forceinline__ void apply_sub(v2d (&v)[kRegs], const int4 sub, const
   }
 }
 
+// Uniform addressing of a register phase.  Logical index x of the phase
+// (bit A <-> tile bit op.t[A]) sits at LDS slot sw(lb) ^ off(x), off XOR-
+// linear; its affine load map (op.phase) and store map (op.active) give
+// register r the logical index map(r) = c ^ (XOR of column j over the set
+// bits j of r), so slot addresses are XOR trees over 4 basis values.
+struct PhaseLds {
+  unsigned l[4], lc, s[4], sc;
+};
+
+__device__ __forceinline__ PhaseLds phase_lds(const qfx_tile_op& op) {
+  const unsigned w0 = sw(1u << op.t[0]), w1 = sw(1u << op.t[1]);
+  const unsigned w2 = sw(1u << op.t[2]), w3 = sw(1u << op.t[3]);
+  auto off = [&](unsigned x) {
+    return ((x & 1u) ? w0 : 0u) ^ ((x & 2u) ? w1 : 0u) ^ ((x & 4u) ? w2 : 0u) ^
+           ((x & 8u) ? w3 : 0u);
+  };
+  const unsigned lm = op.phase, sm = op.active;
+  PhaseLds p;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    p.l[j] = off((lm >> (4 * j)) & 15u);
+    p.s[j] = off((sm >> (4 * j)) & 15u);
+  }
+  p.lc = off(lm >> 16);
+  p.sc = off(sm >> 16);
+  return p;
+}
+
+template <typename U>
+__device__ __forceinline__ void xor_tree(U (&a)[kRegs], U a0, const U (&b)[4]) {
+  a[0] = a0;
+  a[1] = a0 ^ b[0];
+  a[2] = a0 ^ b[1];
+  a[3] = a[1] ^ b[1];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) a[4 + r] = a[r] ^ b[2];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) a[8 + r] = a[r] ^ b[3];
+}
+
+// tile index of lane group g with zeros at the phase's register bits
+__device__ __forceinline__ unsigned phase_lb(const qfx_tile_op& op, unsigned g) {
+#pragma unroll
+  for (int j = 0; j < kRegBits; ++j) g = ins0(g, op.sorted[j]);
+  return g;
+}
+
+__device__ __forceinline__ void run_subs(v2d (&v)[kRegs], const qfx_tile_op& op,
+                                         const v2d* coef, const int* __restrict__ subs) {
+  // the next sub-op record is fetched while the current one runs
+  const int4* rec = reinterpret_cast<const int4*>(subs + op.src);
+  const int nsub = op.coef;
+  int4 next = rec[0];
+  for (int i = 0; i < nsub; ++i) {
+    const int4 cur = next;
+    next = rec[i + 1 < nsub ? i + 1 : i];
+    apply_sub(v, cur, coef);
+  }
+}
+
+__device__ __forceinline__ void phase_lds_load(v2d (&v)[kRegs], const v2d* lds, unsigned slb,
+                                               const PhaseLds& p) {
+  unsigned a[kRegs];
+  xor_tree(a, slb ^ p.lc, p.l);
+#pragma unroll
+  for (int r = 0; r < kRegs; ++r) v[r] = lds[a[r]];
+}
+
+__device__ __forceinline__ void phase_lds_store(const v2d (&v)[kRegs], v2d* lds, unsigned slb,
+                                                const PhaseLds& p) {
+  unsigned a[kRegs];
+  xor_tree(a, slb ^ p.sc, p.s);
+#pragma unroll
+  for (int r = 0; r < kRegs; ++r) lds[a[r]] = v[r];
+}
+
 template <int BS>
 __device__ __forceinline__ void op_rphase(v2d* lds, int T, const qfx_tile_op& op,
                                           const v2d* coef,
                                           const int* __restrict__ subs) {
   const int ngroups = 1 << (T - kRegBits);
-  const int* sub0 = subs + op.src;
-  const int nsub = op.coef;
-  // sw is XOR-linear: the slot of register r is sw(lb) ^ (XOR of w[j] over
-  // the set bits j of r), w[j] = sw(1 << t[j]), built up as a XOR tree
-  const unsigned w0 = sw(1u << op.t[0]), w1 = sw(1u << op.t[1]);
-  const unsigned w2 = sw(1u << op.t[2]), w3 = sw(1u << op.t[3]);
-  // LDS offset of logical index x: XOR of w_k over the set bits k of x
-  auto off = [&](unsigned x) {
-    return ((x & 1u) ? w0 : 0u) ^ ((x & 2u) ? w1 : 0u) ^ ((x & 4u) ? w2 : 0u) ^
-           ((x & 8u) ? w3 : 0u);
-  };
-  // affine register maps (matrix | offset << 16, column j = image of bit j):
-  // load map op.phase, store map op.active
-  const unsigned lm = op.phase, sm = op.active;
-  const unsigned l0 = off(lm & 15u), l1 = off((lm >> 4) & 15u), l2 = off((lm >> 8) & 15u),
-                 l3 = off((lm >> 12) & 15u), lc = off(lm >> 16);
-  const unsigned s0 = off(sm & 15u), s1 = off((sm >> 4) & 15u), s2 = off((sm >> 8) & 15u),
-                 s3 = off((sm >> 12) & 15u), sc = off(sm >> 16);
+  const PhaseLds p = phase_lds(op);
   for (int g = threadIdx.x; g < ngroups; g += BS) {
-    unsigned lb = (unsigned)g;
-#pragma unroll
-    for (int j = 0; j < kRegBits; ++j) lb = ins0(lb, op.sorted[j]);
-    const unsigned slb = sw(lb);
-    unsigned a[kRegs];
-    a[0] = slb ^ lc;
-    a[1] = a[0] ^ l0;
-    a[2] = a[0] ^ l1;
-    a[3] = a[1] ^ l1;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) a[4 + r] = a[r] ^ l2;
-#pragma unroll
-    for (int r = 0; r < 8; ++r) a[8 + r] = a[r] ^ l3;
+    const unsigned slb = sw(phase_lb(op, (unsigned)g));
     v2d v[kRegs];
-#pragma unroll
-    for (int r = 0; r < kRegs; ++r) v[r] = lds[a[r]];
-    // the next sub-op record is fetched while the current one runs
-    const int4* rec = reinterpret_cast<const int4*>(sub0);
-    int4 next = rec[0];
-    for (int i = 0; i < nsub; ++i) {
-      const int4 cur = next;
-      next = rec[i + 1 < nsub ? i + 1 : i];
-      apply_sub(v, cur, coef);
-    }
-    a[0] = slb ^ sc;
-    a[1] = a[0] ^ s0;
-    a[2] = a[0] ^ s1;
-    a[3] = a[1] ^ s1;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) a[4 + r] = a[r] ^ s2;
-#pragma unroll
-    for (int r = 0; r < 8; ++r) a[8 + r] = a[r] ^ s3;
-#pragma unroll
-    for (int r = 0; r < kRegs; ++r) lds[a[r]] = v[r];
+    phase_lds_load(v, lds, slb, p);
+    run_subs(v, op, coef, subs);
+    phase_lds_store(v, lds, slb, p);
   }
 }
 
@@ -665,10 +695,131 @@ __global__ __launch_bounds__(BS) void tile_pipe_kernel(v2d* state, int T, TilePo
   }
 }
 
+// ---- register-phase passes that store through their last phase ----------
+// A persistent pass (one 16-amplitude group per lane, T = LB + 4) whose
+// last op is a register phase stores that phase's registers straight to
+// HBM, saving the LDS write of its results, the LDS read of the store loop
+// and a barrier per tile.  Global offsets: tile bit j <-> index bit
+// pos.d[j]; a logical index x of the phase sits at the sum of 2^d[t[A]]
+// over its set bits A, XOR-linear as the LDS offsets.
+struct PhaseGlobal {
+  uint64_t b[4], c;
+};
+
+__device__ __forceinline__ uint64_t tile_bit_off(int tbit, int T, const int* bitpos) {
+  uint64_t o = 0;
+#pragma unroll
+  for (int j = 0; j < kMaxTileBits; ++j)
+    if (j < T && j == tbit) o = (uint64_t)1 << bitpos[j];
+  return o;
+}
+
+__device__ __forceinline__ uint64_t tile_state_off(unsigned e, int T, const int* bitpos) {
+  uint64_t o = 0;
+#pragma unroll
+  for (int j = 0; j < kMaxTileBits; ++j)
+    if (j < T) o |= (uint64_t)((e >> j) & 1u) << bitpos[j];
+  return o;
+}
+
+__device__ __forceinline__ PhaseGlobal phase_global(const qfx_tile_op& op, unsigned map, int T,
+                                                    const int* bitpos) {
+  uint64_t e[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) e[a] = tile_bit_off(op.t[a], T, bitpos);
+  auto off = [&](unsigned x) {
+    return ((x & 1u) ? e[0] : 0) ^ ((x & 2u) ? e[1] : 0) ^ ((x & 4u) ? e[2] : 0) ^
+           ((x & 8u) ? e[3] : 0);
+  };
+  PhaseGlobal g;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) g.b[j] = off((map >> (4 * j)) & 15u);
+  g.c = off(map >> 16);
+  return g;
+}
+
+__device__ __forceinline__ void phase_gstore(const v2d (&v)[kRegs], v2d* p, const PhaseGlobal& g) {
+  uint64_t a[kRegs];
+  xor_tree(a, g.c, g.b);
+#pragma unroll
+  for (int r = 0; r < kRegs; ++r) st_stream(p + a[r], v[r]);
+}
+
+template <int BS, int LB>
+__global__ __launch_bounds__(BS) void tile_reg_kernel(v2d* state, int T, TilePos pos,
+                                                     const qfx_tile_op* __restrict__ ops,
+                                                     int nops, const v2d* __restrict__ coef,
+                                                     int ncoef, const int* __restrict__ srcs,
+                                                     uint32_t ntiles) {
+  extern __shared__ v2d lds[];
+  const int tid = threadIdx.x;
+  v2d* cf = stage_coef<BS>(lds, T, coef, ncoef);
+  const qfx_tile_op& last = ops[nops - 1];
+  // the lane's group in the last phase, as HBM offsets
+  const uint64_t l_lane = tile_state_off(phase_lb(last, (unsigned)tid), T, pos.d);
+  const PhaseGlobal lg = phase_global(last, last.active, T, pos.d);
+  uint64_t toff = 0;
+#pragma unroll
+  for (int j = 0; j < LB; ++j) toff |= (uint64_t)((tid >> j) & 1) << pos.s[j];
+  const uint64_t hb0 = (uint64_t)1 << pos.s[LB], hb1 = (uint64_t)1 << pos.s[LB + 1];
+  const uint64_t hb2 = (uint64_t)1 << pos.s[LB + 2], hb3 = (uint64_t)1 << pos.s[LB + 3];
+
+  auto tile_base = [&](uint32_t tile) {
+    uint64_t base = tile;
+#pragma unroll
+    for (int j = 0; j < kMaxTileBits; ++j) {
+      if (j < T) {
+        const int b = pos.s[j];
+        base = ((base >> b) << (b + 1)) | (base & (((uint64_t)1 << b) - 1));
+      }
+    }
+    return base;
+  };
+  auto load_tile = [&](v2d (&pf)[kPrefetch], uint64_t base) {
+    const v2d* p0 = state + base + toff;
+    const v2d* p1 = p0 + hb1;
+#pragma unroll
+    for (int u = 0; u < kPrefetch; u += 4) {
+      const uint64_t o = ((u & 4) ? hb2 : 0) + ((u & 8) ? hb3 : 0);
+      pf[u] = ld_stream(p0 + o);
+      pf[u + 1] = ld_stream(p0 + o + hb0);
+      pf[u + 2] = ld_stream(p1 + o);
+      pf[u + 3] = ld_stream(p1 + o + hb0);
+    }
+  };
+
+  v2d pf[kPrefetch];
+  uint32_t tile = blockIdx.x;       // the grid never exceeds ntiles
+  uint64_t base = tile_base(tile);
+  load_tile(pf, base);
+  for (; tile < ntiles; tile += gridDim.x) {
+#pragma unroll
+    for (int u = 0; u < kPrefetch; ++u) lds[sw(tid + u * BS)] = pf[u];
+    // the last tile re-reads itself instead of branching around the loads
+    const uint32_t next = tile + gridDim.x < ntiles ? tile + gridDim.x : tile;
+    const uint64_t nbase = tile_base(next);
+    load_tile(pf, nbase);
+    __syncthreads();
+    for (int i = 0; i < nops - 1; ++i) {
+      apply_op<BS, false, false>(lds, T, ops[i], cf, coef, srcs);
+      __syncthreads();
+    }
+    {
+      const PhaseLds lp = phase_lds(last);
+      v2d v[kRegs];
+      phase_lds_load(v, lds, sw(phase_lb(last, (unsigned)tid)), lp);
+      run_subs(v, last, cf, srcs);
+      phase_gstore(v, state + base + l_lane, lg);
+    }
+    base = nbase;
+    __syncthreads();
+  }
+}
+
 template <int BS, int LB, bool WIDE, bool MF>
 int launch_pass(hipStream_t s, v2d* state, int n, int T, const TilePos& pos,
                 const qfx_tile_op* ops, int nops, const v2d* coef, int ncoef, const int* srcs,
-                bool persistent) {
+                bool persistent, bool reg_io = false) {
   constexpr bool kStage = !WIDE && !MF;
   const size_t lds = sizeof(v2d) * ((size_t(1) << T) + (kStage ? ncoef : 0));
   // occupancy (persistent grid size) for the largest staged table
@@ -690,6 +841,38 @@ int launch_pass(hipStream_t s, v2d* state, int n, int T, const TilePos& pos,
     attr_set = true;
   }
   const uint64_t tiles = (uint64_t)1 << (n - T);
+  if constexpr (!WIDE && !MF && BS <= kRphaseMaxBlock) {
+    if (reg_io) {   // the caller checked T == LB + 4
+      static int occ_reg[kMaxTileBits + 1];
+      static bool reg_attr = false;
+      if (!reg_attr) {
+        int rc = qfx::check_hip(hipFuncSetAttribute((const void*)tile_reg_kernel<BS, LB>,
+                                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                    160 * 1024),
+                                "hipFuncSetAttribute");
+        if (rc) return rc;
+        for (int t = 0; t <= kMaxTileBits; ++t) occ_reg[t] = 0;
+        reg_attr = true;
+      }
+      if (occ_reg[T] == 0) {
+        int per_cu = 0, dev = 0, cus = 0;
+        int rc = qfx::check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                                    &per_cu, (const void*)tile_reg_kernel<BS, LB>, BS, lds_max),
+                                "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+        if (rc) return rc;
+        rc = qfx::check_hip(hipGetDevice(&dev), "hipGetDevice");
+        if (rc) return rc;
+        rc = qfx::check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev),
+                            "hipDeviceGetAttribute");
+        if (rc) return rc;
+        occ_reg[T] = (per_cu > 0 ? per_cu : 1) * cus;
+      }
+      const uint64_t grid = tiles < (uint64_t)occ_reg[T] ? tiles : (uint64_t)occ_reg[T];
+      hipLaunchKernelGGL((tile_reg_kernel<BS, LB>), dim3((unsigned)grid), dim3(BS), lds, s, state,
+                         T, pos, ops, nops, coef, ncoef, srcs, (uint32_t)tiles);
+      return qfx::check_launch("tile_reg_kernel");
+    }
+  }
   if (persistent && BS <= 512 && T - LB == 4) {
     if (occ[T] == 0) {
       int per_cu = 0, dev = 0, cus = 0;
@@ -738,8 +921,15 @@ extern "C" {
 
 int qfx_tile_check_ops(int T, int block, int flags, const qfx_tile_op* ops, int nops,
                        int ncoef, const int* srcs, int nsrcs) {
-  if (flags & ~(QFX_TILE_WIDE | QFX_TILE_PERSISTENT | QFX_TILE_MFMA))
+  if (flags & ~(QFX_TILE_WIDE | QFX_TILE_PERSISTENT | QFX_TILE_MFMA | QFX_TILE_REG_IO))
     return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: unknown flags");
+  if (flags & QFX_TILE_REG_IO) {
+    if ((flags & (QFX_TILE_WIDE | QFX_TILE_MFMA)) || block > kRphaseMaxBlock ||
+        (1 << T) != block * kRegs || nops < 1 || !ops || ops[nops - 1].kind != OP_RPHASE)
+      return qfx::set_error(QFX_EINVAL,
+                            "qfx_tile_check_ops: QFX_TILE_REG_IO needs a last register phase, "
+                            "16 amplitudes per lane, block <= 512");
+  }
   if ((flags & QFX_TILE_WIDE) && block > kWideOpMaxBlock)
     return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: 4-5 qubit ops need block <= 256");
   if ((flags & QFX_TILE_MFMA) && block > kMfmaMaxBlock)
@@ -865,8 +1055,14 @@ int qfx_tile_pass(void* stream, void* state, int n, int tile_bits, const int* ti
   const bool wide = (flags & QFX_TILE_WIDE) != 0;
   const bool pers = (flags & QFX_TILE_PERSISTENT) != 0;
   const bool mf = (flags & QFX_TILE_MFMA) != 0;
-  if (flags & ~(QFX_TILE_WIDE | QFX_TILE_PERSISTENT | QFX_TILE_MFMA))
+  const bool reg_io = (flags & QFX_TILE_REG_IO) != 0;
+  if (flags & ~(QFX_TILE_WIDE | QFX_TILE_PERSISTENT | QFX_TILE_MFMA | QFX_TILE_REG_IO))
     return qfx::set_error(QFX_EINVAL, "qfx_tile_pass: unknown flags");
+  if (reg_io && (wide || mf || block > kRphaseMaxBlock || nops < 1 ||
+                 (1 << tile_bits) != block * kRegs))
+    return qfx::set_error(QFX_EINVAL,
+                          "qfx_tile_pass: QFX_TILE_REG_IO needs 16 amplitudes per lane, "
+                          "block <= 512, no wide ops or matrix-core blocks");
   if (wide && block > kWideOpMaxBlock)
     return qfx::set_error(QFX_EINVAL, "qfx_tile_pass: 4-5 qubit ops need block <= 256");
   if (mf && block > kMfmaMaxBlock)
@@ -888,7 +1084,7 @@ int qfx_tile_pass(void* stream, void* state, int n, int tile_bits, const int* ti
   auto* c = static_cast<const v2d*>(coef);
   auto* sr = static_cast<const int*>(srcs);
 #define QFX_PASS(BS, LB, W, M) \
-  launch_pass<BS, LB, W, M>(s, st, n, tile_bits, pos, o, nops, c, ncoef, sr, pers)
+  launch_pass<BS, LB, W, M>(s, st, n, tile_bits, pos, o, nops, c, ncoef, sr, pers, reg_io)
   switch (block) {
     case 64:
       if (mf) return wide ? QFX_PASS(64, 6, true, true) : QFX_PASS(64, 6, false, true);

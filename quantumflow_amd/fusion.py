@@ -45,6 +45,7 @@ MIN_TILE_BITS = 6
 TILE_WIDE = 1
 TILE_PERSISTENT = 2
 TILE_MFMA = 4
+TILE_REG_IO = 8
 MFMA_MIN_BITS = 3
 
 
@@ -431,9 +432,9 @@ def _encode_phase(R, members, pos_index, T, coefs, subs):
     4-bit logical index and are stored as matrix | offset << 16 (column j
     = image of bit j): ``active`` the store map, ``phase`` the load map."""
     tl = sorted(pos_index[b] for b in R)
-    for j in range(T):                      # pad to 4 register bits
-        if len(tl) >= REG_BITS:
-            break
+    for j in range(T - 1, -1, -1):          # pad to 4 register bits with high tile
+        if len(tl) >= REG_BITS:             # bits: lanes keep the low ones, whose
+            break                           # amplitudes are contiguous in HBM
         if j not in tl:
             tl.append(j)
     regs = tl[:REG_BITS]                    # logical bit A <-> tile bit regs[A]
@@ -738,7 +739,7 @@ class HostPlan:
     index their pass's coefficient table."""
 
     def __init__(self, gates, qubits, tile_bits=MAX_TILE_BITS, low_bits=3, block=256,
-                 fuse_width=4, mode='reg', persistent=True):
+                 fuse_width=4, mode='reg', persistent=True, reg_io=True):
         qubits = tuple(qubits)
         n = len(qubits)
         self.n = n
@@ -756,6 +757,7 @@ class HostPlan:
             self.block = 512
         self.fuse_width = fuse_width
         self.persistent = persistent
+        self.reg_io = reg_io
         self.gates = list(gates)
         bits_list = [tuple(n - 1 - qubits.index(q) for q in g.qubits) for g in gates]
         tileable = [len(b) <= 5 and len(b) <= T for b in bits_list]
@@ -795,6 +797,13 @@ class HostPlan:
                     else:
                         ops_host.append(_encode_op(item[1], pos_index, coefs, srcs))
                 block, flags = self.block, 0
+                # a last register phase stores straight to HBM when its lanes
+                # run over the low (contiguous) tile bits
+                last = ops_host[-1]
+                if (persistent and self.reg_io and last.kind == OP_RPHASE
+                        and (1 << T) == 16 * block and block <= 512
+                        and min(last.t[j] for j in range(REG_BITS)) >= min(low_bits, 3)):
+                    flags = TILE_REG_IO
             else:
                 block = min(self.block, 256) if wide else self.block
                 width = self.fuse_width if block <= 256 else min(self.fuse_width, 3)
@@ -845,7 +854,10 @@ class HostPlan:
             for o in ops[a:b]:
                 _rebase_coef(o, srcs, bounds[a])
             spos = store_pos if b == len(ops) else None
-            self.passes.append(('tile', (pos, start + a, b - a, block, flags, bounds[a],
+            fl = flags
+            if fl & TILE_REG_IO and ops[b - 1].kind != OP_RPHASE:
+                fl &= ~TILE_REG_IO
+            self.passes.append(('tile', (pos, start + a, b - a, block, fl, bounds[a],
                                          bounds[b] - bounds[a], spos)))
             a = b
 
