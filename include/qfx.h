@@ -113,6 +113,11 @@ int qfx_expect_diag(void* stream, void* result, const void* psi,
 /* out[a] = <Z_a> = sum_i |psi_i|^2 (1 - 2 bit_{n-1-a}(i)) for every axis a */
 int qfx_expect_z(void* stream, void* out, const void* psi, int n,
                  void* workspace);
+/* out[0] = sum |psi_i|^2 over i with bit n-1-axis clear (qubit axis in |0>),
+ * out[1] = the same over i with it set; float64.  out[0] is
+ * P0(q).run(ket).norm() of Measure.run (quantumflow/stdops.py:53-54). */
+int qfx_qubit_probs(void* stream, void* out, const void* psi, int n, int axis,
+                    void* workspace);
 /* result[0] = sum_i |psi_i|^2 (float64) */
 int qfx_norm2(void* stream, void* result, const void* psi, int64_t len,
               void* workspace);
@@ -128,6 +133,22 @@ int qfx_permute_qubits(void* stream, void* out, const void* in, int n,
 int qfx_zero_state(void* stream, void* psi, int n);
 /* psi = |+...+> = 2^(-n/2) everywhere. */
 int qfx_plus_state(void* stream, void* psi, int n);
+
+/* ---- sampling the computational basis -----------------------------------
+ * Replaces the numpy draws of State.sample (quantumflow/states.py:121-129,
+ * np.random.multinomial over probabilities()) and State.measure
+ * (quantumflow/states.py:149-160, np.random.choice): out[t] is a basis index
+ * drawn with probability |psi_i|^2 / sum_j |psi_j|^2, from a SplitMix64
+ * stream keyed by (seed, t); reproducible for equal (psi, seed).  Indices of
+ * zero probability are never drawn.  Index bit n-1-a is qubit axis a. */
+/* Bytes of device workspace qfx_sample needs for n qubits (0 if n invalid). */
+int64_t qfx_sample_workspace_bytes(int n);
+int qfx_sample(void* stream, int64_t* out, const void* psi, int n,
+               int64_t trials, uint64_t seed, void* workspace);
+/* counts[idx[i]] += 1 for i < m (counts: 2^n int64, zeroed by the caller;
+ * every idx[i] must lie in [0, 2^n)). */
+int qfx_histogram(void* stream, int64_t* counts, const int64_t* idx, int64_t m,
+                  int n);
 
 /* ---- fused circuit execution: LDS-tiled passes --------------------------
  * Replaces the per-gate loop of Circuit.run (quantumflow/circuits.py:87-99)

@@ -575,5 +575,94 @@ def maxcut_qaoa_gd(n, edges, steps=5, learning_rate=0.01, beta0=None,
     return ratio, beta, gamma
 
 
+# --------------------------------------------------------------------------
+# Sampling and measurement (quantumflow/states.py:121-160,
+# quantumflow/stdops.py:38-111)
+# --------------------------------------------------------------------------
+
+_MASK64 = (1 << 64) - 1
+
+
+def splitmix64(seed, t):
+    """Output t (0-based) of the SplitMix64 generator started at ``seed``
+    (Steele, Lea, Flood, OOPSLA 2014; the state advances by the golden-ratio
+    increment before each output).  The device sampler keys its draws this
+    way; the reference draws through numpy's global generator instead."""
+    z = (seed + (t + 1) * 0x9E3779B97F4A7C15) & _MASK64
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _MASK64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _MASK64
+    return z ^ (z >> 31)
+
+
+def splitmix64_uniform(seed, trials):
+    """float64 uniforms in [0, 1): the top 53 bits of each output * 2^-53."""
+    return np.array([(splitmix64(seed, t) >> 11) * 2.0 ** -53 for t in range(trials)])
+
+
+def sample_icdf(tensor, trials, seed):
+    """Inverse-CDF draws over |psi|^2 keyed by SplitMix64: draw t is the
+    smallest index i with cumsum(p)_i > u_t * sum(p), u_t the t-th uniform.
+    Returns (indices, margin) where margin[t] is the distance of u_t * sum(p)
+    to the nearest CDF boundary (draws with a margin at rounding level may
+    legitimately differ between summation orders).  Restates the sampling
+    of states.py:121-129 (np.random.multinomial over probabilities()) as a
+    per-draw inverse CDF with a reproducible stream."""
+    p = probabilities(np.asarray(tensor)).ravel()
+    cdf = np.cumsum(p)
+    total = cdf[-1]
+    u = splitmix64_uniform(seed, trials) * total
+    u = np.minimum(u, np.nextafter(total, 0.0))
+    idx = np.searchsorted(cdf, u, side='right')
+    lo = np.where(idx > 0, cdf[np.maximum(idx - 1, 0)], 0.0)
+    margin = np.minimum(u - lo, cdf[idx] - u)
+    return idx.astype(np.int64), margin
+
+
+def sample_counts_reference(tensor, trials):
+    """quantumflow/states.py:121-129: multinomial counts over the
+    computational basis from numpy's global generator, shape [2]*N."""
+    probs = np.real(probabilities(np.asarray(tensor)))
+    res = np.random.multinomial(trials, probs.ravel())
+    return res.reshape(probs.shape)
+
+
+def measure_reference(tensor):
+    """quantumflow/states.py:149-160: one basis outcome as N bits."""
+    probs = np.real(probabilities(np.asarray(tensor)))
+    n = probs.ndim
+    indices = np.asarray(list(np.ndindex(*[2] * n)))
+    res = np.random.choice(probs.size, p=probs.ravel())
+    return indices[res]
+
+
+def normalize(tensor):
+    """quantumflow/states.py:108-111: tensor / sqrt(norm())."""
+    return tensor / np.sqrt(norm(tensor))
+
+
+def measure_qubit(tensor, axis, rand):
+    """quantumflow/stdops.py:53-65 with the random roll given: prob_zero is
+    P0(q).run(ket).norm(); rand < prob_zero keeps the |0> branch.  Returns
+    (post-measurement tensor, bit, prob_zero)."""
+    n = tensor.ndim
+    p0 = np.array([[1, 0], [0, 0]], dtype=CTYPE)
+    p1 = np.array([[0, 0], [0, 1]], dtype=CTYPE)
+    ket0 = tensormul(p0.reshape(2, 2), tensor, [axis])
+    prob_zero = float(np.real(norm(ket0)))
+    if rand < prob_zero:
+        return normalize(ket0).reshape([2] * n), 0, prob_zero
+    ket1 = tensormul(p1.reshape(2, 2), tensor, [axis])
+    return normalize(ket1).reshape([2] * n), 1, prob_zero
+
+
+def reset_qubits(tensor, axes):
+    """quantumflow/stdops.py:100-111: tensormul [[1, 1], [0, 0]] on each
+    axis in turn, then normalize."""
+    r = np.array([[1, 1], [0, 0]], dtype=CTYPE)
+    for a in axes:
+        tensor = tensormul(r, tensor, [a])
+    return normalize(tensor)
+
+
 def all_bitstrings(n):
     return np.array(list(itertools.product([0, 1], repeat=n)), dtype=np.int64)

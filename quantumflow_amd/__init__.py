@@ -15,6 +15,7 @@ from .states import *  # noqa: F401,F403
 from .ops import *  # noqa: F401,F403
 from .gates import *  # noqa: F401,F403
 from .stdgates import *  # noqa: F401,F403
+from .stdops import *  # noqa: F401,F403
 from .circuits import *  # noqa: F401,F403
 from .dagcircuit import *  # noqa: F401,F403
 from .measures import *  # noqa: F401,F403

@@ -59,9 +59,13 @@ SIGNATURES = {
     'qfx_expect_diag': (_i, [_vp, _vp, _vp, _vp, _i64, _vp]),
     'qfx_expect_z': (_i, [_vp, _vp, _vp, _i, _vp]),
     'qfx_norm2': (_i, [_vp, _vp, _vp, _i64, _vp]),
+    'qfx_qubit_probs': (_i, [_vp, _vp, _vp, _i, _i, _vp]),
     'qfx_permute_qubits': (_i, [_vp, _vp, _vp, _i, _ip]),
     'qfx_zero_state': (_i, [_vp, _vp, _i]),
     'qfx_plus_state': (_i, [_vp, _vp, _i]),
+    'qfx_sample_workspace_bytes': (_i64, [_i]),
+    'qfx_sample': (_i, [_vp, _vp, _vp, _i, _i64, ctypes.c_uint64, _vp]),
+    'qfx_histogram': (_i, [_vp, _vp, _vp, _i64, _i]),
     'qfx_cut_table': (_i, [_vp, _vp, _i, _vp, _vp, _i]),
     'qfx_cut_counts': (_i, [_vp, _vp, _i, _vp, _i]),
     'qfx_qaoa_check': (_i, [_vp, _i, _vp, _i, _vp, _vp, _i, _i, _i64, _i64, _i64, _i64, _i64, _i]),

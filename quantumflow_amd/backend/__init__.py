@@ -10,7 +10,8 @@ if BACKEND == 'rocm':
     from .rocmbk import (  # noqa: F401  (names star-import skips)
         __version__, name, version, pi, GateKind, classify, apply_gate,
         as_device, stream_handle, workspace, probabilities, expectation,
-        expectation_z, norm, zero_state_tensor)
+        expectation_z, norm, zero_state_tensor, qubit_probs, draw_seed,
+        sample_indices, sample_counts, sample_expectation)
 else:  # pragma: no cover  (config.py already rejected the name)
     raise ValueError('Unknown backend: ' + BACKEND)
 

@@ -443,6 +443,21 @@ def expectation_z(tensor: BKTensor) -> BKTensor:
     return out
 
 
+def qubit_probs(tensor: BKTensor, axis: int) -> BKTensor:
+    """[P(axis in |0>), P(axis in |1>)] (unnormalised sums of |psi|^2), float64
+    device tensor [2], one reduction pass (the P0(q).run(ket).norm() of
+    stdops.py:53-54 without forming the projected state)."""
+    a = as_device(tensor)
+    n = int(math.log2(a.numel()))
+    axis = int(axis)
+    if not 0 <= axis < n:
+        raise ValueError('qubit_probs: axis out of range')
+    out = torch.empty(2, dtype=FTYPE, device=a.device)
+    ws = workspace(_native.load().qfx_reduce_workspace_bytes(a.numel()))
+    _native.call('qfx_qubit_probs', stream_handle(), _ptr(out), _ptr(a), n, axis, _ptr(ws))
+    return out
+
+
 def norm(tensor: BKTensor) -> BKTensor:
     """|<psi|psi>| (the squared 2-norm, qubits.py:180-182), 0-dim float64."""
     a = as_device(tensor)
@@ -450,6 +465,53 @@ def norm(tensor: BKTensor) -> BKTensor:
     ws = workspace(_native.load().qfx_reduce_workspace_bytes(a.numel()))
     _native.call('qfx_norm2', stream_handle(), _ptr(res), _ptr(a), a.numel(), _ptr(ws))
     return res
+
+
+def draw_seed() -> int:
+    """A 64-bit seed for the device sampler, drawn from numpy's global
+    generator, so np.random.seed / set_random_seed make device sampling
+    reproducible as they do the reference's numpy draws."""
+    return int(np.random.randint(0, 2 ** 63 - 1, dtype=np.int64))
+
+
+def sample_indices(tensor: BKTensor, trials: int, seed: int) -> BKTensor:
+    """trials basis indices drawn from |psi|^2 (normalised on the device),
+    int64 device tensor; index bit N-1-a is qubit axis a."""
+    a = as_device(tensor)
+    n = int(math.log2(a.numel()))
+    if 2 ** n != a.numel() or n < 1:
+        raise ValueError('sample: state size is not a power of two')
+    trials = int(trials)
+    if trials < 0:
+        raise ValueError('sample: negative number of trials')
+    out = torch.empty(trials, dtype=torch.int64, device=a.device)
+    if trials == 0:
+        return out
+    ws = workspace(_native.load().qfx_sample_workspace_bytes(n))
+    _native.call('qfx_sample', stream_handle(), _ptr(out), _ptr(a), n, trials,
+                 int(seed) & (2 ** 64 - 1), _ptr(ws))
+    return out
+
+
+def sample_counts(tensor: BKTensor, trials: int, seed: int) -> BKTensor:
+    """Counts of each basis index over trials device draws, int64 [2]*N
+    (the shape of np.random.multinomial(trials, probs).reshape([2]*N))."""
+    a = as_device(tensor)
+    n = int(math.log2(a.numel()))
+    idx = sample_indices(a, trials, seed)
+    counts = torch.zeros(a.numel(), dtype=torch.int64, device=a.device)
+    _native.call('qfx_histogram', stream_handle(), _ptr(counts), _ptr(idx), idx.numel(), n)
+    return counts.view([2] * n)
+
+
+def sample_expectation(diag_hermitian: BKTensor, tensor: BKTensor, trials: int,
+                       seed: int) -> BKTensor:
+    """sum_i real(diag_i) counts_i / trials over ``trials`` device draws
+    (states.py:140-147 with trials given), 0-dim float64 device tensor."""
+    a = as_device(tensor)
+    d = _real_diag(diag_hermitian, a.numel())
+    idx = sample_indices(a, trials, seed)
+    return torch.sum(d[idx]) / int(trials)
 
 
 def zero_state_tensor(n: int) -> BKTensor:

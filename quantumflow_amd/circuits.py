@@ -62,37 +62,52 @@ class Circuit(Operation):
         qbs = sorted(qbs)
         return tuple(qbs)
 
-    def flat_gates(self):
-        """All elements as plain Gates in order, or None if any element is
-        not a Gate (nested circuits are flattened)."""
+    def flat_elements(self) -> list:
+        """All elements in order with nested circuits flattened."""
         out = []
         for elem in self.elements:
             if isinstance(elem, Circuit):
-                sub = elem.flat_gates()
-                if sub is None:
-                    return None
-                out.extend(sub)
-            elif isinstance(elem, Gate):
-                out.append(elem)
+                out.extend(elem.flat_elements())
             else:
-                return None
+                out.append(elem)
+        return out
+
+    def flat_gates(self):
+        """All elements as plain Gates in order, or None if any element is
+        not a Gate (nested circuits are flattened)."""
+        out = self.flat_elements()
+        if any(not isinstance(elem, Gate) for elem in out):
+            return None
         return out
 
     def run(self, ket: State = None, fused: bool = None) -> State:
         """Apply the circuit to ``ket`` (default |0...0> on the sorted
-        qubits).  ``fused=None`` picks the fused executor for plain-gate
-        circuits on >= FUSED_MIN_QUBITS qubits when QUANTUMFLOW_FUSION is on."""
+        qubits).  ``fused=None`` picks the fused executor for circuits on
+        >= FUSED_MIN_QUBITS qubits when QUANTUMFLOW_FUSION is on: each
+        maximal run of plain gates between other operations (measurements,
+        resets, classical control) goes through it; the others run one by
+        one in order."""
         if ket is None:
             ket = zero_state(qubits=self.qubits)
         if fused is None:
             fused = config.FUSION and ket.qubit_nb >= FUSED_MIN_QUBITS
-        if fused:
-            gates = self.flat_gates()
-            if gates is not None and gates:
-                from .executor import run_gates
-                return run_gates(gates, ket)
-        for elem in self.elements:
-            ket = elem.run(ket)
+        if not fused:
+            for elem in self.elements:
+                ket = elem.run(ket)
+            return ket
+        from .executor import run_gates
+        segment = []
+        for elem in self.flat_elements() + [None]:
+            if isinstance(elem, Gate):
+                segment.append(elem)
+                continue
+            if len(segment) > 1:
+                ket = run_gates(segment, ket)
+            elif segment:
+                ket = segment[0].run(ket)
+            segment = []
+            if elem is not None:
+                ket = elem.run(ket)
         return ket
 
     def asgate(self) -> Gate:

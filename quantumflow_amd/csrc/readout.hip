@@ -113,6 +113,23 @@ __global__ __launch_bounds__(kBlock) void reduce_stage2(const double2* __restric
   }
 }
 
+// partial[blk] = {sum |psi_i|^2 with bit b of i clear, ... with it set}
+__global__ __launch_bounds__(kBlock) void bit_prob_stage1(const double2* __restrict__ a,
+                                                         uint64_t len, int b,
+                                                         double2* __restrict__ partial) {
+  __shared__ double lds[2 * 4];
+  double acc[2] = {0.0, 0.0};
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < len; i += stride) {
+    const double2 x = a[i];
+    const double p = fma(x.x, x.x, x.y * x.y);
+    if ((i >> b) & 1) acc[1] += p;
+    else acc[0] += p;
+  }
+  block_sum<2>(acc, lds);
+  if (threadIdx.x == 0) partial[blockIdx.x] = make_double2(acc[0], acc[1]);
+}
+
 __global__ __launch_bounds__(kBlock) void abs_kernel(double* __restrict__ out,
                                                     const double2* __restrict__ in, uint64_t len,
                                                     int squared) {
@@ -236,6 +253,22 @@ int qfx_expect_diag(void* stream, void* result, const void* psi, const void* dia
 
 int qfx_norm2(void* stream, void* result, const void* psi, int64_t len, void* workspace) {
   return run_reduce<2>(qfx::as_stream(stream), result, psi, nullptr, len, workspace, "qfx_norm2");
+}
+
+int qfx_qubit_probs(void* stream, void* out, const void* psi, int n, int axis, void* workspace) {
+  if (!out || !psi || !workspace) return qfx::set_error(QFX_EINVAL, "qfx_qubit_probs: null pointer");
+  if (n < 1 || n > QFX_MAX_QUBITS) return qfx::set_error(QFX_ERANGE, "qfx_qubit_probs: n out of range");
+  if (axis < 0 || axis >= n) return qfx::set_error(QFX_EINVAL, "qfx_qubit_probs: axis out of range");
+  hipStream_t s = qfx::as_stream(stream);
+  const uint64_t len = (uint64_t)1 << n;
+  const int nb = red_blocks(len);
+  auto* part = static_cast<double2*>(workspace);
+  hipLaunchKernelGGL(bit_prob_stage1, dim3(nb), dim3(kBlock), 0, s,
+                     static_cast<const double2*>(psi), len, n - 1 - axis, part);
+  int rc = qfx::check_launch("qfx_qubit_probs");
+  if (rc) return rc;
+  hipLaunchKernelGGL(reduce_stage2<0>, dim3(1), dim3(kBlock), 0, s, part, nb, out);
+  return qfx::check_launch("qfx_qubit_probs");
 }
 
 int qfx_abs(void* stream, void* out, const void* in, int64_t len) {

@@ -73,29 +73,39 @@ class State:
         return bk.probabilities(self.tensor)
 
     def sample(self, trials: int) -> np.ndarray:
-        """Multinomial counts over basis states (states.py:121-129)."""
-        probs = np.real(bk.evaluate(self.probabilities()))
-        res = np.random.multinomial(trials, probs.ravel())
-        return res.reshape(probs.shape)
+        """Counts of each basis configuration over ``trials`` measurements,
+        as a host int64 array [2]*N (states.py:121-129).  The draws run on
+        the device (prefix sums of |psi|^2, inverse CDF per draw); the seed
+        comes from numpy's global generator, so np.random.seed makes them
+        reproducible."""
+        counts = bk.sample_counts(self.tensor, trials, bk.draw_seed())
+        return bk.evaluate(counts)
+
+    def sample_indices(self, trials: int, seed: int = None) -> bk.BKTensor:
+        """``trials`` flat basis indices drawn on the device, int64 device
+        tensor (index bit N-1-a is axis a); ``seed`` defaults to a draw from
+        numpy's global generator."""
+        if seed is None:
+            seed = bk.draw_seed()
+        return bk.sample_indices(self.tensor, trials, seed)
 
     def expectation(self, diag_hermitian: bk.TensorLike, trials: int = None) -> bk.BKTensor:
         """Expectation of a diagonal observable (states.py:131-147): exact
-        (one fused kernel) when trials is None, else from sampling."""
+        (one fused kernel) when trials is None, else the mean of the diagonal
+        over ``trials`` device draws."""
         if trials is None:
             return bk.expectation(diag_hermitian, self.tensor)
-        probs = self.sample(trials) / trials
-        d = np.real(np.asarray(bk.evaluate(diag_hermitian))).reshape(probs.shape)
-        return np.sum(d * probs)
+        return bk.sample_expectation(diag_hermitian, self.tensor, trials, bk.draw_seed())
 
     def expectation_z(self) -> bk.BKTensor:
         """<Z_i> for every qubit, in qubit order, one pass over the state."""
         return bk.expectation_z(self.tensor)
 
     def measure(self) -> np.ndarray:
-        """One computational-basis measurement (states.py:149-160)."""
-        probs = np.real(bk.evaluate(self.probabilities()))
-        res = np.random.choice(probs.size, p=probs.ravel() / probs.sum())
-        return np.array([(res >> (self.qubit_nb - 1 - a)) & 1 for a in range(self.qubit_nb)])
+        """One computational-basis measurement as N bits in axis order
+        (states.py:149-160), drawn on the device."""
+        idx = int(bk.evaluate(self.sample_indices(1))[0])
+        return np.array([(idx >> (self.qubit_nb - 1 - a)) & 1 for a in range(self.qubit_nb)])
 
     def __str__(self) -> str:
         state = self.vec.asarray()

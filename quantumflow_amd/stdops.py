@@ -1,0 +1,142 @@
+"""Measurement and reset operations on the device state
+(quantumflow/stdops.py:38-126).
+
+``Measure`` reads P(|0>) of one qubit with a single reduction kernel
+(``qfx_qubit_probs``), rolls numpy's global generator on the host exactly as
+the reference does (stdops.py:56), and collapses and renormalises the state in
+one diagonal-gate pass: diag(1/sqrt(p0), 0) or diag(0, 1/sqrt(p1)) on the
+measured qubit, which is P0/P1 followed by ``normalize`` (stdops.py:57,61)
+fused.  ``Reset`` applies [[1, 1], [0, 0]] to each qubit with the gate kernel
+and normalises (stdops.py:100-111).  ``Barrier``, ``If`` and ``Move`` are the
+control and classical-memory operations the reference's own measurement tests
+use (stdops.py:129-170, :279-282).
+"""
+
+from math import sqrt
+from typing import Any, Hashable
+
+import numpy as np
+
+from . import backend as bk
+from .ops import Operation
+from .qubits import Qubit, Qubits
+from .states import State
+
+__all__ = ['Measure', 'Reset', 'Barrier', 'If', 'Move']
+
+_RESET = np.array([[1, 1], [0, 0]], dtype=np.complex128)
+
+
+class Measure(Operation):
+    """Measure a qubit in the computational basis and, if ``cbit`` is given,
+    store the outcome in the state's classical memory (stdops.py:38-65)."""
+
+    def __init__(self, qubit: Qubit, cbit: Hashable = None) -> None:
+        self.qubit = qubit
+        self.cbit = cbit
+
+    def quil(self) -> str:
+        if self.cbit is not None:
+            return '{} {} {}'.format(self.name.upper(), self.qubit, self.cbit)
+        return '{} {}'.format(self.name.upper(), self.qubit)
+
+    @property
+    def qubits(self) -> Qubits:
+        return [self.qubit]
+
+    def run(self, ket: State) -> State:
+        axis = ket.qubits.index(self.qubit)
+        p = bk.evaluate(bk.qubit_probs(ket.tensor, axis))
+        prob_zero, prob_one = float(p[0]), float(p[1])
+        if np.random.random() < prob_zero:
+            bit, d = 0, [1.0 / sqrt(prob_zero), 0.0]
+        else:
+            # the reference divides by sqrt(norm) here too; a zero branch
+            # probability gives inf/nan amplitudes there as well
+            bit, d = 1, [0.0, 1.0 / sqrt(prob_one) if prob_one > 0 else float('inf')]
+        tensor = bk.tensormul(bk.classify(np.diag(np.asarray(d, dtype=np.complex128))),
+                              ket.tensor, [axis])
+        ket = State(tensor, ket.qubits, ket.memory)
+        if self.cbit is not None:
+            ket = ket.update({self.cbit: bit})
+        return ket
+
+
+class Reset(Operation):
+    """Reset qubits (all of the state's when none are given) to |0>
+    irrespective of the initial state (stdops.py:85-111)."""
+
+    def __init__(self, *qubits: Qubit) -> None:
+        self._qubits = tuple(qubits)
+
+    @property
+    def H(self) -> 'Reset':
+        return self  # Hermitian
+
+    def quil(self) -> str:
+        if self.qubits:
+            return 'RESET ' + ' '.join(str(q) for q in self.qubits)
+        return 'RESET'
+
+    def run(self, ket: State) -> State:
+        qubits = self.qubits if self.qubits else ket.qubits
+        kind = bk.classify(_RESET)
+        tensor = ket.tensor
+        for q in qubits:
+            tensor = bk.tensormul(kind, tensor, [ket.qubits.index(q)])
+        return State(tensor, ket.qubits, ket.memory).normalize()
+
+
+class Barrier(Operation):
+    """No-op that marks a scheduling barrier (stdops.py:129-148)."""
+
+    def __init__(self, *qubits: Qubit) -> None:
+        self._qubits = tuple(qubits)
+
+    @property
+    def H(self) -> 'Barrier':
+        return self
+
+    def quil(self) -> str:
+        return self.name.upper() + ' ' + ' '.join(str(q) for q in self.qubits)
+
+    def run(self, ket: State) -> State:
+        return ket
+
+
+class If(Operation):
+    """Apply ``elem`` when classical memory at ``condition`` equals
+    ``value`` (stdops.py:151-170)."""
+
+    def __init__(self, elem: Operation, condition: Hashable, value: bool = True) -> None:
+        self.element = elem
+        self.value = value
+        self.condition = condition
+
+    @property
+    def qubits(self) -> Qubits:
+        return self.element.qubits
+
+    def run(self, ket: State) -> State:
+        if ket.memory[self.condition] == self.value:
+            ket = self.element.run(ket)
+        return ket
+
+
+class Move(Operation):
+    """Copy a value (or another memory cell's value) into classical memory
+    (stdops.py:279-282; source lookup as BinaryOP._source, :201-204)."""
+
+    def __init__(self, target: Hashable, source: Any) -> None:
+        self.target = target
+        self.source = source
+
+    def _value(self, ket: State):
+        # a number is a constant; anything else names a memory cell
+        if isinstance(self.source, (bool, int, float, complex, np.number)):
+            return self.source
+        return ket.memory[self.source]
+
+    def run(self, ket: State) -> State:
+        return ket.update({self.target: self._value(ket)})
+
