@@ -173,6 +173,9 @@ typedef struct {
   int32_t sorted[5];  /* t[0..k-1] (register phase: t[0..3]) ascending */
   uint32_t active;    /* sparse: rows written */
   uint32_t phase;     /* sparse: rows multiplied by their phase */
+  uint32_t lds[10];   /* register phase: LDS slot offsets derived from t and
+                         the two maps (filled by qfx_tile_prepare_ops; other
+                         kinds: zero) */
 } qfx_tile_op;
 
 /* A register phase keeps, per lane, the 16 amplitudes spanned by its four
@@ -185,10 +188,12 @@ typedef struct {
  *      of coefs 0 and 3, imaginary parts of coefs 1 and 2 are read);
  *   4  16 phases, one per register slot (exactly 1 where `mask` has no
  *      bit);
- *   5  X on register bit code & 3 where every bit of the control mask
- *      code >> 2 is set (X, CNOT, CCNOT as register exchanges);
- *   6  exchange of register bits (a<b) = pair code & 7 where every bit of
- *      the control mask code >> 3 is set (SWAP, CSWAP).
+ *   5  X on register bit code & 3 where the register bits of the mask
+ *      (code >> 2) & 15 equal the values (code >> 6) & 15 (X, CNOT, CCNOT
+ *      and their negated-control forms as register exchanges).
+ * A register phase's `phase` and `active` words are its load and store
+ * maps: register slot r holds logical phase index c ^ (XOR of column j over
+ * the set bits j of r), column j in bits 4j..4j+3 and c in bits 16..19.
  *
  * A matrix-core block (kind 3) applies a dense 2^k x 2^k complex matrix M
  * with FP64 MFMA.  Its coefficients are the real form
@@ -206,11 +211,18 @@ typedef struct {
                              phase storing straight to HBM; needs
                              2^tile_bits == 16 * block, block <= 512 */
 
+/* Fills the derived `lds` words of every register phase in ops[0..nops-1]
+ * (host array) from its tile bits and maps; zeroes them for other kinds.
+ * Call after the last change to an op and before qfx_tile_check_ops. */
+int qfx_tile_prepare_ops(qfx_tile_op* ops /*host*/, int nops);
+
 /* Host-side validation of an op list against tile width T, the block size
  * and flags it will run with (ops on 4-5 qubits need QFX_TILE_WIDE and
  * block <= 256; matrix-core blocks need QFX_TILE_MFMA; other passes at most
- * 512 coefficients), ncoef coefficients and nsrcs source rows (host arrays).  The device entry point trusts its op
- * list; the Python planner checks every plan with this before upload. */
+ * 512 coefficients), ncoef coefficients and nsrcs source rows (host arrays);
+ * register phases must carry the `lds` words qfx_tile_prepare_ops gives
+ * them.  The device entry point trusts its op list; the Python planner
+ * checks every plan with this before upload. */
 int qfx_tile_check_ops(int T, int block, int flags, const qfx_tile_op* ops /*host*/,
                        int nops, int ncoef, const int* srcs /*host*/,
                        int nsrcs);

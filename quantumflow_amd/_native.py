@@ -30,7 +30,8 @@ class QfxTileOp(ctypes.Structure):
     _fields_ = [('kind', ctypes.c_int32), ('k', ctypes.c_int32),
                 ('coef', ctypes.c_int32), ('src', ctypes.c_int32),
                 ('t', ctypes.c_int32 * 5), ('sorted', ctypes.c_int32 * 5),
-                ('active', ctypes.c_uint32), ('phase', ctypes.c_uint32)]
+                ('active', ctypes.c_uint32), ('phase', ctypes.c_uint32),
+                ('lds', ctypes.c_uint32 * 10)]
 
 
 _vp = ctypes.c_void_p
@@ -78,6 +79,7 @@ SIGNATURES = {
     'qfx_mlp_mse': (_i, [_vp, _vp, _vp, _vp, _vp, _i64, _vp]),
     'qfx_mlp_adam': (_i, [_vp, _vp, _vp, _vp, _vp, _i64, ctypes.c_float, ctypes.c_float,
                           ctypes.c_float, ctypes.c_float, _i, ctypes.c_float]),
+    'qfx_tile_prepare_ops': (_i, [_vp, _i]),
     'qfx_tile_check_ops': (_i, [_i, _i, _i, _vp, _i, _i, _ip, _i]),
     'qfx_tile_pass': (_i, [_vp, _vp, _i, _i, _ip, _ip, _vp, _i, _vp, _i, _vp, _i, _i]),
 }

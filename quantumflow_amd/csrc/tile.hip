@@ -54,7 +54,7 @@ __device__ __forceinline__ unsigned loc_off(const int* t, unsigned r) {
 // Element e of a tile lives at LDS slot sw(e): the 3-bit chunks above bit 2
 // are folded into bits 0..2, so lanes whose elements differ only in higher
 // bits (strided amplitude groups) still hit different LDS banks.
-__device__ __forceinline__ unsigned sw(unsigned e) {
+__host__ __device__ __forceinline__ unsigned sw(unsigned e) {
   return e ^ (((e >> 3) ^ (e >> 6) ^ (e >> 9) ^ (e >> 12)) & 7u);
 }
 
@@ -313,27 +313,37 @@ __device__ __forceinline__ void apply_sub(v2d (&v)[kRegs], const int4 sub, const
 // (bit A <-> tile bit op.t[A]) sits at LDS slot sw(lb) ^ off(x), off XOR-
 // linear; its affine load map (op.phase) and store map (op.active) give
 // register r the logical index map(r) = c ^ (XOR of column j over the set
-// bits j of r), so slot addresses are XOR trees over 4 basis values.
+// bits j of r), so slot addresses are XOR trees over 4 basis values.  The
+// basis values are the same for every tile: the host derives them once
+// (phase_lds_words) into op.lds = {l[0..3], lc, s[0..3], sc}.
 struct PhaseLds {
   unsigned l[4], lc, s[4], sc;
 };
 
-__device__ __forceinline__ PhaseLds phase_lds(const qfx_tile_op& op) {
-  const unsigned w0 = sw(1u << op.t[0]), w1 = sw(1u << op.t[1]);
-  const unsigned w2 = sw(1u << op.t[2]), w3 = sw(1u << op.t[3]);
+__host__ __device__ inline void phase_lds_words(const qfx_tile_op& op, uint32_t (&w)[10]) {
+  unsigned b[4];
+  for (int j = 0; j < 4; ++j) b[j] = sw(1u << op.t[j]);
   auto off = [&](unsigned x) {
-    return ((x & 1u) ? w0 : 0u) ^ ((x & 2u) ? w1 : 0u) ^ ((x & 4u) ? w2 : 0u) ^
-           ((x & 8u) ? w3 : 0u);
+    return ((x & 1u) ? b[0] : 0u) ^ ((x & 2u) ? b[1] : 0u) ^ ((x & 4u) ? b[2] : 0u) ^
+           ((x & 8u) ? b[3] : 0u);
   };
-  const unsigned lm = op.phase, sm = op.active;
+  for (int j = 0; j < 4; ++j) {
+    w[j] = off((op.phase >> (4 * j)) & 15u);
+    w[5 + j] = off((op.active >> (4 * j)) & 15u);
+  }
+  w[4] = off(op.phase >> 16);
+  w[9] = off(op.active >> 16);
+}
+
+__device__ __forceinline__ PhaseLds phase_lds(const qfx_tile_op& op) {
   PhaseLds p;
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    p.l[j] = off((lm >> (4 * j)) & 15u);
-    p.s[j] = off((sm >> (4 * j)) & 15u);
+    p.l[j] = op.lds[j];
+    p.s[j] = op.lds[5 + j];
   }
-  p.lc = off(lm >> 16);
-  p.sc = off(sm >> 16);
+  p.lc = op.lds[4];
+  p.sc = op.lds[9];
   return p;
 }
 
@@ -663,16 +673,23 @@ __global__ __launch_bounds__(BS) void tile_pipe_kernel(v2d* state, int T, TilePo
     }
   };
 
+  // the tile number's bits are deposited at the non-tile index bits; adding
+  // the deposited stride with the tile bits set to one carries across them
+  uint64_t tmask = 0;
+#pragma unroll
+  for (int j = 0; j < kMaxTileBits; ++j)
+    if (j < T) tmask |= (uint64_t)1 << pos.s[j];
+  const uint64_t step = tile_base(gridDim.x);
   v2d pf[kPrefetch];
   uint32_t tile = blockIdx.x;       // the grid never exceeds ntiles
   uint64_t base = tile_base(tile);
   load_tile(pf, base);
-  for (; tile < ntiles; tile += gridDim.x) {
 #pragma unroll
-    for (int u = 0; u < kPrefetch; ++u) lds[sw(tid + u * BS)] = pf[u];
+  for (int u = 0; u < kPrefetch; ++u) lds[sw(tid + u * BS)] = pf[u];
+  for (; tile < ntiles; tile += gridDim.x) {
     // the last tile re-reads itself instead of branching around the loads
-    const uint32_t next = tile + gridDim.x < ntiles ? tile + gridDim.x : tile;
-    const uint64_t nbase = tile_base(next);
+    const uint64_t nbase =
+        tile + gridDim.x < ntiles ? ((base | tmask) + step) & ~tmask : base;
     load_tile(pf, nbase);
     __syncthreads();
     for (int i = 0; i < nops; ++i) {
@@ -692,6 +709,11 @@ __global__ __launch_bounds__(BS) void tile_pipe_kernel(v2d* state, int T, TilePo
     }
     base = nbase;
     __syncthreads();
+    // the next tile goes to LDS here, in the iteration that loaded it: the
+    // wait for its loads then counts the stores issued after them (vector
+    // memory operations complete in order) instead of draining them
+#pragma unroll
+    for (int u = 0; u < kPrefetch; ++u) lds[sw(tid + u * BS)] = pf[u];
   }
 }
 
@@ -788,16 +810,23 @@ __global__ __launch_bounds__(BS) void tile_reg_kernel(v2d* state, int T, TilePos
     }
   };
 
+  // the tile number's bits are deposited at the non-tile index bits; adding
+  // the deposited stride with the tile bits set to one carries across them
+  uint64_t tmask = 0;
+#pragma unroll
+  for (int j = 0; j < kMaxTileBits; ++j)
+    if (j < T) tmask |= (uint64_t)1 << pos.s[j];
+  const uint64_t step = tile_base(gridDim.x);
   v2d pf[kPrefetch];
   uint32_t tile = blockIdx.x;       // the grid never exceeds ntiles
   uint64_t base = tile_base(tile);
   load_tile(pf, base);
-  for (; tile < ntiles; tile += gridDim.x) {
 #pragma unroll
-    for (int u = 0; u < kPrefetch; ++u) lds[sw(tid + u * BS)] = pf[u];
+  for (int u = 0; u < kPrefetch; ++u) lds[sw(tid + u * BS)] = pf[u];
+  for (; tile < ntiles; tile += gridDim.x) {
     // the last tile re-reads itself instead of branching around the loads
-    const uint32_t next = tile + gridDim.x < ntiles ? tile + gridDim.x : tile;
-    const uint64_t nbase = tile_base(next);
+    const uint64_t nbase =
+        tile + gridDim.x < ntiles ? ((base | tmask) + step) & ~tmask : base;
     load_tile(pf, nbase);
     __syncthreads();
     for (int i = 0; i < nops - 1; ++i) {
@@ -813,6 +842,8 @@ __global__ __launch_bounds__(BS) void tile_reg_kernel(v2d* state, int T, TilePos
     }
     base = nbase;
     __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kPrefetch; ++u) lds[sw(tid + u * BS)] = pf[u];
   }
 }
 
@@ -919,6 +950,23 @@ int check_pass(int n, int T, const int* tile_pos, int nops, int block, const cha
 
 extern "C" {
 
+int qfx_tile_prepare_ops(qfx_tile_op* ops, int nops) {
+  if (nops < 0 || (!ops && nops > 0))
+    return qfx::set_error(QFX_EINVAL, "qfx_tile_prepare_ops: bad op list");
+  for (int i = 0; i < nops; ++i) {
+    qfx_tile_op& o = ops[i];
+    uint32_t w[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (o.kind == OP_RPHASE) {
+      for (int j = 0; j < kRegBits; ++j)
+        if (o.t[j] < 0 || o.t[j] >= kMaxTileBits)
+          return qfx::set_error(QFX_EINVAL, "qfx_tile_prepare_ops: register bit outside a tile");
+      phase_lds_words(o, w);
+    }
+    for (int j = 0; j < 10; ++j) o.lds[j] = w[j];
+  }
+  return QFX_OK;
+}
+
 int qfx_tile_check_ops(int T, int block, int flags, const qfx_tile_op* ops, int nops,
                        int ncoef, const int* srcs, int nsrcs) {
   if (flags & ~(QFX_TILE_WIDE | QFX_TILE_PERSISTENT | QFX_TILE_MFMA | QFX_TILE_REG_IO))
@@ -985,6 +1033,15 @@ int qfx_tile_check_ops(int T, int block, int flags, const qfx_tile_op* ops, int 
         }
         if ((m >> 20) != 0 || span != 0xFFFFu)
           return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: bad register-phase load/store map");
+      }
+      {
+        uint32_t w[10];
+        phase_lds_words(o, w);
+        for (int j = 0; j < 10; ++j)
+          if (o.lds[j] != w[j])
+            return qfx::set_error(QFX_EINVAL,
+                                  "qfx_tile_check_ops: register-phase LDS words do not match "
+                                  "its bits and maps (run qfx_tile_prepare_ops)");
       }
       if (o.coef < 0 || o.src < 0 || o.src % 4 != 0 || o.src + 4 * o.coef > nsrcs)
         return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: sub-ops out of range");

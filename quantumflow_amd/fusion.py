@@ -820,6 +820,7 @@ class HostPlan:
         self.n_passes = len(self.passes)
         self.n_ops = len(ops_host)
         self.ops_host, self.coefs, self.srcs = ops_host, coefs, srcs
+        prepare_ops(ops_host)
         # validate every pass with its own block and flags
         lib = _native.load()
         src_arr = _native.int_array(srcs)
@@ -868,6 +869,15 @@ class HostPlan:
                 'persistent': self.persistent,
                 'tile_bits': self.tile_bits, 'low_bits': self.low_bits,
                 'block': self.block, 'fuse_width': self.fuse_width}
+
+
+def prepare_ops(ops):
+    """Fill each register phase's derived LDS words in place
+    (qfx_tile_prepare_ops); run after the last change to the ops."""
+    lib = _native.load()
+    for o in ops:
+        _native.check(lib.qfx_tile_prepare_ops(_native.ctypes.byref(o), 1),
+                      'qfx_tile_prepare_ops')
 
 
 class TilePlan(HostPlan):

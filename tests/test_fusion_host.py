@@ -215,6 +215,7 @@ def test_register_phase_encoding_replays(T, seed):
             else:
                 ops.append(fusion._encode_op(item[1], pidx, coefs, srcs))
         nphase += len(ops)
+        fusion.prepare_ops(ops)
         arr = (_native.QfxTileOp * len(ops))(*ops)
         assert _native.load().qfx_tile_check_ops(T, 256, 0, arr, len(ops), len(coefs),
                                                  _native.int_array(srcs), len(srcs)) == 0
@@ -289,6 +290,7 @@ def test_register_phases_permutation_heavy(seed):
                 ops.append(fusion._encode_phase(item[1], item[2], pidx, T, coefs, srcs))
             else:
                 ops.append(fusion._encode_op(item[1], pidx, coefs, srcs))
+        fusion.prepare_ops(ops)
         arr = (_native.QfxTileOp * len(ops))(*ops)
         assert _native.load().qfx_tile_check_ops(T, 256, 0, arr, len(ops), len(coefs),
                                                  _native.int_array(srcs), len(srcs)) == 0
@@ -476,3 +478,41 @@ def test_pass_level_swap_relabel():
                            plan.coefs[c_lo:c_lo + ncoef], plan.srcs)
         ket = apply_store(ket, n, pos, spos)
     assert np.max(np.abs(ket - ref)) < 1e-12
+
+
+def _sw(e):
+    # tile element e lives at LDS slot sw(e): bits 3.., in groups of three,
+    # folded into bits 0..2
+    return e ^ (((e >> 3) ^ (e >> 6) ^ (e >> 9) ^ (e >> 12)) & 7)
+
+
+def test_prepared_lds_words_and_stale_words_rejected():
+    """qfx_tile_prepare_ops derives, per register phase, the LDS slot offset
+    of each load/store map column and constant: sw() of the tile-bit mask
+    the column selects.  The validator rejects a phase whose words do not
+    match its bits and maps."""
+    desc = random_mixed_circuit(10, 8, seed=4)
+    gates = build_gates(desc)
+    plan = fusion.HostPlan(gates, tuple(range(10)), tile_bits=8, low_bits=3, block=256,
+                           fuse_width=1, mode='reg', persistent=False)
+    phases = [o for o in plan.ops_host if o.kind == fusion.OP_RPHASE]
+    assert phases
+    for o in phases:
+        t = [o.t[j] for j in range(fusion.REG_BITS)]
+
+        def slot(x):
+            return _sw(sum(1 << t[j] for j in range(4) if (x >> j) & 1))
+        want = [slot((o.phase >> (4 * j)) & 15) for j in range(4)] + [slot(o.phase >> 16)]
+        want += [slot((o.active >> (4 * j)) & 15) for j in range(4)] + [slot(o.active >> 16)]
+        assert list(o.lds) == want
+    lib = _native.load()
+    o = phases[0]
+    arr = (_native.QfxTileOp * 1)(o)
+    srcs = _native.int_array(plan.srcs)
+    ncoef = len(plan.coefs)
+    assert lib.qfx_tile_check_ops(8, 256, 0, arr, 1, ncoef, srcs, len(plan.srcs)) == 0
+    arr[0].lds[9] ^= 1
+    assert lib.qfx_tile_check_ops(8, 256, 0, arr, 1, ncoef, srcs,
+                                  len(plan.srcs)) == _native.QFX_EINVAL
+    assert lib.qfx_tile_prepare_ops(arr, 1) == 0
+    assert lib.qfx_tile_check_ops(8, 256, 0, arr, 1, ncoef, srcs, len(plan.srcs)) == 0
