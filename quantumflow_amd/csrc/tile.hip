@@ -335,15 +335,17 @@ __host__ __device__ inline void phase_lds_words(const qfx_tile_op& op, uint32_t 
   w[9] = off(op.active >> 16);
 }
 
+// In bytes (slot x 16), so that an XOR tree yields the LDS byte address
+// itself and no per-access scaling is needed.
 __device__ __forceinline__ PhaseLds phase_lds(const qfx_tile_op& op) {
   PhaseLds p;
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    p.l[j] = op.lds[j];
-    p.s[j] = op.lds[5 + j];
+    p.l[j] = op.lds[j] << 4;
+    p.s[j] = op.lds[5 + j] << 4;
   }
-  p.lc = op.lds[4];
-  p.sc = op.lds[9];
+  p.lc = op.lds[4] << 4;
+  p.sc = op.lds[9] << 4;
   return p;
 }
 
@@ -368,7 +370,8 @@ __device__ __forceinline__ unsigned phase_lb(const qfx_tile_op& op, unsigned g) 
 
 __device__ __forceinline__ void run_subs(v2d (&v)[kRegs], const qfx_tile_op& op,
                                          const v2d* coef, const int* __restrict__ subs) {
-  // the next sub-op record is fetched while the current one runs
+  // the next record's scalar load is issued with the current one's (scalar
+  // loads complete out of order, so one wait covers both)
   const int4* rec = reinterpret_cast<const int4*>(subs + op.src);
   const int nsub = op.coef;
   int4 next = rec[0];
@@ -379,12 +382,22 @@ __device__ __forceinline__ void run_subs(v2d (&v)[kRegs], const qfx_tile_op& op,
   }
 }
 
-__device__ __forceinline__ void phase_lds_load(v2d (&v)[kRegs], const v2d* lds, unsigned slb,
+// The tile kernels declare no static LDS, so their dynamic LDS array starts
+// at LDS address 0 (launch_pass checks this) and an XOR tree of byte offsets
+// is the LDS address itself: no base add per access.
+typedef __attribute__((address_space(3))) v2d v2d_lds;
+
+__device__ __forceinline__ v2d_lds* lds_byte(unsigned a) {
+  return (v2d_lds*)(size_t)a;
+}
+
+// slb: the lane group's base slot in bytes (sw(...) << 4)
+__device__ __forceinline__ void phase_lds_load(v2d (&v)[kRegs], const v2d* /*lds*/, unsigned slb,
                                                const PhaseLds& p) {
   unsigned a[kRegs];
   xor_tree(a, slb ^ p.lc, p.l);
 #pragma unroll
-  for (int r = 0; r < kRegs; ++r) v[r] = lds[a[r]];
+  for (int r = 0; r < kRegs; ++r) v[r] = *lds_byte(a[r]);
 }
 
 __device__ __forceinline__ void phase_lds_store(const v2d (&v)[kRegs], v2d* lds, unsigned slb,
@@ -392,7 +405,7 @@ __device__ __forceinline__ void phase_lds_store(const v2d (&v)[kRegs], v2d* lds,
   unsigned a[kRegs];
   xor_tree(a, slb ^ p.sc, p.s);
 #pragma unroll
-  for (int r = 0; r < kRegs; ++r) lds[a[r]] = v[r];
+  for (int r = 0; r < kRegs; ++r) *lds_byte(a[r]) = v[r];
 }
 
 template <int BS>
@@ -402,7 +415,7 @@ __device__ __forceinline__ void op_rphase(v2d* lds, int T, const qfx_tile_op& op
   const int ngroups = 1 << (T - kRegBits);
   const PhaseLds p = phase_lds(op);
   for (int g = threadIdx.x; g < ngroups; g += BS) {
-    const unsigned slb = sw(phase_lb(op, (unsigned)g));
+    const unsigned slb = sw(phase_lb(op, (unsigned)g)) << 4;
     v2d v[kRegs];
     phase_lds_load(v, lds, slb, p);
     run_subs(v, op, coef, subs);
@@ -836,7 +849,7 @@ __global__ __launch_bounds__(BS) void tile_reg_kernel(v2d* state, int T, TilePos
     {
       const PhaseLds lp = phase_lds(last);
       v2d v[kRegs];
-      phase_lds_load(v, lds, sw(phase_lb(last, (unsigned)tid)), lp);
+      phase_lds_load(v, lds, sw(phase_lb(last, (unsigned)tid)) << 4, lp);
       run_subs(v, last, cf, srcs);
       phase_gstore(v, state + base + l_lane, lg);
     }
@@ -845,6 +858,20 @@ __global__ __launch_bounds__(BS) void tile_reg_kernel(v2d* state, int T, TilePos
 #pragma unroll
     for (int u = 0; u < kPrefetch; ++u) lds[sw(tid + u * BS)] = pf[u];
   }
+}
+
+// Register phases address LDS by absolute byte offsets (lds_byte), valid
+// only when the kernel's dynamic LDS array starts at address 0, i.e. the
+// kernel has no static LDS; also raises the dynamic LDS limit.
+int prepare_kernel(const void* fn, const char* name) {
+  hipFuncAttributes attr;
+  int rc = qfx::check_hip(hipFuncGetAttributes(&attr, fn), "hipFuncGetAttributes");
+  if (rc) return rc;
+  if (attr.sharedSizeBytes != 0)
+    return qfx::set_error(QFX_EKERNEL, std::string(name) + ": unexpected static LDS");
+  return qfx::check_hip(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                            160 * 1024),
+                        "hipFuncSetAttribute");
 }
 
 template <int BS, int LB, bool WIDE, bool MF>
@@ -858,15 +885,9 @@ int launch_pass(hipStream_t s, v2d* state, int n, int T, const TilePos& pos,
   static bool attr_set = false;
   static int occ[kMaxTileBits + 1];
   if (!attr_set) {
-    int rc = qfx::check_hip(hipFuncSetAttribute((const void*)tile_pass_kernel<BS, LB, WIDE, MF>,
-                                                hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                160 * 1024),
-                            "hipFuncSetAttribute");
+    int rc = prepare_kernel((const void*)tile_pass_kernel<BS, LB, WIDE, MF>, "tile_pass_kernel");
     if (rc) return rc;
-    rc = qfx::check_hip(hipFuncSetAttribute((const void*)tile_pipe_kernel<BS, LB, WIDE, MF>,
-                                            hipFuncAttributeMaxDynamicSharedMemorySize,
-                                            160 * 1024),
-                        "hipFuncSetAttribute");
+    rc = prepare_kernel((const void*)tile_pipe_kernel<BS, LB, WIDE, MF>, "tile_pipe_kernel");
     if (rc) return rc;
     for (int t = 0; t <= kMaxTileBits; ++t) occ[t] = 0;
     attr_set = true;
@@ -877,10 +898,7 @@ int launch_pass(hipStream_t s, v2d* state, int n, int T, const TilePos& pos,
       static int occ_reg[kMaxTileBits + 1];
       static bool reg_attr = false;
       if (!reg_attr) {
-        int rc = qfx::check_hip(hipFuncSetAttribute((const void*)tile_reg_kernel<BS, LB>,
-                                                    hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                    160 * 1024),
-                                "hipFuncSetAttribute");
+        int rc = prepare_kernel((const void*)tile_reg_kernel<BS, LB>, "tile_reg_kernel");
         if (rc) return rc;
         for (int t = 0; t <= kMaxTileBits; ++t) occ_reg[t] = 0;
         reg_attr = true;
