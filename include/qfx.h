@@ -284,7 +284,7 @@ typedef struct {
 } qfx_qaoa_inst;
 
 typedef struct {
-  int32_t T;          /* tile bits, <= 13 forward, <= 12 backward */
+  int32_t T;          /* tile bits, <= the program's tile_bits */
   int32_t pos[13];    /* tile bit positions, ascending */
   int32_t nstage;
   int32_t flags;      /* 1 load psi, 2 load lambda, 4 store psi, 8 store lambda */
@@ -312,15 +312,21 @@ typedef struct {
   void* out;              /* float64 [ninst][width] */
   void* sync;             /* >= 8 bytes of scratch: barrier counter + error flag */
   int32_t ninst, nk, width, mode, block, max_items_per_k;
+  int32_t tile_bits;      /* largest pass tile (LDS per workgroup: 2^tile_bits
+                             amplitudes, twice that backward); 0: 13 forward,
+                             12 backward.  Must be what qfx_qaoa_check
+                             accepted the program with. */
+  int32_t pad;
 } qfx_qaoa_args;
 
-/* Host-side validation of a program (host copies of the device arrays). */
+/* Host-side validation of a program (host copies of the device arrays);
+ * tile_bits as in qfx_qaoa_args (0: the mode's default). */
 int qfx_qaoa_check(const qfx_qaoa_inst* insts, int ninst,
                    const qfx_qaoa_pass* passes, int npasses,
                    const qfx_qaoa_item* items, const int* kstart, int nk,
                    int width, int64_t psi_pool_len, int64_t lam_pool_len,
                    int64_t cnt_pool_len, int64_t val_pool_len,
-                   int64_t npartial, int mode);
+                   int64_t npartial, int mode, int tile_bits);
 
 /* Enqueue the program: a barrier reset, the persistent kernel (grid = the
  * workgroups that fit the device at once) and the per-instance reduction

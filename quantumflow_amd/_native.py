@@ -69,7 +69,8 @@ SIGNATURES = {
     'qfx_histogram': (_i, [_vp, _vp, _vp, _i64, _i]),
     'qfx_cut_table': (_i, [_vp, _vp, _i, _vp, _vp, _i]),
     'qfx_cut_counts': (_i, [_vp, _vp, _i, _vp, _i]),
-    'qfx_qaoa_check': (_i, [_vp, _i, _vp, _i, _vp, _vp, _i, _i, _i64, _i64, _i64, _i64, _i64, _i]),
+    'qfx_qaoa_check': (_i, [_vp, _i, _vp, _i, _vp, _vp, _i, _i, _i64, _i64, _i64, _i64, _i64, _i,
+                            _i]),
     'qfx_qaoa_run': (_i, [_vp, _vp]),
     'qfx_qaoa_status': (_i, [_vp, _vp]),
     'qfx_mlp_gemm': (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i64, _i64, _i64, _i64, _vp, _i]),

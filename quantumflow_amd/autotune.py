@@ -123,14 +123,69 @@ def tune_circuit(gates, qubits, reps=2, candidates=None, store=True, verbose=Fal
     return best[0], results
 
 
-def choose_qaoa(default):
-    """Cached QAOA-engine configuration (or ``default``)."""
-    with _lock:
-        cached = _load().get('qaoa')
-    return dict(cached['config']) if cached else dict(default)
+# measured best on MI355X: 512 ER graphs n 10..22 p=4 (forward) and 64
+# 3-regular graphs n 16..24 p=8 (values + adjoint gradients)
+QAOA_DEFAULTS = {'forward': {'tile_bits': 13, 'block': 512},
+                 'backward': {'tile_bits': 12, 'block': 512}}
 
 
-def store_qaoa(cfg, ms):
+def qaoa_key(mode):
+    return 'qaoa:' + ('backward' if mode else 'forward')
+
+
+def choose_qaoa(mode):
+    """QAOA-engine configuration {tile_bits, block} for a mode (0 forward,
+    1 backward): the cached tuning result, else the built-in default."""
     with _lock:
-        _load()['qaoa'] = {'config': cfg, 'ms': ms}
+        cached = _load().get(qaoa_key(mode))
+    if cached:
+        return dict(cached['config'])
+    return dict(QAOA_DEFAULTS['backward' if mode else 'forward'])
+
+
+def store_qaoa(mode, cfg, ms):
+    with _lock:
+        _load()[qaoa_key(mode)] = {'config': dict(cfg), 'ms': ms}
         _save()
+
+
+def tune_qaoa(graphs, p, mode, reps=3, candidates=None, store=True, verbose=False):
+    """Time the batched engine on these instances for each candidate
+    {tile_bits, block} (one HIP-graph replay per repetition) and cache the
+    fastest for the mode.  Returns (best config, {config tuple: ms})."""
+    import numpy as np
+    import torch
+    from . import qaoa_engine as E
+    if candidates is None:
+        tmax = 12 if mode == E.MODE_BACKWARD else 13
+        candidates = [dict(tile_bits=t, block=b)
+                      for t, b in itertools.product((tmax, tmax - 1), (256, 512))]
+    rng = np.random.default_rng(0)
+    beta = rng.uniform(0, np.pi / 2, (len(graphs), p))
+    gamma = rng.uniform(0, np.pi, (len(graphs), p))
+    results = {}
+    best = None
+    for cfg in candidates:
+        batch = E.QAOABatch(graphs, p, mode=mode, **cfg)
+        batch.set_angles(beta, gamma)
+        batch.capture()
+        batch.replay()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            batch.replay()
+        e1.record()
+        torch.cuda.synchronize()
+        batch.check()
+        ms = e0.elapsed_time(e1) / reps
+        results[tuple(sorted(cfg.items()))] = ms
+        if verbose:
+            print('autotune qaoa', cfg, 'ms %.3f' % ms, flush=True)
+        if best is None or ms < best[1]:
+            best = (dict(cfg), ms)
+        del batch
+    if store:
+        store_qaoa(mode, best[0], best[1])
+    return best[0], results

@@ -74,17 +74,36 @@ struct Ctx {
   const double* cval;   // weighted cut values (or null)
 };
 
-// global index of tile element e
-__device__ __forceinline__ uint64_t gidx(const Ctx& c, unsigned e) {
-  uint64_t z = c.base;
-#pragma unroll
-  for (int j = 0; j < kMaxT; ++j)
-    if (j < c.T) z |= (uint64_t)((e >> j) & 1u) << c.ps->pos[j];
-  return z;
-}
-
 __device__ __forceinline__ double cut_of(const Ctx& c, uint64_t z) {
   return c.cnt ? (double)c.cnt[z] : c.cval[z];
+}
+
+// Global indices of a thread's tile elements e = threadIdx.x + k BS: the
+// deposit of e's bits at the tile positions splits into the thread's part
+// (bits below log2 BS, fixed per item) and k's part, which steps by one
+// carry-add per element (the other index bits set to one so the carry
+// crosses them).  Calls f(e, z) for every element the thread owns.
+template <int BS, typename F>
+__device__ __forceinline__ void for_elems(const Ctx& c, F&& f) {
+  constexpr int kLb = __builtin_ctz(BS);
+  const unsigned tsize = 1u << c.T;
+  if (threadIdx.x >= tsize) return;
+  uint64_t zl = c.base, hm = 0, h0 = 0;
+#pragma unroll
+  for (int j = 0; j < kMaxT; ++j) {
+    if (j >= c.T) continue;
+    const uint64_t bit = (uint64_t)1 << c.ps->pos[j];
+    if (j < kLb) zl |= ((threadIdx.x >> j) & 1u) ? bit : 0;
+    else hm |= bit;
+    if (j == kLb) h0 = bit;
+  }
+  const int nk = tsize > (unsigned)BS ? (int)(tsize / BS) : 1;
+  uint64_t zh = 0;
+#pragma unroll 4
+  for (int k = 0; k < nk; ++k) {
+    f(threadIdx.x + (unsigned)k * BS, zl | zh);
+    zh = ((zh | ~hm) + h0) & hm;
+  }
 }
 
 // Block-wide sum; result valid in thread 0.  red must hold >= 16 doubles.
@@ -142,7 +161,7 @@ __device__ __forceinline__ double xgrad_regs(const v2d (&l)[16], const v2d (&pv)
 // Apply RX to the tile bits in `mask` (chunks of <= 4 bits in registers);
 // with ADJ also to lambda (angle -beta) and accumulate the beta gradient.
 template <int BS, int R, bool ADJ>
-__device__ double rx_chunk(v2d* L, v2d* LL, int T, const int* bits, double c, double s) {
+__device__ __forceinline__ double rx_chunk(v2d* L, v2d* LL, int T, const int* bits, double c, double s) {
   const int ngroups = 1 << (T - R);
   int sorted[4];
 #pragma unroll
@@ -194,7 +213,7 @@ __device__ double rx_chunk(v2d* L, v2d* LL, int T, const int* bits, double c, do
 }
 
 template <int BS, bool ADJ>
-__device__ double rx_mask(v2d* L, v2d* LL, int T, unsigned mask, double theta) {
+__device__ __forceinline__ double rx_mask(v2d* L, v2d* LL, int T, unsigned mask, double theta) {
   const double c = cos(0.5 * theta), s = sin(0.5 * theta);
   double g = 0.0;
   while (mask) {
@@ -262,9 +281,9 @@ __device__ void run_item(const qfx_qaoa_inst& in, const qfx_qaoa_pass& ps, uint3
   v2d* lam = lam_pool ? lam_pool + in.lam_off : nullptr;
   if (threadIdx.x < kAccum) accum[threadIdx.x] = 0.0;
   if (ps.flags & PF_LOAD_PSI)
-    for (unsigned e = threadIdx.x; e < tsize; e += BS) L[swz(e)] = psi[gidx(c, e)];
+    for_elems<BS>(c, [&](unsigned e, uint64_t z) { L[swz(e)] = psi[z]; });
   if (ps.flags & PF_LOAD_LAM)
-    for (unsigned e = threadIdx.x; e < tsize; e += BS) LL[swz(e)] = lam[gidx(c, e)];
+    for_elems<BS>(c, [&](unsigned e, uint64_t z) { LL[swz(e)] = lam[z]; });
   __syncthreads();
   const bool unit = !in.weighted;
   for (int si = 0; si < ps.nstage; ++si) {
@@ -283,8 +302,7 @@ __device__ void run_item(const qfx_qaoa_inst& in, const qfx_qaoa_pass& ps, uint3
         const bool adj = type == ST_COST_ADJ;
         if (unit) phase_table<BS>(tab, in.nedges, g, in.wsum, adj);
         double acc = 0.0;
-        for (unsigned e = threadIdx.x; e < tsize; e += BS) {
-          const uint64_t z = gidx(c, e);
+        for_elems<BS>(c, [&](unsigned e, uint64_t z) {
           v2d ph;
           double cz;
           if (unit) {
@@ -306,7 +324,7 @@ __device__ void run_item(const qfx_qaoa_inst& in, const qfx_qaoa_pass& ps, uint3
             LL[sl] = cmul(ph, l);
           }
           L[sl] = cmul(ph, x);
-        }
+        });
         if (adj) {
           const double t = block_total<BS>(acc, red);
           if (threadIdx.x == 0) accum[1 + c.p + layer] += t;
@@ -327,31 +345,31 @@ __device__ void run_item(const qfx_qaoa_inst& in, const qfx_qaoa_pass& ps, uint3
       }
       case ST_EXPECT: {
         double acc = 0.0;
-        for (unsigned e = threadIdx.x; e < tsize; e += BS) {
+        for_elems<BS>(c, [&](unsigned e, uint64_t z) {
           const v2d x = L[swz(e)];
-          acc += fma(x.x, x.x, x.y * x.y) * cut_of(c, gidx(c, e));
-        }
+          acc += fma(x.x, x.x, x.y * x.y) * cut_of(c, z);
+        });
         const double t = block_total<BS>(acc, red);
         if (threadIdx.x == 0) accum[0] += t;
         __syncthreads();
         break;
       }
       case ST_MAKE_LAMBDA: {
-        for (unsigned e = threadIdx.x; e < tsize; e += BS) {
+        for_elems<BS>(c, [&](unsigned e, uint64_t z) {
           const unsigned sl = swz(e);
-          const double cz = cut_of(c, gidx(c, e));
+          const double cz = cut_of(c, z);
           const v2d x = L[sl];
           LL[sl] = v2d{cz * x.x, cz * x.y};
-        }
+        });
         __syncthreads();
         break;
       }
     }
   }
   if (ps.flags & PF_STORE_PSI)
-    for (unsigned e = threadIdx.x; e < tsize; e += BS) psi[gidx(c, e)] = L[swz(e)];
+    for_elems<BS>(c, [&](unsigned e, uint64_t z) { psi[z] = L[swz(e)]; });
   if (ps.flags & PF_STORE_LAM)
-    for (unsigned e = threadIdx.x; e < tsize; e += BS) lam[gidx(c, e)] = LL[swz(e)];
+    for_elems<BS>(c, [&](unsigned e, uint64_t z) { lam[z] = LL[swz(e)]; });
   __syncthreads();
   if ((int)threadIdx.x < width) partial[threadIdx.x] = accum[threadIdx.x];
   __syncthreads();
@@ -482,8 +500,11 @@ extern "C" {
 int qfx_qaoa_check(const qfx_qaoa_inst* insts, int ninst, const qfx_qaoa_pass* passes,
                    int npasses, const qfx_qaoa_item* items, const int* kstart, int nk, int width,
                    int64_t psi_pool_len, int64_t lam_pool_len, int64_t cnt_pool_len,
-                   int64_t val_pool_len, int64_t npartial, int mode) {
-  const int tmax = mode == QFX_QAOA_BACKWARD ? 12 : 13;
+                   int64_t val_pool_len, int64_t npartial, int mode, int tile_bits) {
+  const int tdef = mode == QFX_QAOA_BACKWARD ? 12 : 13;
+  if (tile_bits < 0 || tile_bits > tdef)
+    return qfx::set_error(QFX_ERANGE, "qfx_qaoa_check: tile_bits out of range");
+  const int tmax = tile_bits ? tile_bits : tdef;
   for (int i = 0; i < ninst; ++i) {
     const qfx_qaoa_inst& in = insts[i];
     if (in.n < 1 || in.n > 30 || in.p < 1 || in.p > kMaxPLayers)
@@ -511,6 +532,8 @@ int qfx_qaoa_check(const qfx_qaoa_inst* insts, int ninst, const qfx_qaoa_pass* p
     const qfx_qaoa_pass& ps = passes[q];
     if (ps.T < 1 || ps.T > tmax || ps.nstage < 0 || ps.nstage > QFX_QAOA_MAX_STAGES)
       return qfx::set_error(QFX_ERANGE, "qfx_qaoa_check: pass tile or stage count out of range");
+    if (mode != QFX_QAOA_BACKWARD && (ps.flags & (PF_LOAD_LAM | PF_STORE_LAM)))
+      return qfx::set_error(QFX_EINVAL, "qfx_qaoa_check: lambda traffic in a forward program");
     for (int j = 0; j < ps.T; ++j)
       if (ps.pos[j] < 0 || ps.pos[j] > 30 || (j && ps.pos[j] <= ps.pos[j - 1]))
         return qfx::set_error(QFX_EINVAL, "qfx_qaoa_check: tile positions not ascending");
@@ -521,6 +544,8 @@ int qfx_qaoa_check(const qfx_qaoa_inst* insts, int ninst, const qfx_qaoa_pass* p
         return qfx::set_error(QFX_EINVAL, "qfx_qaoa_check: bad stage");
       if ((ty == ST_RX || ty == ST_RX_ADJ) && (mask >> ps.T))
         return qfx::set_error(QFX_EINVAL, "qfx_qaoa_check: RX mask outside the tile");
+      if (mode != QFX_QAOA_BACKWARD && ty >= ST_MAKE_LAMBDA)
+        return qfx::set_error(QFX_EINVAL, "qfx_qaoa_check: backward stage in a forward program");
     }
   }
   if (nk < 0 || kstart[0] != 0)
@@ -554,7 +579,10 @@ int qfx_qaoa_run(void* stream, const qfx_qaoa_args* args) {
     return qfx::set_error(QFX_EINVAL, "qfx_qaoa_run: backward needs lam_pool");
   if (a.width < 1 || a.width > kAccum || a.ninst < 1 || a.nk < 1)
     return qfx::set_error(QFX_ERANGE, "qfx_qaoa_run: sizes out of range");
-  const int T = a.mode == QFX_QAOA_BACKWARD ? 12 : 13;
+  const int tdef = a.mode == QFX_QAOA_BACKWARD ? 12 : 13;
+  if (a.tile_bits < 0 || a.tile_bits > tdef)
+    return qfx::set_error(QFX_ERANGE, "qfx_qaoa_run: tile_bits out of range");
+  const int T = a.tile_bits ? a.tile_bits : tdef;
   hipStream_t s = qfx::as_stream(stream);
   switch (a.block) {
     case 256: return launch_qaoa<256>(s, a, T);

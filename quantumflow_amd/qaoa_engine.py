@@ -61,7 +61,8 @@ class QaoaArgs(ctypes.Structure):
                 ('insts', 'passes', 'items', 'kstart', 'angles', 'psi_pool', 'lam_pool',
                  'cnt_pool', 'val_pool', 'partials', 'out', 'sync')] + \
                [(nm, ctypes.c_int32) for nm in
-                ('ninst', 'nk', 'width', 'mode', 'block', 'max_items_per_k')]
+                ('ninst', 'nk', 'width', 'mode', 'block', 'max_items_per_k', 'tile_bits',
+                 'pad')]
 
 
 def tile_sets(n, T, low=None):
@@ -159,15 +160,23 @@ class QAOABatch:
     optional); p: layers; mode: MODE_FORWARD (values) or MODE_BACKWARD
     (values and gradients)."""
 
-    def __init__(self, graphs, p, mode=MODE_BACKWARD, block=256):
+    def __init__(self, graphs, p, mode=MODE_BACKWARD, block=None, tile_bits=None):
         if not 1 <= p <= MAX_LAYERS:
             raise ValueError('p must be 1..%d' % MAX_LAYERS)
+        if block is None or tile_bits is None:
+            from .autotune import choose_qaoa
+            tuned = choose_qaoa(mode)
+            block = tuned['block'] if block is None else block
+            tile_bits = tuned['tile_bits'] if tile_bits is None else tile_bits
         self.graphs = list(graphs)
         self.p = p
         self.mode = mode
         self.block = block
         self.width = 1 if mode == MODE_FORWARD else 1 + 2 * p
-        T = 13 if mode == MODE_FORWARD else 12
+        T_max = 13 if mode == MODE_FORWARD else 12
+        T = T_max if tile_bits is None else int(tile_bits)
+        if not 6 <= T <= T_max:
+            raise ValueError('tile_bits must be 6..%d in this mode' % T_max)
         self.T = T
         dev = bk.DEVICE
         bk.stream_handle()                          # initialise the device
@@ -259,7 +268,7 @@ class QAOABatch:
                                          self.nk, self.width, psi_len,
                                          lam_len if mode == MODE_BACKWARD else 0,
                                          max(cnt_len, 1) if cnt_len else 0,
-                                         max(val_len, 1) if val_len else 0, nslots, mode),
+                                         max(val_len, 1) if val_len else 0, nslots, mode, T),
                       'qfx_qaoa_check')
 
         def up(arr):
@@ -304,6 +313,7 @@ class QAOABatch:
             self.sync.data_ptr()
         a.ninst, a.nk, a.width, a.mode, a.block = ninst, self.nk, self.width, mode, block
         a.max_items_per_k = max(kstart[k + 1] - kstart[k] for k in range(self.nk))
+        a.tile_bits = T
         self._graph = None
 
     @staticmethod
