@@ -423,17 +423,25 @@ __global__ __launch_bounds__(BS) void qaoa_kernel(
   }
 }
 
-// out[i * width + j] = sum of the instance's partial slots, in slot order
-__global__ void qaoa_reduce_kernel(const qfx_qaoa_inst* __restrict__ insts, int ninst,
-                                   const double* __restrict__ partials, int width,
-                                   double* __restrict__ out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= ninst * width) return;
+// out[i * width + j] = sum of the instance's partial slots: one workgroup
+// per output, a fixed strided split over its threads and a fixed-order tree
+// (bitwise reproducible).
+constexpr int kReduceBlock = 256;
+
+__global__ __launch_bounds__(kReduceBlock) void qaoa_reduce_kernel(
+    const qfx_qaoa_inst* __restrict__ insts, int ninst, const double* __restrict__ partials,
+    int width, double* __restrict__ out) {
+  const int i = blockIdx.x;
   const int inst = i / width, j = i % width;
   const qfx_qaoa_inst& in = insts[inst];
   double s = 0.0;
-  for (int q = 0; q < in.nslots; ++q) s += partials[(uint64_t)(in.slot0 + q) * width + j];
-  out[i] = s;
+  for (int q = threadIdx.x; q < in.nslots; q += kReduceBlock)
+    s += partials[(uint64_t)(in.slot0 + q) * width + j];
+  __shared__ double red[kReduceBlock / 64];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) out[i] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
 __global__ void zero_sync_kernel(unsigned* sync, int* err) {
@@ -485,8 +493,8 @@ int launch_qaoa(hipStream_t s, const qfx_qaoa_args& a, int T) {
                      a.width, sync, err);
   rc = qfx::check_launch("qaoa_kernel");
   if (rc) return rc;
-  const int nthr = a.ninst * a.width;
-  hipLaunchKernelGGL(qaoa_reduce_kernel, dim3((nthr + 255) / 256), dim3(256), 0, s,
+  const int nout = a.ninst * a.width;
+  hipLaunchKernelGGL(qaoa_reduce_kernel, dim3(nout), dim3(kReduceBlock), 0, s,
                      static_cast<const qfx_qaoa_inst*>(a.insts), a.ninst,
                      static_cast<const double*>(a.partials), a.width,
                      static_cast<double*>(a.out));
