@@ -160,6 +160,9 @@ __device__ __forceinline__ void op_sparse(v2d* lds, int T, const qfx_tile_op& op
 //   SUB_D1X 2x2 matrix with real diagonal and imaginary off-diagonal
 //           (RX-like: real parts of m00, m11, imaginary parts of m01, m10)
 //   SUB_DIAG  16 phases (one per slot; exactly 1 where `mask` has no bit)
+//   SUB_DIAGV 2^m tables of 16 phases, m = code & 3 (1..3): table v where
+//           bit j of v is the tile's index bit (code >> (2 + 6 j)) & 63, a
+//           bit outside the tile (diagonal gates on qubits outside the tile)
 //   SUB_CX  X on register bit t = code & 3 for the slots whose bits under
 //           the control mask (code >> 2) & 15 equal (code >> 6) & 15: a pure
 //           register exchange (CNOT, CCNOT; controlled SWAP as three)
@@ -174,7 +177,7 @@ __device__ __forceinline__ void op_sparse(v2d* lds, int T, const qfx_tile_op& op
 // matrix next to 16 register amplitudes would raise the register allocation
 // of the whole kernel past two waves per SIMD.
 enum SubKind : int {
-  SUB_D1 = 1, SUB_DIAG = 4, SUB_CX = 5, SUB_D1R = 7, SUB_D1X = 8
+  SUB_D1 = 1, SUB_DIAG = 4, SUB_CX = 5, SUB_D1R = 7, SUB_D1X = 8, SUB_DIAGV = 9
 };
 constexpr int kRegBits = 4;
 constexpr int kRegs = 1 << kRegBits;
@@ -262,7 +265,8 @@ __device__ __forceinline__ void sub_cx(v2d (&v)[kRegs], unsigned cm, unsigned cv
 // type and code below is one of the listed cases; the unreachable defaults
 // let the dispatch drop its do-nothing path, which would otherwise copy all
 // 16 register amplitudes on every sub-op.
-__device__ __forceinline__ void apply_sub(v2d (&v)[kRegs], const int4 sub, const v2d* coef) {
+__device__ __forceinline__ void apply_sub(v2d (&v)[kRegs], const int4 sub, const v2d* coef,
+                                          uint64_t gbase) {
   const int type = sub.x, code = sub.y;
   const v2d* m = coef + sub.z;
   switch (type) {
@@ -294,6 +298,16 @@ __device__ __forceinline__ void apply_sub(v2d (&v)[kRegs], const int4 sub, const
       }
       break;
     case SUB_DIAG: sub_diag(v, m); break;
+    case SUB_DIAGV: {
+      // gbase: the tile's index with its tile bits zero (uniform)
+      const unsigned c = (unsigned)code;
+      unsigned sel = 0;
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        if (j < (int)(c & 3u)) sel |= (unsigned)((gbase >> ((c >> (2 + 6 * j)) & 63u)) & 1u) << j;
+      sub_diag(v, m + 16 * sel);
+      break;
+    }
     case SUB_CX: {
       const unsigned cm = ((unsigned)code >> 2) & 15u, cv = ((unsigned)code >> 6) & 15u;
       switch (code & 3) {
@@ -369,7 +383,8 @@ __device__ __forceinline__ unsigned phase_lb(const qfx_tile_op& op, unsigned g) 
 }
 
 __device__ __forceinline__ void run_subs(v2d (&v)[kRegs], const qfx_tile_op& op,
-                                         const v2d* coef, const int* __restrict__ subs) {
+                                         const v2d* coef, const int* __restrict__ subs,
+                                         uint64_t gbase) {
   // the next record's scalar load is issued with the current one's (scalar
   // loads complete out of order, so one wait covers both)
   const int4* rec = reinterpret_cast<const int4*>(subs + op.src);
@@ -378,7 +393,7 @@ __device__ __forceinline__ void run_subs(v2d (&v)[kRegs], const qfx_tile_op& op,
   for (int i = 0; i < nsub; ++i) {
     const int4 cur = next;
     next = rec[i + 1 < nsub ? i + 1 : i];
-    apply_sub(v, cur, coef);
+    apply_sub(v, cur, coef, gbase);
   }
 }
 
@@ -411,14 +426,14 @@ __device__ __forceinline__ void phase_lds_store(const v2d (&v)[kRegs], v2d* lds,
 template <int BS>
 __device__ __forceinline__ void op_rphase(v2d* lds, int T, const qfx_tile_op& op,
                                           const v2d* coef,
-                                          const int* __restrict__ subs) {
+                                          const int* __restrict__ subs, uint64_t gbase) {
   const int ngroups = 1 << (T - kRegBits);
   const PhaseLds p = phase_lds(op);
   for (int g = threadIdx.x; g < ngroups; g += BS) {
     const unsigned slb = sw(phase_lb(op, (unsigned)g)) << 4;
     v2d v[kRegs];
     phase_lds_load(v, lds, slb, p);
-    run_subs(v, op, coef, subs);
+    run_subs(v, op, coef, subs, gbase);
     phase_lds_store(v, lds, slb, p);
   }
 }
@@ -505,10 +520,10 @@ template <int BS, bool WIDE, bool MF>
 __device__ __forceinline__ void apply_op(v2d* lds, int T, const qfx_tile_op& op,
                                          const v2d* coef,
                                          const v2d* __restrict__ coef_g,
-                                         const int* __restrict__ srcs) {
+                                         const int* __restrict__ srcs, uint64_t gbase) {
   if (op.kind == OP_RPHASE) {
     // 16 register amplitudes per lane exceed what 1024-lane blocks leave
-    if constexpr (BS <= kRphaseMaxBlock) op_rphase<BS>(lds, T, op, coef, srcs);
+    if constexpr (BS <= kRphaseMaxBlock) op_rphase<BS>(lds, T, op, coef, srcs, gbase);
   } else if (op.kind == OP_MFMA) {
     if constexpr (MF && BS <= kMfmaMaxBlock) {
       switch (op.k) {
@@ -608,8 +623,8 @@ __global__ __launch_bounds__(BS) void tile_pass_kernel(v2d* state, int T, TilePo
   }
   __syncthreads();
   for (int i = 0; i < nops; ++i) {
-    if constexpr (kStage) apply_op<BS, WIDE, MF>(lds, T, ops[i], cf, coef, srcs);
-    else apply_op<BS, WIDE, MF>(lds, T, ops[i], coef, coef, srcs);
+    if constexpr (kStage) apply_op<BS, WIDE, MF>(lds, T, ops[i], cf, coef, srcs, base);
+    else apply_op<BS, WIDE, MF>(lds, T, ops[i], coef, coef, srcs, base);
     __syncthreads();
   }
   v2d* dst = state + base + soff;
@@ -706,8 +721,8 @@ __global__ __launch_bounds__(BS) void tile_pipe_kernel(v2d* state, int T, TilePo
     load_tile(pf, nbase);
     __syncthreads();
     for (int i = 0; i < nops; ++i) {
-      if constexpr (kStage) apply_op<BS, WIDE, MF>(lds, T, ops[i], cf, coef, srcs);
-      else apply_op<BS, WIDE, MF>(lds, T, ops[i], coef, coef, srcs);
+      if constexpr (kStage) apply_op<BS, WIDE, MF>(lds, T, ops[i], cf, coef, srcs, base);
+      else apply_op<BS, WIDE, MF>(lds, T, ops[i], coef, coef, srcs, base);
       __syncthreads();
     }
     v2d* q0 = state + base + soff;
@@ -843,14 +858,14 @@ __global__ __launch_bounds__(BS) void tile_reg_kernel(v2d* state, int T, TilePos
     load_tile(pf, nbase);
     __syncthreads();
     for (int i = 0; i < nops - 1; ++i) {
-      apply_op<BS, false, false>(lds, T, ops[i], cf, coef, srcs);
+      apply_op<BS, false, false>(lds, T, ops[i], cf, coef, srcs, base);
       __syncthreads();
     }
     {
       const PhaseLds lp = phase_lds(last);
       v2d v[kRegs];
       phase_lds_load(v, lds, sw(phase_lb(last, (unsigned)tid)) << 4, lp);
-      run_subs(v, last, cf, srcs);
+      run_subs(v, last, cf, srcs, base);
       phase_gstore(v, state + base + l_lane, lg);
     }
     base = nbase;
@@ -1067,9 +1082,18 @@ int qfx_tile_check_ops(int T, int block, int flags, const qfx_tile_op* ops, int 
         const int* sub = srcs + o.src + 4 * q;
         // codes per type: D1/D1R/D1X 4, DIAG 1, CX 4 targets x 16 control
         // masks x 16 control values; control masks may not touch the target
+        // DIAGV: 1..3 selector bits, each an index bit < 64, 16 << m coefs
         static const int ncode[9] = {0, 4, 0, 0, 1, 1024, 0, 4, 4};
         static const int need[9] = {0, 4, 0, 0, kRegs, 0, 0, 4, 4};
         const int ty = sub[0], cd = sub[1];
+        if (ty == SUB_DIAGV) {
+          const unsigned c = (unsigned)cd, m = c & 3u;
+          if (m == 0 || (c >> (2 + 6 * m)) != 0)
+            return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: bad selector diagonal");
+          if (sub[2] < 0 || sub[2] + (kRegs << m) > ncoef)
+            return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: sub-op coefficients out of range");
+          continue;
+        }
         if (ty < SUB_D1 || ty > SUB_D1X || ncode[ty] == 0 || cd < 0 || cd >= ncode[ty])
           return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: bad sub-op");
         if (ty == SUB_CX) {

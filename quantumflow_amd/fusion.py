@@ -38,7 +38,8 @@ from . import backend as bk
 from .states import State
 
 OP_DENSE, OP_SPARSE, OP_RPHASE, OP_MFMA = 0, 1, 2, 3
-SUB_D1, SUB_DIAG, SUB_CX, SUB_D1R, SUB_D1X = 1, 4, 5, 7, 8
+SUB_D1, SUB_DIAG, SUB_CX, SUB_D1R, SUB_D1X, SUB_DIAGV = 1, 4, 5, 7, 8, 9
+MAX_SEL_BITS = 3          # index bits outside the tile a register diagonal may read
 REG_BITS = 4
 MAX_TILE_BITS = 13
 MIN_TILE_BITS = 6
@@ -110,40 +111,42 @@ def apply_gates_inplace(gates, qubits, work, prof=None):
 # planning
 # ---------------------------------------------------------------------------
 
-def _absorb(bits_list, pending, S, ok=None):
+def _absorb(bits_list, pending, S, ok=None, free=None):
     """Items (indices into bits_list, in order) a set of bits S takes: an item
-    joins if it is allowed, its bits lie in S and it touches no bit of an
-    item left behind, so every item it overtakes acts on other bits and
+    joins if it is allowed, its bits lie in S (or it is ``free``: a diagonal
+    gate that needs none of its bits in the tile) and it touches no bit of
+    an item left behind, so every item it overtakes acts on other bits and
     commutes with it."""
     blocked = set()
     taken = []
     for i in pending:
         b = bits_list[i]
-        if blocked.intersection(b) or not S.issuperset(b) or (ok is not None and not ok[i]):
+        if (blocked.intersection(b) or (ok is not None and not ok[i])
+                or not (S.issuperset(b) or (free is not None and free[i]))):
             blocked.update(b)
             continue
         taken.append(i)
     return taken
 
 
-def _grow(bits_list, pending, S, width, ok=None, cap=40):
+def _grow(bits_list, pending, S, width, ok=None, cap=40, free=None):
     """Grow the bit set S to at most ``width`` bits, one candidate at a time:
     the candidates are the bit sets of the first ``cap`` distinct items not
     yet taken (in circuit order), and the one that takes the most further
     items per added bit wins.  Returns (S, items taken)."""
-    cur = _absorb(bits_list, pending, S, ok)
+    cur = _absorb(bits_list, pending, S, ok, free)
     while len(S) < width:
         taken = set(cur)
         best = None
         seen = set()
         for i in pending:
-            if i in taken or (ok is not None and not ok[i]):
+            if i in taken or (ok is not None and not ok[i]) or (free is not None and free[i]):
                 continue
             new = frozenset(bits_list[i]) - S
             if not new or new in seen or len(S) + len(new) > width:
                 continue
             seen.add(new)
-            got = _absorb(bits_list, pending, S | new, ok)
+            got = _absorb(bits_list, pending, S | new, ok, free)
             gain = (len(got) - len(cur)) / len(new)
             if best is None or gain > best[0]:
                 best = (gain, new, got)
@@ -156,10 +159,13 @@ def _grow(bits_list, pending, S, width, ok=None, cap=40):
     return S, cur
 
 
-def plan_passes(bits_list, n, T, L, tileable=None, strategy='gain'):
+def plan_passes(bits_list, n, T, L, tileable=None, strategy='gain', free=None):
     """Group gates (given by their index-bit tuples, in circuit order) into
     passes.  Returns a list of (sorted tile bits, [gate indices]); a gate
     that is not tileable forms a pass of its own with tile bits None.
+    ``free[i]`` marks diagonal gates that a pass can apply whatever its tile
+    bits (register phases select their phases by the tile's other index
+    bits): they join the first pass that reaches them and add no tile bits.
 
     strategy 'gain' grows each pass's tile set by ``_grow`` (the bits that
     take the most gates per added bit); 'greedy' takes gates first come,
@@ -175,7 +181,7 @@ def plan_passes(bits_list, n, T, L, tileable=None, strategy='gain'):
             pending = pending[1:]
             continue
         if strategy == 'gain':
-            S, _ = _grow(bits_list, pending, set(low), T, tileable)
+            S, _ = _grow(bits_list, pending, set(low), T, tileable, free=free)
         else:
             S = set(low)
             blocked = set()
@@ -194,7 +200,7 @@ def plan_passes(bits_list, n, T, L, tileable=None, strategy='gain'):
             if b not in S:
                 S.add(b)
             b += 1
-        taken = _absorb(bits_list, pending, S, tileable)
+        taken = _absorb(bits_list, pending, S, tileable, free)
         passes.append((sorted(S), taken))
         done = set(taken)
         pending = [g for g in pending if g not in done]
@@ -210,6 +216,35 @@ def embed(matrix, pos, nq):
     out = np.tensordot(g, t, axes=(list(range(k, 2 * k)), list(pos)))
     out = np.moveaxis(out, list(range(k)), list(pos))
     return out.reshape(2 ** nq, 2 ** nq)
+
+
+class SelDiagonal:
+    """A diagonal gate of a pass, part of whose qubits lie outside the tile:
+    a diagonal on its in-tile bits whose phases depend on the tile's other
+    index bits ``sel`` (global index bits).  ``tables[v]`` is the diagonal on
+    the in-tile bits (gate order) when bit j of v is the value of index bit
+    sel[j].  Duck-types a diagonal GateKind."""
+    kind = 'diagonal'
+    src = phase = matrix = None
+
+    def __init__(self, kind, bits, tile_bits):
+        ins = [j for j, b in enumerate(bits) if b in tile_bits]
+        outs = [j for j, b in enumerate(bits) if b not in tile_bits]
+        self.in_bits = [bits[j] for j in ins]
+        self.sel = [bits[j] for j in outs]
+        self.k = len(ins)
+        k = len(bits)
+        tables = np.empty((2 ** len(outs), 2 ** len(ins)), dtype=np.complex128)
+        for v in range(2 ** len(outs)):
+            for x in range(2 ** len(ins)):
+                r = 0
+                for a, j in enumerate(ins):
+                    r |= ((x >> (len(ins) - 1 - a)) & 1) << (k - 1 - j)
+                for a, j in enumerate(outs):
+                    r |= ((v >> a) & 1) << (k - 1 - j)
+                tables[v, x] = kind.diag[r]
+        self.tables = tables
+        self.diag = None
 
 
 class Op:
@@ -234,6 +269,8 @@ class Op:
     def active_rows(self):
         kd = self.kind
         d = 2 ** kd.k
+        if isinstance(kd, SelDiagonal):
+            return d
         if kd.kind == bk.GateKind.DIAGONAL:
             return int(np.sum(kd.diag != 1.0))
         src = kd.src
@@ -463,19 +500,38 @@ def _encode_phase(R, members, pos_index, T, coefs, subs):
     # Diagonal factors wait in `pending` (register space) and are emitted as
     # one SUB_DIAG only before a 1-qubit sub-op on a bit they depend on, or
     # at the end: an exchange E moves past a pending diagonal D as
-    # E D = D' E with D'[i] = D[E(i)].
+    # E D = D' E with D'[i] = D[E(i)].  A pending diagonal is a table
+    # [2^m, 16]: row v holds the phases when bit j of v is the value of the
+    # index bit sel[j] outside the tile (m <= MAX_SEL_BITS; SUB_DIAGV).
     pending = [None]
     regs16 = np.arange(2 ** REG_BITS)
 
-    def push_diag(ph_log):
-        ph = np.array([ph_log[phi(i)] for i in range(2 ** REG_BITS)])
-        pending[0] = ph if pending[0] is None else pending[0] * ph
+    def expand(tab, sel, union):
+        rows = [sum(((v >> union.index(b)) & 1) << j for j, b in enumerate(sel))
+                for v in range(2 ** len(union))]
+        return tab[rows]
+
+    def push_diag(ph_log, sel=(), tab_log=None):
+        tab = np.asarray(ph_log)[None, :] if tab_log is None else np.asarray(tab_log)
+        tab = tab[:, [phi(i) for i in range(2 ** REG_BITS)]]
+        sel = list(sel)
+        if pending[0] is not None:
+            psel, ptab = pending[0]
+            union = psel + [b for b in sel if b not in psel]
+            if len(union) <= MAX_SEL_BITS:
+                pending[0] = (union, expand(ptab, psel, union) * expand(tab, sel, union))
+                return
+            flush()
+        pending[0] = (sel, tab)
 
     def flush(bit=None):
         d = pending[0]
-        if d is None or (bit is not None and np.array_equal(d, d[regs16 ^ (1 << bit)])):
+        if d is None:
             return
-        out.append([SUB_DIAG, 0, d])
+        sel, tab = d
+        if bit is not None and np.array_equal(tab, tab[:, regs16 ^ (1 << bit)]):
+            return
+        out.append([SUB_DIAGV if sel else SUB_DIAG, sel, tab])
         pending[0] = None
 
     def push_cx(t, ctrls):
@@ -488,7 +544,7 @@ def _encode_phase(R, members, pos_index, T, coefs, subs):
             cv |= (1 ^ ((flip[0] >> b) & 1)) << q[b]
         if pending[0] is not None:
             e = np.where((regs16 & cm) == cv, regs16 ^ (1 << q[t]), regs16)
-            pending[0] = pending[0][e]
+            pending[0] = (pending[0][0], pending[0][1][:, e])
         out.append([SUB_CX, q[t] | (cm << 2) | (cv << 6), None])
 
     # Leading exchanges with at most one control (SWAP, X, CNOT: affine maps
@@ -499,6 +555,8 @@ def _encode_phase(R, members, pos_index, T, coefs, subs):
     for op in members:
         p = [reg_of[pos_index[b]] for b in op.bits]
         kd = op.kind
+        if isinstance(kd, SelDiagonal):
+            break
         if op.matrix is None and kd.kind == bk.GateKind.DIAGONAL:
             k0 += 1
             continue
@@ -529,6 +587,9 @@ def _encode_phase(R, members, pos_index, T, coefs, subs):
     for op in members[k0:]:
         p = [reg_of[pos_index[b]] for b in op.bits]        # logical bit per gate qubit
         kd = op.kind
+        if isinstance(kd, SelDiagonal):
+            push_diag(None, kd.sel, [_slot_phases(t, p) for t in kd.tables])
+            continue
         if op.matrix is None and kd.kind == bk.GateKind.DIAGONAL:
             push_diag(_slot_phases(kd.diag, p))
             continue
@@ -564,15 +625,23 @@ def _encode_phase(R, members, pos_index, T, coefs, subs):
     for typ, code, data in out:
         if typ == SUB_D1:
             typ = d1_variant(data)
-        if typ == SUB_DIAG:
-            mask = 0
-            for r in range(2 ** REG_BITS):
-                if data[r] != 1.0:
-                    mask |= 1 << r
-            if not mask:
+        if typ in (SUB_DIAG, SUB_DIAGV):
+            sel, tab = code, data
+            if np.all(tab == 1.0):
                 continue
-            subs.extend([SUB_DIAG, 0, len(coefs), mask])
-            coefs.extend(data)
+            if not sel:
+                mask = 0
+                for r in range(2 ** REG_BITS):
+                    if tab[0, r] != 1.0:
+                        mask |= 1 << r
+                subs.extend([SUB_DIAG, 0, len(coefs), mask])
+                coefs.extend(tab[0])
+            else:
+                cd = len(sel)
+                for j, b in enumerate(sel):
+                    cd |= b << (2 + 6 * j)
+                subs.extend([SUB_DIAGV, cd, len(coefs), 0])
+                coefs.extend(tab.reshape(-1))
         elif typ == SUB_CX:
             subs.extend([typ, code, 0, 0])
         else:
@@ -709,7 +778,7 @@ def _encode_mfma(bits, matrix, pos_index, coefs):
 
 
 MAX_STAGED = 512          # coefficients a pass without wide ops stages in LDS
-_COEF_SUBS = (SUB_D1, SUB_DIAG, SUB_D1R, SUB_D1X)
+_COEF_SUBS = (SUB_D1, SUB_DIAG, SUB_D1R, SUB_D1X, SUB_DIAGV)
 
 
 def _op_coef_start(o, srcs):
@@ -761,17 +830,31 @@ class HostPlan:
         self.gates = list(gates)
         bits_list = [tuple(n - 1 - qubits.index(q) for q in g.qubits) for g in gates]
         tileable = [len(b) <= 5 and len(b) <= T for b in bits_list]
+        # register phases apply diagonal gates whatever the tile bits: the
+        # phases of the qubits outside the tile are selected per tile (only
+        # where no pass can turn wide, i.e. every gate acts on <= 3 qubits)
+        free = None
+        if self.mode == 'reg' and all(len(b) <= 3 for b in bits_list):
+            free = [g.kind.kind == bk.GateKind.DIAGONAL for g in gates]
         self.passes = []           # (kind, payload)
         ops_host, coefs, srcs = [], [], []
         self.n_phases = 0
         self.n_blocks = 0
-        for pos, members in plan_passes(bits_list, n, T, low_bits, tileable):
+        for pos, members in plan_passes(bits_list, n, T, low_bits, tileable, free=free):
             if pos is None:
                 g = gates[members[0]]
                 self.passes.append(('gate', (list(bits_list[members[0]]), g.kind)))
                 continue
             pos_index = {b: j for j, b in enumerate(pos)}
-            ops = [Op(bits_list[i], gates[i].kind) for i in members]
+            tset = set(pos)
+            ops = []
+            for i in members:
+                bits = bits_list[i]
+                if tset.issuperset(bits):
+                    ops.append(Op(bits, gates[i].kind))
+                else:
+                    sd = SelDiagonal(gates[i].kind, bits, tset)
+                    ops.append(Op(sd.in_bits, sd))
             wide = any(len(op.bits) > 3 for op in ops)
             start = len(ops_host)
             c_lo = len(coefs)

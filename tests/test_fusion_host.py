@@ -128,59 +128,91 @@ def apply_store(ket, n, pos, spos):
     return np.transpose(ket, axes)
 
 
+def _phase_unitary(o, coefs, srcs, sel_val):
+    """16x16 operator of register phase o on its register bits (register bit
+    j <-> tile bit t[j]); sel_val maps an index bit outside the tile to its
+    value (selector diagonals)."""
+    u = np.eye(16, dtype=complex)
+    d2 = [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]
+    for q in range(o.coef):
+        typ, code, c0, mask = srcs[o.src + 4 * q: o.src + 4 * q + 4]
+        if typ == fusion.SUB_DIAG:
+            ph = np.where([(mask >> r) & 1 for r in range(16)], coefs[c0:c0 + 16], 1.0)
+            u = np.diag(ph) @ u
+            continue
+        if typ == fusion.SUB_DIAGV:
+            m = code & 3
+            v = sum(sel_val[(code >> (2 + 6 * j)) & 63] << j for j in range(m))
+            u = np.diag(coefs[c0 + 16 * v:c0 + 16 * v + 16]) @ u
+            continue
+        if typ == fusion.SUB_CX:
+            perm = np.zeros((16, 16))
+            t, cm, cv = code & 3, (code >> 2) & 15, (code >> 6) & 15
+            for r in range(16):
+                dst = r ^ (1 << t) if (r & cm) == cv else r
+                perm[dst, r] = 1
+            u = perm @ u
+            continue
+        if typ in (fusion.SUB_D1, fusion.SUB_D1R, fusion.SUB_D1X):
+            bits = [code]
+        else:
+            bits = list(d2[code])
+        k = len(bits)
+        m = coefs[c0:c0 + 4 ** k].reshape(2 ** k, 2 ** k)
+        # the specialised kernels read only part of each coefficient
+        if typ == fusion.SUB_D1R:
+            m = m.real.astype(complex)
+        elif typ == fusion.SUB_D1X:
+            m = np.array([[m[0, 0].real, 1j * m[0, 1].imag],
+                          [1j * m[1, 0].imag, m[1, 1].real]])
+        axes = [3 - b for b in reversed(bits)]      # gate MSB <-> highest bit
+        u = O.full_operator(m, 4, axes) @ u
+
+    # load map: register r starts with logical slot L(r); store map:
+    # register r goes to logical slot S(r)
+    def affine(m, r):
+        y = m >> 16
+        for j in range(4):
+            if (r >> j) & 1:
+                y ^= (m >> (4 * j)) & 15
+        return y
+    load = np.zeros((16, 16))
+    store = np.zeros((16, 16))
+    for r in range(16):
+        load[r, affine(o.phase, r)] = 1
+        store[affine(o.active, r), r] = 1
+    return store @ u @ load
+
+
 def _emulate_ops(ket, n, pos, ops, coefs, srcs):
     """Apply encoded qfx_tile_op records to a numpy state, as the tile kernel
-    defines them (register phases: register bit j <-> tile bit t[j])."""
+    defines them (register phases: register bit j <-> tile bit t[j];
+    selector diagonals read the index bits outside the tile)."""
     coefs = np.asarray(coefs)
     for o in ops:
         if o.kind == fusion.OP_RPHASE:
             gb = [pos[o.t[j]] for j in range(fusion.REG_BITS)]
-            u = np.eye(16, dtype=complex)
-            d2 = [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]
+            sel = set()
             for q in range(o.coef):
-                typ, code, c0, mask = srcs[o.src + 4 * q: o.src + 4 * q + 4]
-                if typ == fusion.SUB_DIAG:
-                    ph = np.where([(mask >> r) & 1 for r in range(16)], coefs[c0:c0 + 16], 1.0)
-                    u = np.diag(ph) @ u
-                    continue
-                if typ == fusion.SUB_CX:
-                    perm = np.zeros((16, 16))
-                    t, cm, cv = code & 3, (code >> 2) & 15, (code >> 6) & 15
-                    for r in range(16):
-                        dst = r ^ (1 << t) if (r & cm) == cv else r
-                        perm[dst, r] = 1
-                    u = perm @ u
-                    continue
-                if typ in (fusion.SUB_D1, fusion.SUB_D1R, fusion.SUB_D1X):
-                    bits = [code]
-                else:
-                    bits = list(d2[code])
-                k = len(bits)
-                m = coefs[c0:c0 + 4 ** k].reshape(2 ** k, 2 ** k)
-                # the specialised kernels read only part of each coefficient
-                if typ == fusion.SUB_D1R:
-                    m = m.real.astype(complex)
-                elif typ == fusion.SUB_D1X:
-                    m = np.array([[m[0, 0].real, 1j * m[0, 1].imag],
-                                  [1j * m[1, 0].imag, m[1, 1].real]])
-                axes = [3 - b for b in reversed(bits)]      # gate MSB <-> highest bit
-                u = O.full_operator(m, 4, axes) @ u
-            # load map: register r starts with logical slot L(r); store map:
-            # register r goes to logical slot S(r)
-            def affine(m, r):
-                y = m >> 16
-                for j in range(4):
-                    if (r >> j) & 1:
-                        y ^= (m >> (4 * j)) & 15
-                return y
-            load = np.zeros((16, 16))
-            store = np.zeros((16, 16))
-            for r in range(16):
-                load[r, affine(o.phase, r)] = 1
-                store[affine(o.active, r), r] = 1
-            u = store @ u @ load
+                typ, code = srcs[o.src + 4 * q], srcs[o.src + 4 * q + 1]
+                if typ == fusion.SUB_DIAGV:
+                    sel |= {(code >> (2 + 6 * j)) & 63 for j in range(code & 3)}
+            sel = sorted(sel)
+            assert not set(sel) & set(pos)
             axes = [n - 1 - gb[j] for j in (3, 2, 1, 0)]
-            ket = O.tensormul(u.reshape([2] * 8), ket, axes)
+            out = np.array(ket)
+            for a in range(2 ** len(sel)):
+                val = {b: (a >> i) & 1 for i, b in enumerate(sel)}
+                u = _phase_unitary(o, coefs, srcs, val)
+                # the slice of the state whose selector bits take these values
+                idx = [slice(None)] * n
+                for b, x in val.items():
+                    idx[n - 1 - b] = x
+                kept = [ax for ax in range(n) if isinstance(idx[ax], slice)]
+                sub = O.tensormul(u.reshape([2] * 8), ket[tuple(idx)],
+                                  [kept.index(ax) for ax in axes])
+                out[tuple(idx)] = sub
+            ket = out
         else:
             k = o.k
             axes = [n - 1 - pos[o.t[j]] for j in range(k)]
@@ -516,3 +548,65 @@ def test_prepared_lds_words_and_stale_words_rejected():
                                   len(plan.srcs)) == _native.QFX_EINVAL
     assert lib.qfx_tile_prepare_ops(arr, 1) == 0
     assert lib.qfx_tile_check_ops(8, 256, 0, arr, 1, ncoef, srcs, len(plan.srcs)) == 0
+
+
+@pytest.mark.parametrize("n,T,seed", [(12, 8, 0), (12, 9, 1), (13, 8, 2)])
+def test_selector_diagonals_replay(n, T, seed):
+    """Diagonal gates join the first pass that reaches them whatever its tile
+    bits; their phases on index bits outside the tile become selector
+    diagonals (SUB_DIAGV, up to 3 such bits per table).  Passes drop, and
+    replaying the plan gives the circuit's state."""
+    rng = np.random.default_rng(seed)
+    desc = []
+    for _ in range(8):
+        for q in range(n):
+            desc.append((('RZ', 'PHASE', 'RX')[rng.integers(3)], (float(rng.uniform(0, 6)),), (q,)))
+        order = rng.permutation(n)
+        for j in range(0, n - 1, 2):
+            a, b = int(order[j]), int(order[j + 1])
+            nm = ('CZ', 'CNOT', 'CPHASE')[rng.integers(3)]
+            desc.append((nm, (float(rng.uniform(0, 6)),) if nm == 'CPHASE' else (), (a, b)))
+    gates = build_gates(desc)
+    ref = O.run_circuit(desc, qubits=tuple(range(n)))
+    plan = fusion.HostPlan(gates, tuple(range(n)), tile_bits=T, low_bits=3, block=256,
+                           fuse_width=1, mode='reg')
+    nsel = 0
+    ket = O.zero_state(n)
+    for kind, payload in plan.passes:
+        pos, start, nops, block, flags, c_lo, ncoef, spos = payload
+        ops = plan.ops_host[start:start + nops]
+        for o in ops:
+            if o.kind == fusion.OP_RPHASE:
+                nsel += sum(plan.srcs[o.src + 4 * q] == fusion.SUB_DIAGV for q in range(o.coef))
+        ket = _emulate_ops(ket, n, pos, ops, plan.coefs[c_lo:c_lo + ncoef], plan.srcs)
+        ket = apply_store(ket, n, pos, spos)
+    assert nsel > 0
+    assert np.max(np.abs(ket - ref)) < 1e-12
+    # without selector diagonals the same circuit needs more passes
+    bits_list = [tuple(n - 1 - q for q in g.qubits) for g in gates]
+    plain = fusion.plan_passes(bits_list, n, T, 3)
+    assert plan.n_passes < len(plain)
+
+
+def test_selector_diagonal_validator_rules():
+    """SUB_DIAGV records: 1..3 selector bits, index bits < 64, no stray code
+    bits, and 16 * 2^m coefficients in range."""
+    lib = _native.load()
+    o = _native.QfxTileOp()
+    o.kind = fusion.OP_RPHASE
+    o.k = 4
+    for j in range(4):
+        o.t[j] = o.sorted[j] = j
+    o.coef, o.src = 1, 0
+    o.active = o.phase = sum(1 << (5 * j) for j in range(4))   # identity maps
+    fusion.prepare_ops([o])
+    arr = (_native.QfxTileOp * 1)(o)
+
+    def check(code, ncoef):
+        srcs = _native.int_array([fusion.SUB_DIAGV, code, 0, 0])
+        return lib.qfx_tile_check_ops(8, 256, 0, arr, 1, ncoef, srcs, 4)
+    assert check(2 | (20 << 2) | (33 << 8), 64) == 0
+    assert check(2 | (20 << 2) | (33 << 8), 63) == _native.QFX_EINVAL
+    assert check(0, 16) == _native.QFX_EINVAL
+    assert check(1 | (5 << 2) | (1 << 8), 32) == _native.QFX_EINVAL
+    assert check(3 | (10 << 2) | (11 << 8) | (12 << 14), 128) == 0
