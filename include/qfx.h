@@ -223,7 +223,7 @@ int qfx_tile_prepare_ops(qfx_tile_op* ops /*host*/, int nops);
 /* Host-side validation of an op list against tile width T, the block size
  * and flags it will run with (ops on 4-5 qubits need QFX_TILE_WIDE and
  * block <= 256; matrix-core blocks need QFX_TILE_MFMA; other passes at most
- * 512 coefficients), ncoef coefficients and nsrcs source rows (host arrays);
+ * 768 coefficients), ncoef coefficients and nsrcs source rows (host arrays);
  * register phases must carry the `lds` words qfx_tile_prepare_ops gives
  * them.  The device entry point trusts its op list; the Python planner
  * checks every plan with this before upload. */
@@ -235,7 +235,7 @@ int qfx_tile_check_ops(int T, int block, int flags, const qfx_tile_op* ops /*hos
  * 1024} with 2^tile_bits >= block; flags as accepted by qfx_tile_check_ops
  * for this op list.  ops/coef/srcs in device memory; coef holds the op
  * list's ncoef coefficients (indices in the ops are relative to coef).  A
- * pass without QFX_TILE_WIDE or QFX_TILE_MFMA takes at most 512
+ * pass without QFX_TILE_WIDE or QFX_TILE_MFMA takes at most 768
  * coefficients: it stages them in LDS.  Tile bit j is loaded from index bit
  * tile_pos[j] and stored to index bit store_pos[j] (a permutation of
  * tile_pos; NULL: the same), so a pass can end with a relabelling of its

@@ -28,8 +28,9 @@ constexpr int kMfmaMaxBlock = 512;    // largest block running matrix-core block
 // table (at most kMaxStaged entries) into LDS behind the tile once per
 // workgroup: uniform coefficient reads then come from LDS broadcasts instead
 // of scalar registers, which a pass's 2x2/4x4 matrices and 16-slot phase
-// vectors would otherwise exhaust.  8 KiB keeps two 2^12-amplitude tiles per CU.
-constexpr int kMaxStaged = 512;
+// vectors would otherwise exhaust.  12 KiB keeps two 2^12-amplitude tiles per
+// CU (2 x 76 KiB of the 160 KiB).
+constexpr int kMaxStaged = 768;
 
 struct TilePos {
   int s[QFX_MAX_QUBITS];  // tile bit positions, ascending (where tile bit j loads from)
@@ -1018,7 +1019,7 @@ int qfx_tile_check_ops(int T, int block, int flags, const qfx_tile_op* ops, int 
   if (!ops && nops > 0) return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: null ops");
   if (!(flags & (QFX_TILE_WIDE | QFX_TILE_MFMA)) && ncoef > kMaxStaged)
     return qfx::set_error(QFX_EINVAL,
-                          "qfx_tile_check_ops: more than 512 coefficients in a pass without wide ops");
+                          "qfx_tile_check_ops: more than 768 coefficients in a pass without wide ops");
   const int kmax = (flags & QFX_TILE_WIDE) ? 5 : 3;
   for (int i = 0; i < nops; ++i) {
     const qfx_tile_op& o = ops[i];
@@ -1170,7 +1171,7 @@ int qfx_tile_pass(void* stream, void* state, int n, int tile_bits, const int* ti
     return qfx::set_error(QFX_EINVAL, "qfx_tile_pass: null pointer");
   if (ncoef < 0 || (!wide && !mf && ncoef > kMaxStaged))
     return qfx::set_error(QFX_ERANGE,
-                          "qfx_tile_pass: more than 512 coefficients in a pass without wide ops");
+                          "qfx_tile_pass: more than 768 coefficients in a pass without wide ops");
   TilePos pos;
   std::memset(&pos, 0, sizeof(pos));
   for (int j = 0; j < tile_bits; ++j) {

@@ -111,30 +111,57 @@ def apply_gates_inplace(gates, qubits, work, prof=None):
 # planning
 # ---------------------------------------------------------------------------
 
-def _absorb(bits_list, pending, S, ok=None, free=None):
+def gate_types(kind, bits):
+    """How a gate acts on each of its bits, for commutation: 'Z' where it
+    commutes with Z on that bit (diagonal gates; controls of controlled-X
+    gates), 'X' where it acts as a function of X alone (X, RX; targets of
+    controlled-X gates), 'G' otherwise.  Two gates whose shared bits all
+    carry the same letter, Z or X, commute."""
+    if kind.kind == bk.GateKind.DIAGONAL:
+        return {b: 'Z' for b in bits}
+    pat = exchange_pattern(kind) if kind.kind == bk.GateKind.MONOMIAL else None
+    if pat is not None and pat[0] == 'cx' and kind.phase is None:
+        t = {b: 'Z' for b in bits}
+        t[bits[pat[1]]] = 'X'
+        return t
+    if kind.k == 1:
+        m = kind.matrix
+        if m[0, 0] == m[1, 1] and m[0, 1] == m[1, 0]:
+            return {bits[0]: 'X'}
+    return {b: 'G' for b in bits}
+
+
+def _absorb(bits_list, pending, S, ok=None, free=None, types=None):
     """Items (indices into bits_list, in order) a set of bits S takes: an item
     joins if it is allowed, its bits lie in S (or it is ``free``: a diagonal
-    gate that needs none of its bits in the tile) and it touches no bit of
-    an item left behind, so every item it overtakes acts on other bits and
-    commutes with it."""
-    blocked = set()
+    gate that needs none of its bits in the tile) and it commutes with every
+    item left behind before it: it shares no bit with them or, with
+    ``types`` (gate_types per item), every shared bit carries the same 'Z'
+    or 'X' in both."""
+    blocked = {}
     taken = []
     for i in pending:
         b = bits_list[i]
-        if (blocked.intersection(b) or (ok is not None and not ok[i])
-                or not (S.issuperset(b) or (free is not None and free[i]))):
-            blocked.update(b)
+        if types is None:
+            clear = not any(x in blocked for x in b)
+        else:
+            ti = types[i]
+            clear = all(x not in blocked or (ti[x] != 'G' and blocked[x] <= {ti[x]}) for x in b)
+        if (clear and (ok is None or ok[i])
+                and (S.issuperset(b) or (free is not None and free[i]))):
+            taken.append(i)
             continue
-        taken.append(i)
+        for x in b:
+            blocked.setdefault(x, set()).add('G' if types is None else types[i][x])
     return taken
 
 
-def _grow(bits_list, pending, S, width, ok=None, cap=40, free=None):
+def _grow(bits_list, pending, S, width, ok=None, cap=40, free=None, types=None):
     """Grow the bit set S to at most ``width`` bits, one candidate at a time:
     the candidates are the bit sets of the first ``cap`` distinct items not
     yet taken (in circuit order), and the one that takes the most further
     items per added bit wins.  Returns (S, items taken)."""
-    cur = _absorb(bits_list, pending, S, ok, free)
+    cur = _absorb(bits_list, pending, S, ok, free, types)
     while len(S) < width:
         taken = set(cur)
         best = None
@@ -146,7 +173,7 @@ def _grow(bits_list, pending, S, width, ok=None, cap=40, free=None):
             if not new or new in seen or len(S) + len(new) > width:
                 continue
             seen.add(new)
-            got = _absorb(bits_list, pending, S | new, ok, free)
+            got = _absorb(bits_list, pending, S | new, ok, free, types)
             gain = (len(got) - len(cur)) / len(new)
             if best is None or gain > best[0]:
                 best = (gain, new, got)
@@ -159,13 +186,15 @@ def _grow(bits_list, pending, S, width, ok=None, cap=40, free=None):
     return S, cur
 
 
-def plan_passes(bits_list, n, T, L, tileable=None, strategy='gain', free=None):
+def plan_passes(bits_list, n, T, L, tileable=None, strategy='gain', free=None, types=None):
     """Group gates (given by their index-bit tuples, in circuit order) into
     passes.  Returns a list of (sorted tile bits, [gate indices]); a gate
     that is not tileable forms a pass of its own with tile bits None.
     ``free[i]`` marks diagonal gates that a pass can apply whatever its tile
     bits (register phases select their phases by the tile's other index
     bits): they join the first pass that reaches them and add no tile bits.
+    ``types`` (gate_types per gate) lets a gate overtake the commuting gates
+    a pass leaves behind even where they share bits.
 
     strategy 'gain' grows each pass's tile set by ``_grow`` (the bits that
     take the most gates per added bit); 'greedy' takes gates first come,
@@ -181,7 +210,7 @@ def plan_passes(bits_list, n, T, L, tileable=None, strategy='gain', free=None):
             pending = pending[1:]
             continue
         if strategy == 'gain':
-            S, _ = _grow(bits_list, pending, set(low), T, tileable, free=free)
+            S, _ = _grow(bits_list, pending, set(low), T, tileable, free=free, types=types)
         else:
             S = set(low)
             blocked = set()
@@ -200,7 +229,7 @@ def plan_passes(bits_list, n, T, L, tileable=None, strategy='gain', free=None):
             if b not in S:
                 S.add(b)
             b += 1
-        taken = _absorb(bits_list, pending, S, tileable, free)
+        taken = _absorb(bits_list, pending, S, tileable, free, types)
         passes.append((sorted(S), taken))
         done = set(taken)
         pending = [g for g in pending if g not in done]
@@ -777,7 +806,7 @@ def _encode_mfma(bits, matrix, pos_index, coefs):
     return o
 
 
-MAX_STAGED = 512          # coefficients a pass without wide ops stages in LDS
+MAX_STAGED = 768          # coefficients a pass without wide ops stages in LDS
 _COEF_SUBS = (SUB_D1, SUB_DIAG, SUB_D1R, SUB_D1X, SUB_DIAGV)
 
 
@@ -836,11 +865,18 @@ class HostPlan:
         free = None
         if self.mode == 'reg' and all(len(b) <= 3 for b in bits_list):
             free = [g.kind.kind == bk.GateKind.DIAGONAL for g in gates]
+        # commutation-aware overtaking can pack the same gates into fewer
+        # passes, or into as many heavier ones: keep it only when it saves a pass
+        types = [gate_types(g.kind, b) for g, b in zip(gates, bits_list)]
+        planned = plan_passes(bits_list, n, T, low_bits, tileable, free=free)
+        typed = plan_passes(bits_list, n, T, low_bits, tileable, free=free, types=types)
+        if len(typed) < len(planned):
+            planned = typed
         self.passes = []           # (kind, payload)
         ops_host, coefs, srcs = [], [], []
         self.n_phases = 0
         self.n_blocks = 0
-        for pos, members in plan_passes(bits_list, n, T, low_bits, tileable, free=free):
+        for pos, members in planned:
             if pos is None:
                 g = gates[members[0]]
                 self.passes.append(('gate', (list(bits_list[members[0]]), g.kind)))
@@ -882,8 +918,8 @@ class HostPlan:
                 block, flags = self.block, 0
                 # a last register phase stores straight to HBM when its lanes
                 # run over the low (contiguous) tile bits
-                last = ops_host[-1]
-                if (persistent and self.reg_io and last.kind == OP_RPHASE
+                last = ops_host[-1] if len(ops_host) > start else None
+                if (last is not None and persistent and self.reg_io and last.kind == OP_RPHASE
                         and (1 << T) == 16 * block and block <= 512
                         and min(last.t[j] for j in range(REG_BITS)) >= min(low_bits, 3)):
                     flags = TILE_REG_IO
@@ -923,6 +959,10 @@ class HostPlan:
         tile bits (only the last one stores through store_pos)."""
         ops = ops_host[start:]
         if not ops:
+            # only absorbed exchanges: the pass is their store permutation
+            if store_pos is not None:
+                self.passes.append(('tile', (pos, start, 0, block, flags & ~TILE_REG_IO, c_lo,
+                                             0, store_pos)))
             return
         bounds = [_op_coef_start(o, srcs) for o in ops] + [len(coefs)]
         # an op without coefficients starts where the next op starts

@@ -610,3 +610,66 @@ def test_selector_diagonal_validator_rules():
     assert check(0, 16) == _native.QFX_EINVAL
     assert check(1 | (5 << 2) | (1 << 8), 32) == _native.QFX_EINVAL
     assert check(3 | (10 << 2) | (11 << 8) | (12 << 14), 128) == 0
+
+
+def test_gate_types_commutation_brute_force():
+    """Every pair of standard gates on 3 qubits whose shared qubits all carry
+    the same 'Z' or 'X' type (fusion.gate_types) commutes, checked on the
+    full 8x8 operators; pairs that gate_types does not clear include
+    non-commuting ones (the rule is not vacuous)."""
+    from quantumflow_amd import stdgates as S
+    rng = np.random.default_rng(5)
+    makers = [lambda q: S.RX(float(rng.uniform(0, 6)), q[0]),
+              lambda q: S.RY(float(rng.uniform(0, 6)), q[0]),
+              lambda q: S.RZ(float(rng.uniform(0, 6)), q[0]),
+              lambda q: S.PHASE(float(rng.uniform(0, 6)), q[0]),
+              lambda q: S.X(q[0]), lambda q: S.H(q[0]),
+              lambda q: S.CNOT(q[0], q[1]), lambda q: S.CZ(q[0], q[1]),
+              lambda q: S.ISWAP(q[0], q[1]), lambda q: S.SWAP(q[0], q[1]),
+              lambda q: S.CPHASE(float(rng.uniform(0, 6)), q[0], q[1]),
+              lambda q: S.CCNOT(q[0], q[1], q[2])]
+    arity = [1, 1, 1, 1, 1, 1, 2, 2, 2, 2, 2, 3]
+
+    def full(g):
+        qs = list(g.qubits)
+        return O.full_operator(g.matrix.reshape([2] * (2 * len(qs))), 3, qs)
+
+    cleared = blocked_noncomm = 0
+    for _ in range(600):
+        a, b = rng.integers(len(makers), size=2)
+        qa = list(rng.permutation(3)[:arity[a]])
+        qb = list(rng.permutation(3)[:arity[b]])
+        ga, gb = makers[a]([int(x) for x in qa]), makers[b]([int(x) for x in qb])
+        ta = fusion.gate_types(ga.kind, [2 - q for q in ga.qubits])
+        tb = fusion.gate_types(gb.kind, [2 - q for q in gb.qubits])
+        shared = set(ta) & set(tb)
+        ma, mb = full(ga), full(gb)
+        comm = np.max(np.abs(ma @ mb - mb @ ma))
+        if all(ta[x] == tb[x] != 'G' for x in shared):
+            cleared += 1
+            assert comm < 1e-12, (ga, gb, ta, tb)
+        elif comm > 1e-6:
+            blocked_noncomm += 1
+    assert cleared > 50 and blocked_noncomm > 50
+
+
+def test_pass_of_only_absorbed_swaps_keeps_its_store_permutation():
+    """A pass whose gates are all absorbed exchanges has no ops left: it must
+    still run, as a pass that only stores through the permutation."""
+    n, T = 10, 8
+    desc = [('SWAP', (), (0, 1)), ('SWAP', (), (1, 2))]
+    gates = build_gates(desc)
+    rng = np.random.default_rng(9)
+    psi = rng.normal(size=2 ** n) + 1j * rng.normal(size=2 ** n)
+    psi = (psi / np.linalg.norm(psi)).reshape([2] * n)
+    ref = O.run_circuit(desc, qubits=tuple(range(n)), ket=psi)
+    plan = fusion.HostPlan(gates, tuple(range(n)), tile_bits=T, low_bits=3, block=256,
+                           fuse_width=1, mode='reg')
+    assert any(p[1][2] == 0 and p[1][7] is not None for p in plan.passes)
+    ket = psi
+    for kind, payload in plan.passes:
+        pos, start, nops, block, flags, c_lo, ncoef, spos = payload
+        ket = _emulate_ops(ket, n, pos, plan.ops_host[start:start + nops],
+                           plan.coefs[c_lo:c_lo + ncoef], plan.srcs)
+        ket = apply_store(ket, n, pos, spos)
+    assert np.array_equal(ket, ref)

@@ -220,3 +220,22 @@ def test_tile_plan_permutations_exact(bk):
         got = plan.run(qf.State(psi)).vec.asarray()
         # permutation blocks on the matrix cores multiply by exact 0/1
         assert np.array_equal(got, ref), mode
+
+
+def test_tile_pass_of_only_absorbed_swaps(bk):
+    """A pass left with no ops (its SWAP gates absorbed as a store
+    permutation) still moves the amplitudes, bit for bit."""
+    import quantumflow_amd as qf
+    from quantumflow_amd.fusion import TilePlan
+    from quantumflow_amd.workloads import build_gates
+    n = 14
+    desc = [('SWAP', (), (0, 1)), ('SWAP', (), (1, 3))]
+    rng = np.random.default_rng(4)
+    psi = rand_state(n, rng)
+    ref = O.run_circuit(desc, qubits=tuple(range(n)), ket=psi)
+    for persistent in (False, True):
+        plan = TilePlan(build_gates(desc), tuple(range(n)), tile_bits=12, low_bits=3,
+                        block=256, fuse_width=1, mode='reg', persistent=persistent)
+        assert any(p[1][2] == 0 for p in plan.passes)
+        got = plan.run(qf.State(psi)).vec.asarray()
+        assert np.array_equal(got, ref)
