@@ -162,8 +162,10 @@ __device__ __forceinline__ void op_sparse(v2d* lds, int T, const qfx_tile_op& op
 //           (RX-like: real parts of m00, m11, imaginary parts of m01, m10)
 //   SUB_DIAG  16 phases (one per slot; exactly 1 where `mask` has no bit)
 //   SUB_DIAGV 2^m tables of 16 phases, m = code & 3 (1..3): table v where
-//           bit j of v is the tile's index bit (code >> (2 + 6 j)) & 63, a
-//           bit outside the tile (diagonal gates on qubits outside the tile)
+//           bit j of v is selector p = (code >> (2 + 7 j)) & 127: state-index
+//           bit p (< 64, outside the tile) or, for p >= 64, tile bit p - 64
+//           of the lane group (a non-register bit of the phase).  Diagonal
+//           gates thus need none of their qubits in registers.
 //   SUB_CX  X on register bit t = code & 3 for the slots whose bits under
 //           the control mask (code >> 2) & 15 equal (code >> 6) & 15: a pure
 //           register exchange (CNOT, CCNOT; controlled SWAP as three)
@@ -267,7 +269,7 @@ __device__ __forceinline__ void sub_cx(v2d (&v)[kRegs], unsigned cm, unsigned cv
 // let the dispatch drop its do-nothing path, which would otherwise copy all
 // 16 register amplitudes on every sub-op.
 __device__ __forceinline__ void apply_sub(v2d (&v)[kRegs], const int4 sub, const v2d* coef,
-                                          uint64_t gbase) {
+                                          uint64_t gbase, unsigned egrp) {
   const int type = sub.x, code = sub.y;
   const v2d* m = coef + sub.z;
   switch (type) {
@@ -300,12 +302,18 @@ __device__ __forceinline__ void apply_sub(v2d (&v)[kRegs], const int4 sub, const
       break;
     case SUB_DIAG: sub_diag(v, m); break;
     case SUB_DIAGV: {
-      // gbase: the tile's index with its tile bits zero (uniform)
+      // gbase: the tile's state index with its tile bits zero (uniform);
+      // egrp: the lane group's tile index with its register bits zero
       const unsigned c = (unsigned)code;
       unsigned sel = 0;
 #pragma unroll
-      for (int j = 0; j < 3; ++j)
-        if (j < (int)(c & 3u)) sel |= (unsigned)((gbase >> ((c >> (2 + 6 * j)) & 63u)) & 1u) << j;
+      for (int j = 0; j < 3; ++j) {
+        if (j < (int)(c & 3u)) {
+          const unsigned p = (c >> (2 + 7 * j)) & 127u;
+          const unsigned b = p < 64u ? (unsigned)((gbase >> p) & 1u) : (egrp >> (p - 64u)) & 1u;
+          sel |= b << j;
+        }
+      }
       sub_diag(v, m + 16 * sel);
       break;
     }
@@ -385,7 +393,7 @@ __device__ __forceinline__ unsigned phase_lb(const qfx_tile_op& op, unsigned g) 
 
 __device__ __forceinline__ void run_subs(v2d (&v)[kRegs], const qfx_tile_op& op,
                                          const v2d* coef, const int* __restrict__ subs,
-                                         uint64_t gbase) {
+                                         uint64_t gbase, unsigned egrp) {
   // the next record's scalar load is issued with the current one's (scalar
   // loads complete out of order, so one wait covers both)
   const int4* rec = reinterpret_cast<const int4*>(subs + op.src);
@@ -394,7 +402,7 @@ __device__ __forceinline__ void run_subs(v2d (&v)[kRegs], const qfx_tile_op& op,
   for (int i = 0; i < nsub; ++i) {
     const int4 cur = next;
     next = rec[i + 1 < nsub ? i + 1 : i];
-    apply_sub(v, cur, coef, gbase);
+    apply_sub(v, cur, coef, gbase, egrp);
   }
 }
 
@@ -431,10 +439,11 @@ __device__ __forceinline__ void op_rphase(v2d* lds, int T, const qfx_tile_op& op
   const int ngroups = 1 << (T - kRegBits);
   const PhaseLds p = phase_lds(op);
   for (int g = threadIdx.x; g < ngroups; g += BS) {
-    const unsigned slb = sw(phase_lb(op, (unsigned)g)) << 4;
+    const unsigned egrp = phase_lb(op, (unsigned)g);
+    const unsigned slb = sw(egrp) << 4;
     v2d v[kRegs];
     phase_lds_load(v, lds, slb, p);
-    run_subs(v, op, coef, subs, gbase);
+    run_subs(v, op, coef, subs, gbase, egrp);
     phase_lds_store(v, lds, slb, p);
   }
 }
@@ -865,8 +874,9 @@ __global__ __launch_bounds__(BS) void tile_reg_kernel(v2d* state, int T, TilePos
     {
       const PhaseLds lp = phase_lds(last);
       v2d v[kRegs];
-      phase_lds_load(v, lds, sw(phase_lb(last, (unsigned)tid)) << 4, lp);
-      run_subs(v, last, cf, srcs, base);
+      const unsigned egrp = phase_lb(last, (unsigned)tid);
+      phase_lds_load(v, lds, sw(egrp) << 4, lp);
+      run_subs(v, last, cf, srcs, base, egrp);
       phase_gstore(v, state + base + l_lane, lg);
     }
     base = nbase;
@@ -1089,8 +1099,17 @@ int qfx_tile_check_ops(int T, int block, int flags, const qfx_tile_op* ops, int 
         const int ty = sub[0], cd = sub[1];
         if (ty == SUB_DIAGV) {
           const unsigned c = (unsigned)cd, m = c & 3u;
-          if (m == 0 || (c >> (2 + 6 * m)) != 0)
-            return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: bad selector diagonal");
+          bool good = m != 0 && (c >> (2 + 7 * m)) == 0;
+          for (unsigned j = 0; good && j < m; ++j) {
+            const unsigned p = (c >> (2 + 7 * j)) & 127u;
+            // a lane-group selector must be a tile bit outside the registers
+            if (p >= 64u) {
+              const int t = (int)(p - 64u);
+              good = t < T;
+              for (int a = 0; good && a < kRegBits; ++a) good = o.t[a] != t;
+            }
+          }
+          if (!good) return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: bad selector diagonal");
           if (sub[2] < 0 || sub[2] + (kRegs << m) > ncoef)
             return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: sub-op coefficients out of range");
           continue;

@@ -437,13 +437,21 @@ def phaseable(op):
     return kd.k == 1
 
 
-def plan_phases(ops, width=REG_BITS):
+def is_diagonal_op(op):
+    return op.matrix is None and op.kind.kind == bk.GateKind.DIAGONAL
+
+
+def plan_phases(ops, width=REG_BITS, free_diagonals=False):
     """Group a pass's ops (in order) into register phases of <= width bits
     and standalone LDS ops, with the commuting-overtake rule of
-    plan_passes and its bit-set growth.  Returns [('phase', bits, ops) |
-    ('lds', op)]."""
+    plan_passes and its bit-set growth.  With ``free_diagonals`` a diagonal
+    op joins the first phase that reaches it whatever that phase's register
+    bits (its other bits select its phases per lane group).  Returns
+    [('phase', bits, ops) | ('lds', op)]."""
     bits_list = [tuple(op.bits) for op in ops]
     ok = [phaseable(op) for op in ops]
+    free = [is_diagonal_op(op) and len(op.bits) + len(getattr(op.kind, 'sel', ())) <= MAX_SEL_BITS
+            for op in ops] if free_diagonals else None
     pending = list(range(len(ops)))
     items = []
     while pending:
@@ -452,11 +460,51 @@ def plan_phases(ops, width=REG_BITS):
             items.append(('lds', ops[first]))
             pending = pending[1:]
             continue
-        R, taken = _grow(bits_list, pending, set(), width, ok)
+        R, taken = _grow(bits_list, pending, set(), width, ok, free=free)
         items.append(('phase', R, [ops[i] for i in taken]))
         done = set(taken)
         pending = [i for i in pending if i not in done]
     return items
+
+
+def _diag_tables(op, reg_of, pos_index):
+    """Selector codes and [2^m, 16] logical-slot phase tables of a diagonal
+    op in a register phase: gate qubits on the phase's register bits index
+    the 16 slots; the others are selectors, code 64 + t for tile bit t (the
+    lane group's bit) or the state-index bit b itself for bits outside the
+    tile (SelDiagonal.sel)."""
+    kd = op.kind
+    if isinstance(kd, SelDiagonal):
+        k_in = len(op.bits)
+        gsel = list(kd.sel)
+
+        def value(x_in, v_glob):
+            return kd.tables[v_glob][x_in]
+    else:
+        k_in = len(op.bits)
+        gsel = []
+
+        def value(x_in, v_glob):
+            return kd.diag[x_in]
+    regs, lane = [], []
+    for j, b in enumerate(op.bits):
+        t = pos_index[b]
+        if t in reg_of:
+            regs.append((j, reg_of[t]))
+        else:
+            lane.append((j, t))
+    sel = [64 + t for _, t in lane] + gsel
+    m = len(sel)
+    tab = np.empty((2 ** m, 2 ** REG_BITS), dtype=np.complex128)
+    for v in range(2 ** m):
+        for r in range(2 ** REG_BITS):
+            x_in = 0
+            for j, a in regs:
+                x_in |= ((r >> a) & 1) << (k_in - 1 - j)
+            for i, (j, _) in enumerate(lane):
+                x_in |= ((v >> i) & 1) << (k_in - 1 - j)
+            tab[v, r] = value(x_in, v >> len(lane))
+    return sel, tab
 
 
 def _slot_phases(diag, p):
@@ -498,6 +546,10 @@ def _encode_phase(R, members, pos_index, T, coefs, subs):
     4-bit logical index and are stored as matrix | offset << 16 (column j
     = image of bit j): ``active`` the store map, ``phase`` the load map."""
     tl = sorted(pos_index[b] for b in R)
+    # diagonals whose bits all lie in R may fold into the load map; the others
+    # (bits outside R: lane-group or outside-tile selectors) may not
+    def in_regs(op):
+        return all(b in R for b in op.bits)
     for j in range(T - 1, -1, -1):          # pad to 4 register bits with high tile
         if len(tl) >= REG_BITS:             # bits: lanes keep the low ones, whose
             break                           # amplitudes are contiguous in HBM
@@ -582,10 +634,10 @@ def _encode_phase(R, members, pos_index, T, coefs, subs):
     # holding E psi.  Diagonals met on the way are carried past E.
     k0 = 0
     for op in members:
-        p = [reg_of[pos_index[b]] for b in op.bits]
         kd = op.kind
-        if isinstance(kd, SelDiagonal):
+        if isinstance(kd, SelDiagonal) or not in_regs(op):
             break
+        p = [reg_of[pos_index[b]] for b in op.bits]
         if op.matrix is None and kd.kind == bk.GateKind.DIAGONAL:
             k0 += 1
             continue
@@ -614,11 +666,12 @@ def _encode_phase(R, members, pos_index, T, coefs, subs):
         push_diag(lead_diag)
 
     for op in members[k0:]:
-        p = [reg_of[pos_index[b]] for b in op.bits]        # logical bit per gate qubit
         kd = op.kind
-        if isinstance(kd, SelDiagonal):
-            push_diag(None, kd.sel, [_slot_phases(t, p) for t in kd.tables])
+        if isinstance(kd, SelDiagonal) or (is_diagonal_op(op) and not in_regs(op)):
+            sel, tab = _diag_tables(op, reg_of, pos_index)
+            push_diag(None, sel, tab)
             continue
+        p = [reg_of[pos_index[b]] for b in op.bits]        # logical bit per gate qubit
         if op.matrix is None and kd.kind == bk.GateKind.DIAGONAL:
             push_diag(_slot_phases(kd.diag, p))
             continue
@@ -668,7 +721,7 @@ def _encode_phase(R, members, pos_index, T, coefs, subs):
             else:
                 cd = len(sel)
                 for j, b in enumerate(sel):
-                    cd |= b << (2 + 6 * j)
+                    cd |= b << (2 + 7 * j)
                 subs.extend([SUB_DIAGV, cd, len(coefs), 0])
                 coefs.extend(tab.reshape(-1))
         elif typ == SUB_CX:
@@ -909,12 +962,25 @@ class HostPlan:
                 block, flags = self.block, TILE_MFMA
             elif self.mode == 'reg' and not wide:
                 ops, store_pos = relabel_swaps(ops, pos, low_bits)
-                for item in plan_phases(ops):
-                    if item[0] == 'phase':
-                        ops_host.append(_encode_phase(item[1], item[2], pos_index, T, coefs, srcs))
-                        self.n_phases += 1
-                    else:
-                        ops_host.append(_encode_op(item[1], pos_index, coefs, srcs))
+                # diagonals free of the register bits first; a phase whose
+                # selector tables exceed the staging limit replans the pass
+                # with diagonals on register bits
+                marks = (len(ops_host), len(coefs), len(srcs), self.n_phases)
+                for free_diag in ((True, False) if free is not None else (False,)):
+                    del ops_host[marks[0]:], coefs[marks[1]:], srcs[marks[2]:]
+                    self.n_phases = marks[3]
+                    fits = True
+                    for item in plan_phases(ops, free_diagonals=free_diag):
+                        c0 = len(coefs)
+                        if item[0] == 'phase':
+                            ops_host.append(_encode_phase(item[1], item[2], pos_index, T,
+                                                          coefs, srcs))
+                            self.n_phases += 1
+                        else:
+                            ops_host.append(_encode_op(item[1], pos_index, coefs, srcs))
+                        fits = fits and len(coefs) - c0 <= MAX_STAGED
+                    if fits:
+                        break
                 block, flags = self.block, 0
                 # a last register phase stores straight to HBM when its lanes
                 # run over the low (contiguous) tile bits

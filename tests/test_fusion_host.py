@@ -142,7 +142,7 @@ def _phase_unitary(o, coefs, srcs, sel_val):
             continue
         if typ == fusion.SUB_DIAGV:
             m = code & 3
-            v = sum(sel_val[(code >> (2 + 6 * j)) & 63] << j for j in range(m))
+            v = sum(sel_val[(code >> (2 + 7 * j)) & 127] << j for j in range(m))
             u = np.diag(coefs[c0 + 16 * v:c0 + 16 * v + 16]) @ u
             continue
         if typ == fusion.SUB_CX:
@@ -192,22 +192,26 @@ def _emulate_ops(ket, n, pos, ops, coefs, srcs):
     for o in ops:
         if o.kind == fusion.OP_RPHASE:
             gb = [pos[o.t[j]] for j in range(fusion.REG_BITS)]
-            sel = set()
+            sel = set()                     # selector codes of the phase
             for q in range(o.coef):
                 typ, code = srcs[o.src + 4 * q], srcs[o.src + 4 * q + 1]
                 if typ == fusion.SUB_DIAGV:
-                    sel |= {(code >> (2 + 6 * j)) & 63 for j in range(code & 3)}
+                    sel |= {(code >> (2 + 7 * j)) & 127 for j in range(code & 3)}
             sel = sorted(sel)
-            assert not set(sel) & set(pos)
+
+            def index_bit(c):               # selector code -> state-index bit
+                return c if c < 64 else pos[c - 64]
+            assert not {c for c in sel if c < 64} & set(pos)
+            assert not {index_bit(c) for c in sel if c >= 64} & set(gb)
             axes = [n - 1 - gb[j] for j in (3, 2, 1, 0)]
             out = np.array(ket)
             for a in range(2 ** len(sel)):
-                val = {b: (a >> i) & 1 for i, b in enumerate(sel)}
+                val = {c: (a >> i) & 1 for i, c in enumerate(sel)}
                 u = _phase_unitary(o, coefs, srcs, val)
                 # the slice of the state whose selector bits take these values
                 idx = [slice(None)] * n
-                for b, x in val.items():
-                    idx[n - 1 - b] = x
+                for c, x in val.items():
+                    idx[n - 1 - index_bit(c)] = x
                 kept = [ax for ax in range(n) if isinstance(idx[ax], slice)]
                 sub = O.tensormul(u.reshape([2] * 8), ket[tuple(idx)],
                                   [kept.index(ax) for ax in axes])
@@ -605,11 +609,15 @@ def test_selector_diagonal_validator_rules():
     def check(code, ncoef):
         srcs = _native.int_array([fusion.SUB_DIAGV, code, 0, 0])
         return lib.qfx_tile_check_ops(8, 256, 0, arr, 1, ncoef, srcs, 4)
-    assert check(2 | (20 << 2) | (33 << 8), 64) == 0
-    assert check(2 | (20 << 2) | (33 << 8), 63) == _native.QFX_EINVAL
+    assert check(2 | (20 << 2) | (33 << 9), 64) == 0
+    assert check(2 | (20 << 2) | (33 << 9), 63) == _native.QFX_EINVAL
     assert check(0, 16) == _native.QFX_EINVAL
-    assert check(1 | (5 << 2) | (1 << 8), 32) == _native.QFX_EINVAL
-    assert check(3 | (10 << 2) | (11 << 8) | (12 << 14), 128) == 0
+    assert check(1 | (5 << 2) | (1 << 9), 32) == _native.QFX_EINVAL      # stray bits
+    assert check(3 | (10 << 2) | (11 << 9) | (12 << 16), 128) == 0
+    # lane-group selectors: tile bits outside the registers (here t = 0..3)
+    assert check(1 | ((64 + 5) << 2), 32) == 0
+    assert check(1 | ((64 + 2) << 2), 32) == _native.QFX_EINVAL         # a register bit
+    assert check(1 | ((64 + 8) << 2), 32) == _native.QFX_EINVAL         # beyond T = 8
 
 
 def test_gate_types_commutation_brute_force():
