@@ -191,10 +191,12 @@ typedef struct {
  *   5  X on register bit code & 3 where the register bits of the mask
  *      (code >> 2) & 15 equal the values (code >> 6) & 15 (X, CNOT, CCNOT
  *      and their negated-control forms as register exchanges);
- *   9  2^m tables of 16 phases, m = code & 3 in 1..3, applied as type 4 with
- *      table v, where bit j of v is the tile's state-index bit
- *      (code >> (2 + 6 j)) & 63 (a bit outside the tile): diagonal gates on
- *      qubits outside the tile.
+ *   9  2^m tables of 16 phases (table v at coef index 16 v), m = code & 3
+ *      in 1..3, applied as type 4 with table v, where bit j of v is selector
+ *      p = (code >> (2 + 7 j)) & 127: the tile's state-index bit p (p < 64,
+ *      a bit outside the tile) or the lane group's tile bit p - 64 (a tile
+ *      bit outside the phase's registers).  Diagonal gates need none of
+ *      their qubits in registers or in the tile.
  * A register phase's `phase` and `active` words are its load and store
  * maps: register slot r holds logical phase index c ^ (XOR of column j over
  * the set bits j of r), column j in bits 4j..4j+3 and c in bits 16..19.
