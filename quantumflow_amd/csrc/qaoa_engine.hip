@@ -40,6 +40,7 @@ enum Stage : int {
   ST_MAKE_LAMBDA = 5,  // lambda <- C psi
   ST_RX_ADJ = 6,       // grad_beta += Im sum <lam|X_j|psi>; RX(-beta) on psi, lam
   ST_COST_ADJ = 7,     // grad_gamma += 2 Im <lam|C|psi> - W Im <lam|psi>; conj phases
+  ST_COST_RX = 8,      // ST_COST then ST_RX of the same layer (phases in one RX chunk)
 };
 
 enum PassFlags : int { PF_LOAD_PSI = 1, PF_LOAD_LAM = 2, PF_STORE_PSI = 4, PF_STORE_LAM = 8 };
@@ -76,6 +77,12 @@ struct Ctx {
 
 __device__ __forceinline__ double cut_of(const Ctx& c, uint64_t z) {
   return c.cnt ? (double)c.cnt[z] : c.cval[z];
+}
+
+// Cost phase of index z (unit weights) from the table tab[c] =
+// exp(i g (W/2 - c)) by its cut count c.
+__device__ __forceinline__ v2d cost_phase(const Ctx& c, const v2d* tab, uint32_t z) {
+  return tab[c.cnt[z]];
 }
 
 // Global indices of a thread's tile elements e = threadIdx.x + k BS: the
@@ -160,8 +167,13 @@ __device__ __forceinline__ double xgrad_regs(const v2d (&l)[16], const v2d (&pv)
 
 // Apply RX to the tile bits in `mask` (chunks of <= 4 bits in registers);
 // with ADJ also to lambda (angle -beta) and accumulate the beta gradient.
-template <int BS, int R, bool ADJ>
-__device__ __forceinline__ double rx_chunk(v2d* L, v2d* LL, int T, const int* bits, double c, double s) {
+// With COST the chunk first multiplies its elements by their cost phases
+// (unit-weight instances: tab[c] = exp(i g (W/2 - c))).  Each thread then
+// needs its elements' state indices, z = base | deposit(group bits) |
+// deposit(register bits), the last part fixed per chunk.
+template <int BS, int R, bool ADJ, bool COST>
+__device__ __forceinline__ double rx_chunk(v2d* L, v2d* LL, int T, const int* bits, double c,
+                                           double s, const Ctx& cx, const v2d* tab) {
   const int ngroups = 1 << (T - R);
   int sorted[4];
 #pragma unroll
@@ -177,12 +189,18 @@ __device__ __forceinline__ double rx_chunk(v2d* L, v2d* LL, int T, const int* bi
         sorted[b] = t;
       }
   unsigned soff[1 << R];
+  uint32_t zoff[1 << R];
 #pragma unroll
   for (int r = 0; r < (1 << R); ++r) {
     unsigned o = 0;
+    uint32_t zo = 0;
 #pragma unroll
-    for (int j = 0; j < R; ++j) o |= ((unsigned)(r >> j) & 1u) << bits[j];
+    for (int j = 0; j < R; ++j) {
+      o |= ((unsigned)(r >> j) & 1u) << bits[j];
+      if (COST && ((r >> j) & 1)) zo |= 1u << cx.ps->pos[bits[j]];
+    }
     soff[r] = swz(o);
+    zoff[r] = zo;
   }
   double g = 0.0;
   for (int grp = threadIdx.x; grp < ngroups; grp += BS) {
@@ -192,10 +210,23 @@ __device__ __forceinline__ double rx_chunk(v2d* L, v2d* LL, int T, const int* bi
       const int b = sorted[j];
       lb = ((lb >> b) << (b + 1)) | (lb & ((1u << b) - 1u));
     }
+    uint32_t zb = 0;
+    if (COST) {
+      zb = (uint32_t)cx.base;
+#pragma unroll
+      for (int t = 0; t < kMaxT; ++t)
+        if (t < T && ((lb >> t) & 1u)) zb |= 1u << cx.ps->pos[t];
+    }
     const unsigned slb = swz(lb);
     v2d v[16];
 #pragma unroll
     for (int r = 0; r < (1 << R); ++r) v[r] = L[slb ^ soff[r]];
+    if (COST) {
+#pragma unroll
+      for (int r = 0; r < (1 << R); ++r) {
+        v[r] = cmul(cost_phase(cx, tab, zb | zoff[r]), v[r]);
+      }
+    }
     if (ADJ) {
       v2d l[16];
 #pragma unroll
@@ -212,24 +243,49 @@ __device__ __forceinline__ double rx_chunk(v2d* L, v2d* LL, int T, const int* bi
   return g;
 }
 
-template <int BS, bool ADJ>
-__device__ __forceinline__ double rx_mask(v2d* L, v2d* LL, int T, unsigned mask, double theta) {
+// RX(theta) on the tile bits of mask; returns the beta-gradient terms (ADJ).
+// With COST, the first chunk applies the cost phases before its butterflies.
+// That chunk holds the highest (up to) four bits, so that neighbouring lanes
+// differ in low state-index bits and their cut-table reads share cache
+// lines; RX gates on different bits commute, so the chunk order is free.
+template <int BS, bool ADJ, bool COST>
+__device__ __forceinline__ double rx_mask(v2d* L, v2d* LL, int T, unsigned mask, double theta,
+                                          const Ctx& cx, const v2d* tab) {
   const double c = cos(0.5 * theta), s = sin(0.5 * theta);
   double g = 0.0;
+  bool first = true;
   while (mask) {
     int bits[4];
     int r = 0;
-    while (mask && r < 4) {
-      const int b = __builtin_ctz(mask);
-      mask &= mask - 1;
-      bits[r++] = b;
+    if (COST && first) {
+      while (mask && r < 4) {  // any order: register bit j <-> bits[j]
+        const int b = 31 - __builtin_clz(mask);
+        mask &= ~(1u << b);
+        bits[r++] = b;
+      }
+    } else {
+      while (mask && r < 4) {
+        const int b = __builtin_ctz(mask);
+        mask &= mask - 1;
+        bits[r++] = b;
+      }
     }
-    switch (r) {
-      case 1: g += rx_chunk<BS, 1, ADJ>(L, LL, T, bits, c, s); break;
-      case 2: g += rx_chunk<BS, 2, ADJ>(L, LL, T, bits, c, s); break;
-      case 3: g += rx_chunk<BS, 3, ADJ>(L, LL, T, bits, c, s); break;
-      default: g += rx_chunk<BS, 4, ADJ>(L, LL, T, bits, c, s); break;
+    if (COST && first) {
+      switch (r) {
+        case 1: g += rx_chunk<BS, 1, ADJ, true>(L, LL, T, bits, c, s, cx, tab); break;
+        case 2: g += rx_chunk<BS, 2, ADJ, true>(L, LL, T, bits, c, s, cx, tab); break;
+        case 3: g += rx_chunk<BS, 3, ADJ, true>(L, LL, T, bits, c, s, cx, tab); break;
+        default: g += rx_chunk<BS, 4, ADJ, true>(L, LL, T, bits, c, s, cx, tab); break;
+      }
+    } else {
+      switch (r) {
+        case 1: g += rx_chunk<BS, 1, ADJ, false>(L, LL, T, bits, c, s, cx, tab); break;
+        case 2: g += rx_chunk<BS, 2, ADJ, false>(L, LL, T, bits, c, s, cx, tab); break;
+        case 3: g += rx_chunk<BS, 3, ADJ, false>(L, LL, T, bits, c, s, cx, tab); break;
+        default: g += rx_chunk<BS, 4, ADJ, false>(L, LL, T, bits, c, s, cx, tab); break;
+      }
     }
+    first = false;
     __syncthreads();
   }
   return g;
@@ -249,7 +305,10 @@ __device__ void phase_table(v2d* tab, int ncut, double g, double wsum, bool conj
   __syncthreads();
 }
 
-template <int BS>
+// BWD selects the kernel of a mode: the forward kernel holds no adjoint
+// stages, the backward kernel no fused cost chunk (its programs keep the
+// cost stages separate).
+template <int BS, bool BWD>
 __device__ void run_item(const qfx_qaoa_inst& in, const qfx_qaoa_pass& ps, uint32_t tile,
                          const double* __restrict__ angles, v2d* __restrict__ psi_pool,
                          v2d* __restrict__ lam_pool, const uint16_t* __restrict__ cnt_pool,
@@ -282,93 +341,112 @@ __device__ void run_item(const qfx_qaoa_inst& in, const qfx_qaoa_pass& ps, uint3
   if (threadIdx.x < kAccum) accum[threadIdx.x] = 0.0;
   if (ps.flags & PF_LOAD_PSI)
     for_elems<BS>(c, [&](unsigned e, uint64_t z) { L[swz(e)] = psi[z]; });
-  if (ps.flags & PF_LOAD_LAM)
+  if (BWD && (ps.flags & PF_LOAD_LAM))
     for_elems<BS>(c, [&](unsigned e, uint64_t z) { LL[swz(e)] = lam[z]; });
   __syncthreads();
   const bool unit = !in.weighted;
+  // Weighted instances run a fused stage as its two parts (their phases
+  // come from sincos per amplitude, not from the cut-count table).
   for (int si = 0; si < ps.nstage; ++si) {
-    const int type = ps.stage[si][0], layer = ps.stage[si][1];
+    const int stype = ps.stage[si][0], layer = ps.stage[si][1];
     const unsigned mask = (unsigned)ps.stage[si][2];
-    switch (type) {
-      case ST_INIT: {
-        const double a = ldexp(1.0, -(c.n / 2)) * ((c.n & 1) ? 0.70710678118654752440 : 1.0);
-        for (unsigned e = threadIdx.x; e < tsize; e += BS) L[swz(e)] = v2d{a, 0.0};
-        __syncthreads();
-        break;
-      }
-      case ST_COST:
-      case ST_COST_ADJ: {
-        const double g = c.gamma[layer];
-        const bool adj = type == ST_COST_ADJ;
-        if (unit) phase_table<BS>(tab, in.nedges, g, in.wsum, adj);
-        double acc = 0.0;
-        for_elems<BS>(c, [&](unsigned e, uint64_t z) {
-          v2d ph;
-          double cz;
-          if (unit) {
-            const int k = c.cnt[z];
-            cz = (double)k;
-            ph = tab[k];
-          } else {
-            cz = c.cval[z];
-            double sn, cs;
-            sincos(g * (0.5 * in.wsum - cz), &sn, &cs);
-            ph = v2d{cs, adj ? -sn : sn};
-          }
-          const unsigned sl = swz(e);
-          const v2d x = L[sl];
-          if (adj) {
-            const v2d l = LL[sl];
-            const v2d lp = cmulc(l, x);  // conj(l) psi
-            acc += 2.0 * cz * lp.y - in.wsum * lp.y;
-            LL[sl] = cmul(ph, l);
-          }
-          L[sl] = cmul(ph, x);
-        });
-        if (adj) {
-          const double t = block_total<BS>(acc, red);
-          if (threadIdx.x == 0) accum[1 + c.p + layer] += t;
+    const bool split = !BWD && !unit && stype == ST_COST_RX;
+    for (int part = 0; part < (split ? 2 : 1); ++part) {
+      int type = stype;
+      if (split) type = part == 0 ? ST_COST : ST_RX;
+      switch (type) {
+        case ST_INIT: {
+          const double a = ldexp(1.0, -(c.n / 2)) * ((c.n & 1) ? 0.70710678118654752440 : 1.0);
+          for (unsigned e = threadIdx.x; e < tsize; e += BS) L[swz(e)] = v2d{a, 0.0};
+          __syncthreads();
+          break;
         }
-        __syncthreads();
-        break;
+        case ST_COST:
+        case ST_COST_ADJ: {
+          const double g = c.gamma[layer];
+          const bool adj = BWD && type == ST_COST_ADJ;
+          if (unit) phase_table<BS>(tab, in.nedges, g, in.wsum, adj);
+          double acc = 0.0;
+          for_elems<BS>(c, [&](unsigned e, uint64_t z) {
+            v2d ph;
+            double cz;
+            if (unit) {
+              const int k = c.cnt[z];
+              cz = (double)k;
+              ph = tab[k];
+            } else {
+              cz = c.cval[z];
+              double sn, cs;
+              sincos(g * (0.5 * in.wsum - cz), &sn, &cs);
+              ph = v2d{cs, adj ? -sn : sn};
+            }
+            const unsigned sl = swz(e);
+            const v2d x = L[sl];
+            if (adj) {
+              const v2d l = LL[sl];
+              const v2d lp = cmulc(l, x);  // conj(l) psi
+              acc += 2.0 * cz * lp.y - in.wsum * lp.y;
+              LL[sl] = cmul(ph, l);
+            }
+            L[sl] = cmul(ph, x);
+          });
+          if (adj) {
+            const double t = block_total<BS>(acc, red);
+            if (threadIdx.x == 0) accum[1 + c.p + layer] += t;
+          }
+          __syncthreads();
+          break;
+        }
+        case ST_RX: {
+          rx_mask<BS, false, false>(L, LL, c.T, mask, c.beta[layer], c, tab);
+          break;
+        }
+        case ST_COST_RX: {  // unit weights, forward kernel only
+          if constexpr (!BWD) {
+            phase_table<BS>(tab, in.nedges, c.gamma[layer], in.wsum, false);
+            rx_mask<BS, false, true>(L, LL, c.T, mask, c.beta[layer], c, tab);
+          }
+          break;
+        }
+        case ST_RX_ADJ: {
+          if constexpr (BWD) {
+            const double gsum =
+                rx_mask<BS, true, false>(L, LL, c.T, mask, -c.beta[layer], c, tab);
+            const double t = block_total<BS>(gsum, red);
+            if (threadIdx.x == 0) accum[1 + layer] += t;
+            __syncthreads();
+          }
+          break;
+        }
+        case ST_EXPECT: {
+          double acc = 0.0;
+          for_elems<BS>(c, [&](unsigned e, uint64_t z) {
+            const v2d x = L[swz(e)];
+            acc += fma(x.x, x.x, x.y * x.y) * cut_of(c, z);
+          });
+          const double t = block_total<BS>(acc, red);
+          if (threadIdx.x == 0) accum[0] += t;
+          __syncthreads();
+          break;
+        }
+        case ST_MAKE_LAMBDA: {
+          if constexpr (BWD) {
+            for_elems<BS>(c, [&](unsigned e, uint64_t z) {
+              const unsigned sl = swz(e);
+              const double cz = cut_of(c, z);
+              const v2d x = L[sl];
+              LL[sl] = v2d{cz * x.x, cz * x.y};
+            });
+            __syncthreads();
+          }
+          break;
+        }
       }
-      case ST_RX: {
-        rx_mask<BS, false>(L, LL, c.T, mask, c.beta[layer]);
-        break;
-      }
-      case ST_RX_ADJ: {
-        const double gsum = rx_mask<BS, true>(L, LL, c.T, mask, -c.beta[layer]);
-        const double t = block_total<BS>(gsum, red);
-        if (threadIdx.x == 0) accum[1 + layer] += t;
-        __syncthreads();
-        break;
-      }
-      case ST_EXPECT: {
-        double acc = 0.0;
-        for_elems<BS>(c, [&](unsigned e, uint64_t z) {
-          const v2d x = L[swz(e)];
-          acc += fma(x.x, x.x, x.y * x.y) * cut_of(c, z);
-        });
-        const double t = block_total<BS>(acc, red);
-        if (threadIdx.x == 0) accum[0] += t;
-        __syncthreads();
-        break;
-      }
-      case ST_MAKE_LAMBDA: {
-        for_elems<BS>(c, [&](unsigned e, uint64_t z) {
-          const unsigned sl = swz(e);
-          const double cz = cut_of(c, z);
-          const v2d x = L[sl];
-          LL[sl] = v2d{cz * x.x, cz * x.y};
-        });
-        __syncthreads();
-        break;
-      }
-    }
+    }  // part
   }
   if (ps.flags & PF_STORE_PSI)
     for_elems<BS>(c, [&](unsigned e, uint64_t z) { psi[z] = L[swz(e)]; });
-  if (ps.flags & PF_STORE_LAM)
+  if (BWD && (ps.flags & PF_STORE_LAM))
     for_elems<BS>(c, [&](unsigned e, uint64_t z) { lam[z] = LL[swz(e)]; });
   __syncthreads();
   if ((int)threadIdx.x < width) partial[threadIdx.x] = accum[threadIdx.x];
@@ -400,7 +478,7 @@ __device__ bool grid_barrier(unsigned* counter, unsigned target, int* err) {
   return s_ok != 0;
 }
 
-template <int BS>
+template <int BS, bool BWD>
 __global__ __launch_bounds__(BS) void qaoa_kernel(
     const qfx_qaoa_inst* __restrict__ insts, const qfx_qaoa_pass* __restrict__ passes,
     const qfx_qaoa_item* __restrict__ items, const int* __restrict__ kstart, int nk,
@@ -415,9 +493,9 @@ __global__ __launch_bounds__(BS) void qaoa_kernel(
     for (int it = kstart[k] + (int)blockIdx.x; it < kstart[k + 1]; it += gridDim.x) {
       const qfx_qaoa_item item = items[it];
       const qfx_qaoa_inst& in = insts[item.inst];
-      run_item<BS>(in, passes[item.pass], (uint32_t)item.tile, angles, psi_pool, lam_pool,
-                   cnt_pool, val_pool, partials + (uint64_t)item.partial * width, width, lds,
-                   red, accum, tab);
+      run_item<BS, BWD>(in, passes[item.pass], (uint32_t)item.tile, angles, psi_pool,
+                        lam_pool, cnt_pool, val_pool,
+                        partials + (uint64_t)item.partial * width, width, lds, red, accum, tab);
     }
     if (k + 1 < nk && !grid_barrier(sync, (unsigned)(k + 1) * gridDim.x, err)) return;
   }
@@ -451,20 +529,20 @@ __global__ void zero_sync_kernel(unsigned* sync, int* err) {
   }
 }
 
-template <int BS>
+template <int BS, bool BWD>
 int launch_qaoa(hipStream_t s, const qfx_qaoa_args& a, int T) {
-  const size_t lds = (sizeof(v2d) << T) * (a.mode == QFX_QAOA_BACKWARD ? 2 : 1);
+  const size_t lds = (sizeof(v2d) << T) * (BWD ? 2 : 1);
   static int grid = 0;
   static size_t grid_lds = 0;
   if (grid == 0 || grid_lds != lds) {
-    int rc = qfx::check_hip(hipFuncSetAttribute((const void*)qaoa_kernel<BS>,
+    int rc = qfx::check_hip(hipFuncSetAttribute((const void*)qaoa_kernel<BS, BWD>,
                                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                                 (int)lds),
                             "hipFuncSetAttribute");
     if (rc) return rc;
     int per_cu = 0, dev = 0, cus = 0;
     rc = qfx::check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                            &per_cu, (const void*)qaoa_kernel<BS>, BS, lds),
+                            &per_cu, (const void*)qaoa_kernel<BS, BWD>, BS, lds),
                         "hipOccupancyMaxActiveBlocksPerMultiprocessor");
     if (rc) return rc;
     if (per_cu < 1) return qfx::set_error(QFX_ERANGE, "qfx_qaoa_run: kernel does not fit a CU");
@@ -483,7 +561,7 @@ int launch_qaoa(hipStream_t s, const qfx_qaoa_args& a, int T) {
   hipLaunchKernelGGL(zero_sync_kernel, dim3(1), dim3(64), 0, s, sync, err);
   int rc = qfx::check_launch("zero_sync_kernel");
   if (rc) return rc;
-  hipLaunchKernelGGL((qaoa_kernel<BS>), dim3(g), dim3(BS), lds, s,
+  hipLaunchKernelGGL((qaoa_kernel<BS, BWD>), dim3(g), dim3(BS), lds, s,
                      static_cast<const qfx_qaoa_inst*>(a.insts),
                      static_cast<const qfx_qaoa_pass*>(a.passes),
                      static_cast<const qfx_qaoa_item*>(a.items), static_cast<const int*>(a.kstart),
@@ -548,11 +626,17 @@ int qfx_qaoa_check(const qfx_qaoa_inst* insts, int ninst, const qfx_qaoa_pass* p
     for (int si = 0; si < ps.nstage; ++si) {
       const int ty = ps.stage[si][0], l = ps.stage[si][1];
       const unsigned mask = (unsigned)ps.stage[si][2];
-      if (ty < ST_INIT || ty > ST_COST_ADJ || l < 0 || l >= kMaxPLayers)
+      if (ty < ST_INIT || ty > ST_COST_RX || l < 0 || l >= kMaxPLayers)
         return qfx::set_error(QFX_EINVAL, "qfx_qaoa_check: bad stage");
-      if ((ty == ST_RX || ty == ST_RX_ADJ) && (mask >> ps.T))
+      const bool rx = ty == ST_RX || ty == ST_RX_ADJ || ty == ST_COST_RX;
+      if (rx && (mask >> ps.T))
         return qfx::set_error(QFX_EINVAL, "qfx_qaoa_check: RX mask outside the tile");
-      if (mode != QFX_QAOA_BACKWARD && ty >= ST_MAKE_LAMBDA)
+      if (ty == ST_COST_RX && mask == 0)
+        return qfx::set_error(QFX_EINVAL, "qfx_qaoa_check: fused cost stage without RX bits");
+      if (ty == ST_COST_RX && mode == QFX_QAOA_BACKWARD)
+        return qfx::set_error(QFX_EINVAL, "qfx_qaoa_check: fused cost stage in a backward program");
+      if (mode != QFX_QAOA_BACKWARD &&
+          (ty == ST_MAKE_LAMBDA || ty == ST_RX_ADJ || ty == ST_COST_ADJ))
         return qfx::set_error(QFX_EINVAL, "qfx_qaoa_check: backward stage in a forward program");
     }
   }
@@ -592,10 +676,11 @@ int qfx_qaoa_run(void* stream, const qfx_qaoa_args* args) {
     return qfx::set_error(QFX_ERANGE, "qfx_qaoa_run: tile_bits out of range");
   const int T = a.tile_bits ? a.tile_bits : tdef;
   hipStream_t s = qfx::as_stream(stream);
+  const bool bwd = a.mode == QFX_QAOA_BACKWARD;
   switch (a.block) {
-    case 256: return launch_qaoa<256>(s, a, T);
-    case 512: return launch_qaoa<512>(s, a, T);
-    case 1024: return launch_qaoa<1024>(s, a, T);
+    case 256: return bwd ? launch_qaoa<256, true>(s, a, T) : launch_qaoa<256, false>(s, a, T);
+    case 512: return bwd ? launch_qaoa<512, true>(s, a, T) : launch_qaoa<512, false>(s, a, T);
+    case 1024: return bwd ? launch_qaoa<1024, true>(s, a, T) : launch_qaoa<1024, false>(s, a, T);
   }
   return qfx::set_error(QFX_EINVAL, "qfx_qaoa_run: block must be 256, 512 or 1024");
 }

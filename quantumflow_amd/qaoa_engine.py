@@ -32,6 +32,9 @@ from . import backend as bk
 
 MODE_FORWARD, MODE_BACKWARD = 0, 1
 ST_INIT, ST_COST, ST_RX, ST_EXPECT, ST_MAKE_LAMBDA, ST_RX_ADJ, ST_COST_ADJ = 1, 2, 3, 4, 5, 6, 7
+# fused stage: cost layer l then RX layer l (the phases ride in the first
+# mixer chunk)
+ST_COST_RX = 8
 PF_LOAD_PSI, PF_LOAD_LAM, PF_STORE_PSI, PF_STORE_LAM = 1, 2, 4, 8
 MAX_LAYERS = 16
 MAX_STAGES = 40
@@ -91,9 +94,31 @@ def tile_sets(n, T, low=None):
     return sets
 
 
-def build_program(n, p, mode, T):
+def fuse_stages(stages):
+    """Merge a cost stage into the mixer stage of the same layer that follows
+    it in a pass: (COST l, RX l) -> COST_RX l.  The kernel then applies the
+    phases inside the first RX chunk instead of in a sweep of their own.
+    The adjoint stages stay separate: with lambda and psi both in registers,
+    a fused adjoint chunk measured slower than the two sweeps."""
+    out = []
+    for st in stages:
+        typ, layer, bits = st
+        if out:
+            pt, pl, _ = out[-1]
+            if pt == ST_COST and typ == ST_RX and pl == layer and bits:
+                out[-1] = (ST_COST_RX, layer, bits)
+                continue
+        out.append(st)
+    return out
+
+
+def build_program(n, p, mode, T, fuse=True):
     """Passes of one instance: list of (tile bits, [(stage, layer, global bit
-    set)], flags)."""
+    set)], flags).  With `fuse`, forward-mode programs merge each cost stage
+    into the mixer stage of its layer (fuse_stages).  Backward programs keep
+    them separate: their 12-bit tiles leave a 4-bit chunk only 256 groups,
+    half of a 512-thread workgroup, and the fused stage measured slower
+    there than the separate sweep."""
     allbits = frozenset(range(n))
     seq = [(ST_INIT, 0, None)]
     for l in range(p):
@@ -138,7 +163,8 @@ def build_program(n, p, mode, T):
             flags |= PF_STORE_PSI
             if lam_live:
                 flags |= PF_STORE_LAM
-        out.append((tuple(tset), stages, flags))
+        fused = fuse and mode == MODE_FORWARD
+        out.append((tuple(tset), fuse_stages(stages) if fused else stages, flags))
     return out
 
 
@@ -160,7 +186,7 @@ class QAOABatch:
     optional); p: layers; mode: MODE_FORWARD (values) or MODE_BACKWARD
     (values and gradients)."""
 
-    def __init__(self, graphs, p, mode=MODE_BACKWARD, block=None, tile_bits=None):
+    def __init__(self, graphs, p, mode=MODE_BACKWARD, block=None, tile_bits=None, fuse=True):
         if not 1 <= p <= MAX_LAYERS:
             raise ValueError('p must be 1..%d' % MAX_LAYERS)
         if block is None or tile_bits is None:
@@ -193,7 +219,7 @@ class QAOABatch:
         for i, g in enumerate(self.graphs):
             n, edges, w, weighted = graph_arrays(g)
             self.n.append(n)
-            prog = build_program(n, p, mode, T)
+            prog = build_program(n, p, mode, T, fuse)
             inst = insts[i]
             inst.n, inst.p, inst.weighted, inst.nedges = n, p, int(weighted), len(edges)
             inst.wsum = float(np.sum(w))

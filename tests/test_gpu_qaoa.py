@@ -129,6 +129,38 @@ def test_weighted_graph(E):
     assert np.max(np.abs(grad[0] - np.concatenate([sb, sg]))) <= 1e-10
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+def test_fused_cost_stages_match_separate_stages(E, mode):
+    """Cost layers fused into mixer chunks (the default program) against
+    the same program with separate cost sweeps, on multi-pass instances at
+    two tile widths, plus a weighted multi-pass instance (whose fused stages
+    run as their two parts) against the oracle."""
+    import torch
+    rng = np.random.default_rng(8 + mode)
+    graphs = [er(n, 0.5, 80 + n) for n in (7, 11, 14, 17)]
+    wg = er(12, 0.5, 99)
+    for u, v in wg.edges():
+        wg[u][v]['weight'] = float(rng.uniform(0.1, 2.0))
+    graphs.append(wg)
+    p = 3
+    beta = rng.uniform(0, np.pi / 2, size=(len(graphs), p))
+    gamma = rng.uniform(0, np.pi, size=(len(graphs), p))
+    for T in (None, 8):
+        print("fused vs separate, mode", mode, "tile bits", T, flush=True)
+        va, ga = E.QAOABatch(graphs, p, mode=mode, tile_bits=T).evaluate(beta, gamma)
+        vb, gb = E.QAOABatch(graphs, p, mode=mode, tile_bits=T, fuse=False).evaluate(beta, gamma)
+        assert torch.max(torch.abs(va - vb)).item() <= 1e-12
+        if mode == 1:
+            assert torch.max(torch.abs(ga - gb)).item() <= 1e-12
+    edges = list(wg.edges())
+    w = [wg[u][v]['weight'] for u, v in edges]
+    i = len(graphs) - 1
+    assert abs(va[i].item() - O.qaoa_expectation(12, edges, beta[i], gamma[i], w)) <= 1e-10
+    if mode == 1:
+        sb, sg = O.qaoa_grad_shift(12, edges, beta[i], gamma[i], w)
+        assert np.max(np.abs(ga[i].cpu().numpy() - np.concatenate([sb, sg]))) <= 1e-10
+
+
 def test_graph_replay_and_determinism(E):
     import torch
     rng = np.random.default_rng(5)

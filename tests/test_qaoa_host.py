@@ -16,11 +16,11 @@ from oracle import qf_oracle as O
 from quantumflow_amd import qaoa_engine as E
 
 
-def emulate(n, edges, weights, beta, gamma, mode, T):
+def emulate(n, edges, weights, beta, gamma, mode, T, fuse=True):
     cuts = O.graph_cuts(n, edges, weights).reshape(-1)
     W = float(len(edges) if weights is None else np.sum(weights))
     p = len(beta)
-    prog = E.build_program(n, p, mode, T)
+    prog = E.build_program(n, p, mode, T, fuse)
     idx = np.arange(2 ** n)
     psi = lam = None
     value = 0.0
@@ -46,6 +46,12 @@ def emulate(n, edges, weights, beta, gamma, mode, T):
                 psi = np.full(2 ** n, 2.0 ** (-n / 2), dtype=complex)
             elif typ == E.ST_COST:
                 psi = psi * np.exp(1j * gamma[layer] * (W / 2 - cuts))
+            elif typ == E.ST_COST_RX:
+                # phases first, then the mixer chunk(s) of the same layer
+                psi = psi * np.exp(1j * gamma[layer] * (W / 2 - cuts))
+                for b in bits:
+                    seen[(layer, b)] = seen.get((layer, b), 0) + 1
+                psi = rx(psi, sorted(bits), beta[layer])
             elif typ == E.ST_RX:
                 for b in bits:
                     seen[(layer, b)] = seen.get((layer, b), 0) + 1
@@ -69,20 +75,21 @@ def emulate(n, edges, weights, beta, gamma, mode, T):
     return value, gb, gg
 
 
+@pytest.mark.parametrize("fuse", [True, False])
 @pytest.mark.parametrize("n,T,p", [(5, 12, 2), (7, 4, 2), (8, 5, 3), (9, 6, 1)])
-def test_program_emulation_matches_oracle(n, T, p):
+def test_program_emulation_matches_oracle(n, T, p, fuse):
     rng = np.random.default_rng(n * 10 + T)
     g = nx.gnp_random_graph(n, 0.5, seed=int(rng.integers(1000)))
     edges = list(g.edges())
     beta = rng.uniform(0, np.pi / 2, size=p)
     gamma = rng.uniform(0, np.pi, size=p)
     ref = O.qaoa_expectation(n, edges, beta, gamma)
-    v, gb, gg = emulate(n, edges, None, beta, gamma, E.MODE_BACKWARD, T)
+    v, gb, gg = emulate(n, edges, None, beta, gamma, E.MODE_BACKWARD, T, fuse)
     assert abs(v - ref) < 1e-12
     sb, sg = O.qaoa_grad_shift(n, edges, beta, gamma)
     assert np.max(np.abs(gb - sb)) < 1e-12
     assert np.max(np.abs(gg - sg)) < 1e-12
-    v2, _, _ = emulate(n, edges, None, beta, gamma, E.MODE_FORWARD, T + 1)
+    v2, _, _ = emulate(n, edges, None, beta, gamma, E.MODE_FORWARD, T + 1, fuse)
     assert abs(v2 - ref) < 1e-12
 
 
@@ -110,6 +117,30 @@ def test_pass_counts(n):
         assert prog[-1][2] & E.PF_STORE_PSI == 0        # last pass stores nothing
         for tset, stages, flags in prog:
             assert len(tset) <= 13 and list(tset) == sorted(tset)
+
+
+@pytest.mark.parametrize("n,T", [(9, 12), (20, 13), (24, 12)])
+def test_fused_stages(n, T):
+    """In forward programs every cost stage merges into the mixer stage of
+    its layer that follows it (chained passes always hold one), with a
+    non-empty bit set; backward programs stay unfused."""
+    p = 3
+    for mode in (E.MODE_FORWARD, E.MODE_BACKWARD):
+        plain = E.build_program(n, p, mode, T, fuse=False)
+        fused = E.build_program(n, p, mode, T)
+        assert [x[0] for x in plain] == [x[0] for x in fused]
+        assert [x[2] for x in plain] == [x[2] for x in fused]
+        ncost = sum(st[0] == E.ST_COST for _, sts, _ in plain for st in sts)
+        nf = sum(st[0] == E.ST_COST_RX for _, sts, _ in fused for st in sts)
+        assert ncost == p
+        if mode == E.MODE_FORWARD:
+            assert nf == p
+        else:
+            assert nf == 0 and plain == fused
+        for _, sts, _ in fused:
+            for typ, layer, bits in sts:
+                if typ == E.ST_COST_RX:
+                    assert bits
 
 
 def test_tile_sets_cover():
