@@ -294,7 +294,11 @@ typedef struct {
   int32_t pos[13];    /* tile bit positions, ascending */
   int32_t nstage;
   int32_t flags;      /* 1 load psi, 2 load lambda, 4 store psi, 8 store lambda */
-  int32_t stage[QFX_QAOA_MAX_STAGES][3];  /* {type, layer, tile-bit mask} */
+  /* {type, layer, tile-bit mask}; types: 1 init |+>, 2 cost layer, 3 RX on
+   * the mask, 4 <C>, 5 lambda = C psi, 6 adjoint RX, 7 adjoint cost layer,
+   * 8 cost layer then RX of the same layer (forward programs only; the
+   * phases ride in the first RX chunk; mask must be non-empty) */
+  int32_t stage[QFX_QAOA_MAX_STAGES][3];
 } qfx_qaoa_pass;
 
 typedef struct {
