@@ -399,10 +399,19 @@ __device__ __forceinline__ void run_subs(v2d (&v)[kRegs], const qfx_tile_op& op,
   const int4* rec = reinterpret_cast<const int4*>(subs + op.src);
   const int nsub = op.coef;
   int4 next = rec[0];
-  for (int i = 0; i < nsub; ++i) {
+  // Two records per trip: every case of apply_sub writes its results to
+  // fresh registers, and with one record per trip the loop latch copied
+  // all 16 amplitudes back (32 64-bit moves per sub-op).  With two, the
+  // second copy's results land in the loop's own registers.
+  for (int i = 0; i < nsub; i += 2) {
     const int4 cur = next;
     next = rec[i + 1 < nsub ? i + 1 : i];
     apply_sub(v, cur, coef, gbase, egrp);
+    if (i + 1 < nsub) {
+      const int4 cur2 = next;
+      next = rec[i + 2 < nsub ? i + 2 : i + 1];
+      apply_sub(v, cur2, coef, gbase, egrp);
+    }
   }
 }
 
