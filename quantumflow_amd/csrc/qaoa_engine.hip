@@ -41,6 +41,8 @@ enum Stage : int {
   ST_RX_ADJ = 6,       // grad_beta += Im sum <lam|X_j|psi>; RX(-beta) on psi, lam
   ST_COST_ADJ = 7,     // grad_gamma += 2 Im <lam|C|psi> - W Im <lam|psi>; conj phases
   ST_COST_RX = 8,      // ST_COST then ST_RX of the same layer (phases in one RX chunk)
+  ST_RX_LOAD = 10,     // ST_RX applied while the tile is loaded (first stage)
+  ST_RX_STORE = 11,    // ST_RX applied while the tile is stored (last stage)
 };
 
 enum PassFlags : int { PF_LOAD_PSI = 1, PF_LOAD_LAM = 2, PF_STORE_PSI = 4, PF_STORE_LAM = 8 };
@@ -110,6 +112,81 @@ __device__ __forceinline__ void for_elems(const Ctx& c, F&& f) {
   for (int k = 0; k < nk; ++k) {
     f(threadIdx.x + (unsigned)k * BS, zl | zh);
     zh = ((zh | ~hm) + h0) & hm;
+  }
+}
+
+// Elements a thread holds at once in a load or store with RX: bits 0..3 of
+// its element index k.
+constexpr int kIoBits = 4;
+constexpr int kIoRegs = 1 << kIoBits;
+
+// RX(theta) on the bits of km (bit j <-> bit j of the register index) of
+// kIoRegs register amplitudes: rx_regs with a run-time bit set.
+__device__ __forceinline__ void rx_kregs(v2d (&v)[kIoRegs], unsigned km, double c, double s) {
+#pragma unroll
+  for (int j = 0; j < kIoBits; ++j) {
+    if (!((km >> j) & 1u)) continue;
+#pragma unroll
+    for (int r = 0; r < kIoRegs; ++r) {
+      if (r & (1 << j)) continue;
+      const v2d x = v[r], y = v[r | (1 << j)];
+      v2d nx, ny;
+      nx.x = fma(c, x.x, s * y.y);
+      nx.y = fma(c, x.y, -s * y.x);
+      ny.x = fma(c, y.x, s * x.y);
+      ny.y = fma(c, y.y, -s * x.x);
+      v[r] = nx;
+      v[r | (1 << j)] = ny;
+    }
+  }
+}
+
+// Tile load (LOAD) or store of the state with RX(theta) on tile bits held in
+// registers on the way.  Thread t owns elements e = t + k BS; it takes them
+// kIoRegs consecutive k at a time, so bits 0..3 of k (tile bits log2 BS ..
+// +3) pair elements it holds.  km: those bits, as bits of k.
+template <int BS, bool LOAD>
+__device__ __forceinline__ void io_tile_rx(const Ctx& c, v2d* L, v2d* g, unsigned km,
+                                           double cs, double sn) {
+  constexpr int kLb = __builtin_ctz(BS);
+  const unsigned tsize = 1u << c.T;
+  if (threadIdx.x >= tsize) return;
+  uint64_t zl = c.base, hm = 0, h0 = 0;
+#pragma unroll
+  for (int j = 0; j < kMaxT; ++j) {
+    if (j >= c.T) continue;
+    const uint64_t bit = (uint64_t)1 << c.ps->pos[j];
+    if (j < kLb) zl |= ((threadIdx.x >> j) & 1u) ? bit : 0;
+    else hm |= bit;
+    if (j == kLb) h0 = bit;
+  }
+  const int nk = tsize > (unsigned)BS ? (int)(tsize / BS) : 1;
+  uint64_t zh = 0;
+  // the loop that touches HBM steps zh, once per element
+  for (int k0 = 0; k0 < nk; k0 += kIoRegs) {
+    v2d v[kIoRegs];
+#pragma unroll
+    for (int i = 0; i < kIoRegs; ++i) {
+      const unsigned e = threadIdx.x + (unsigned)(k0 + i) * BS;
+      v[i] = v2d{0.0, 0.0};
+      if (LOAD) {
+        if (k0 + i < nk) v[i] = g[zl | zh];
+        zh = ((zh | ~hm) + h0) & hm;
+      } else if (k0 + i < nk) {
+        v[i] = L[swz(e)];
+      }
+    }
+    rx_kregs(v, km, cs, sn);
+#pragma unroll
+    for (int i = 0; i < kIoRegs; ++i) {
+      const unsigned e = threadIdx.x + (unsigned)(k0 + i) * BS;
+      if (LOAD) {
+        if (k0 + i < nk) L[swz(e)] = v[i];
+      } else {
+        if (k0 + i < nk) g[zl | zh] = v[i];
+        zh = ((zh | ~hm) + h0) & hm;
+      }
+    }
   }
 }
 
@@ -339,15 +416,43 @@ __device__ void run_item(const qfx_qaoa_inst& in, const qfx_qaoa_pass& ps, uint3
   v2d* psi = psi_pool + in.psi_off;
   v2d* lam = lam_pool ? lam_pool + in.lam_off : nullptr;
   if (threadIdx.x < kAccum) accum[threadIdx.x] = 0.0;
-  if (ps.flags & PF_LOAD_PSI)
-    for_elems<BS>(c, [&](unsigned e, uint64_t z) { L[swz(e)] = psi[z]; });
+  // RX bits applied on the way in from / out to HBM (forward programs): the
+  // bits a thread holds in registers (kIoBits above log2 BS) there, the
+  // others as an LDS chunk right after the load or before the store.
+  constexpr int kLb = __builtin_ctz(BS);
+  constexpr unsigned kRegMask = (kIoRegs - 1u) << kLb;
+  int s_begin = 0, s_end = ps.nstage;
+  unsigned ld_mask = 0, st_mask = 0;
+  int ld_layer = 0, st_layer = 0;
+  if (!BWD) {
+    if (s_end > 0 && ps.stage[0][0] == ST_RX_LOAD) {
+      ld_layer = ps.stage[0][1];
+      ld_mask = (unsigned)ps.stage[0][2];
+      s_begin = 1;
+    }
+    if (s_end > s_begin && ps.stage[s_end - 1][0] == ST_RX_STORE) {
+      st_layer = ps.stage[s_end - 1][1];
+      st_mask = (unsigned)ps.stage[s_end - 1][2];
+      s_end -= 1;
+    }
+  }
+  if (ps.flags & PF_LOAD_PSI) {
+    if (!BWD && (ld_mask & kRegMask)) {
+      const double th = 0.5 * c.beta[ld_layer];
+      io_tile_rx<BS, true>(c, L, psi, (ld_mask & kRegMask) >> kLb, cos(th), sin(th));
+    } else {
+      for_elems<BS>(c, [&](unsigned e, uint64_t z) { L[swz(e)] = psi[z]; });
+    }
+  }
   if (BWD && (ps.flags & PF_LOAD_LAM))
     for_elems<BS>(c, [&](unsigned e, uint64_t z) { LL[swz(e)] = lam[z]; });
   __syncthreads();
+  if (!BWD && (ld_mask & ~kRegMask))
+    rx_mask<BS, false, false>(L, LL, c.T, ld_mask & ~kRegMask, c.beta[ld_layer], c, tab);
   const bool unit = !in.weighted;
   // Weighted instances run a fused stage as its two parts (their phases
   // come from sincos per amplitude, not from the cut-count table).
-  for (int si = 0; si < ps.nstage; ++si) {
+  for (int si = s_begin; si < s_end; ++si) {
     const int stype = ps.stage[si][0], layer = ps.stage[si][1];
     const unsigned mask = (unsigned)ps.stage[si][2];
     const bool split = !BWD && !unit && stype == ST_COST_RX;
@@ -444,8 +549,16 @@ __device__ void run_item(const qfx_qaoa_inst& in, const qfx_qaoa_pass& ps, uint3
       }
     }  // part
   }
-  if (ps.flags & PF_STORE_PSI)
-    for_elems<BS>(c, [&](unsigned e, uint64_t z) { psi[z] = L[swz(e)]; });
+  if (!BWD && (st_mask & ~kRegMask))
+    rx_mask<BS, false, false>(L, LL, c.T, st_mask & ~kRegMask, c.beta[st_layer], c, tab);
+  if (ps.flags & PF_STORE_PSI) {
+    if (!BWD && (st_mask & kRegMask)) {
+      const double th = 0.5 * c.beta[st_layer];
+      io_tile_rx<BS, false>(c, L, psi, (st_mask & kRegMask) >> kLb, cos(th), sin(th));
+    } else {
+      for_elems<BS>(c, [&](unsigned e, uint64_t z) { psi[z] = L[swz(e)]; });
+    }
+  }
   if (BWD && (ps.flags & PF_STORE_LAM))
     for_elems<BS>(c, [&](unsigned e, uint64_t z) { lam[z] = LL[swz(e)]; });
   __syncthreads();
@@ -626,9 +739,17 @@ int qfx_qaoa_check(const qfx_qaoa_inst* insts, int ninst, const qfx_qaoa_pass* p
     for (int si = 0; si < ps.nstage; ++si) {
       const int ty = ps.stage[si][0], l = ps.stage[si][1];
       const unsigned mask = (unsigned)ps.stage[si][2];
-      if (ty < ST_INIT || ty > ST_COST_RX || l < 0 || l >= kMaxPLayers)
+      if (ty < ST_INIT || (ty > ST_COST_RX && ty != ST_RX_LOAD && ty != ST_RX_STORE) || l < 0 ||
+          l >= kMaxPLayers)
         return qfx::set_error(QFX_EINVAL, "qfx_qaoa_check: bad stage");
-      const bool rx = ty == ST_RX || ty == ST_RX_ADJ || ty == ST_COST_RX;
+      const bool io = ty == ST_RX_LOAD || ty == ST_RX_STORE;
+      const bool rx = ty == ST_RX || ty == ST_RX_ADJ || ty == ST_COST_RX || io;
+      if (io && (mode == QFX_QAOA_BACKWARD || mask == 0))
+        return qfx::set_error(QFX_EINVAL, "qfx_qaoa_check: bad load/store RX stage");
+      if (ty == ST_RX_LOAD && (si != 0 || !(ps.flags & PF_LOAD_PSI)))
+        return qfx::set_error(QFX_EINVAL, "qfx_qaoa_check: load RX stage not first or no load");
+      if (ty == ST_RX_STORE && (si != ps.nstage - 1 || !(ps.flags & PF_STORE_PSI)))
+        return qfx::set_error(QFX_EINVAL, "qfx_qaoa_check: store RX stage not last or no store");
       if (rx && (mask >> ps.T))
         return qfx::set_error(QFX_EINVAL, "qfx_qaoa_check: RX mask outside the tile");
       if (ty == ST_COST_RX && mask == 0)

@@ -35,6 +35,10 @@ ST_INIT, ST_COST, ST_RX, ST_EXPECT, ST_MAKE_LAMBDA, ST_RX_ADJ, ST_COST_ADJ = 1, 
 # fused stage: cost layer l then RX layer l (the phases ride in the first
 # mixer chunk)
 ST_COST_RX = 8
+# RX layer l on tile bits applied in registers while the tile is loaded
+# from / stored to HBM (the pass's first / last stage)
+ST_RX_LOAD, ST_RX_STORE = 10, 11
+IO_BITS = 4               # tile bits above log2(block) a thread holds in registers
 PF_LOAD_PSI, PF_LOAD_LAM, PF_STORE_PSI, PF_STORE_LAM = 1, 2, 4, 8
 MAX_LAYERS = 16
 MAX_STAGES = 40
@@ -112,13 +116,40 @@ def fuse_stages(stages):
     return out
 
 
-def build_program(n, p, mode, T, fuse=True):
+def io_stages(tset, stages, flags, block):
+    """Move RX bits that a thread holds in registers during the tile's HBM
+    load or store out of the pass's first or last mixer stage.  Thread t
+    owns tile elements t + k*block and holds 2^IO_BITS consecutive k at a
+    time, so the tile bits log2(block) .. +IO_BITS-1 pair elements of one
+    thread: RX on them needs no LDS sweep.  The first stage gives up such bits to ST_RX_LOAD when the
+    pass loads the state; the last to ST_RX_STORE when it stores it.  A
+    stage keeps at least one bit (a fused cost stage needs its chunk)."""
+    lb = block.bit_length() - 1
+    kset = {b for j, b in enumerate(tset) if lb <= j < lb + IO_BITS}
+    stages = list(stages)
+    if (flags & PF_LOAD_PSI) and stages and stages[0][0] == ST_RX:
+        typ, layer, bits = stages[0]
+        mv = frozenset(bits & kset)
+        if mv:
+            rest = frozenset(bits - mv)
+            stages[0:1] = [(ST_RX_LOAD, layer, mv)] + ([(typ, layer, rest)] if rest else [])
+    if (flags & PF_STORE_PSI) and stages and stages[-1][0] in (ST_RX, ST_COST_RX):
+        typ, layer, bits = stages[-1]
+        mv = frozenset(bits & kset)
+        if mv and bits - mv:
+            stages[-1:] = [(typ, layer, frozenset(bits - mv)), (ST_RX_STORE, layer, mv)]
+    return stages
+
+
+def build_program(n, p, mode, T, fuse=True, block=512):
     """Passes of one instance: list of (tile bits, [(stage, layer, global bit
     set)], flags).  With `fuse`, forward-mode programs merge each cost stage
-    into the mixer stage of its layer (fuse_stages).  Backward programs keep
-    them separate: their 12-bit tiles leave a 4-bit chunk only 256 groups,
-    half of a 512-thread workgroup, and the fused stage measured slower
-    there than the separate sweep."""
+    into the mixer stage of its layer (fuse_stages) and apply RX bits held
+    in registers during the tile's load and store (io_stages, for
+    `block`-thread workgroups).  Backward programs keep their cost stages
+    separate: their 12-bit tiles leave a 4-bit chunk only 256 groups, half
+    of a 512-thread workgroup, and the fused stage measured slower there
+    than the separate sweep."""
     allbits = frozenset(range(n))
     seq = [(ST_INIT, 0, None)]
     for l in range(p):
@@ -163,8 +194,9 @@ def build_program(n, p, mode, T, fuse=True):
             flags |= PF_STORE_PSI
             if lam_live:
                 flags |= PF_STORE_LAM
-        fused = fuse and mode == MODE_FORWARD
-        out.append((tuple(tset), fuse_stages(stages) if fused else stages, flags))
+        if fuse and mode == MODE_FORWARD:
+            stages = io_stages(tuple(tset), fuse_stages(stages), flags, block)
+        out.append((tuple(tset), stages, flags))
     return out
 
 
@@ -219,7 +251,7 @@ class QAOABatch:
         for i, g in enumerate(self.graphs):
             n, edges, w, weighted = graph_arrays(g)
             self.n.append(n)
-            prog = build_program(n, p, mode, T, fuse)
+            prog = build_program(n, p, mode, T, fuse, block)
             inst = insts[i]
             inst.n, inst.p, inst.weighted, inst.nedges = n, p, int(weighted), len(edges)
             inst.wsum = float(np.sum(w))

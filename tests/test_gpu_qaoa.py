@@ -131,10 +131,11 @@ def test_weighted_graph(E):
 
 @pytest.mark.parametrize("mode", [0, 1])
 def test_fused_cost_stages_match_separate_stages(E, mode):
-    """Cost layers fused into mixer chunks (the default program) against
-    the same program with separate cost sweeps, on multi-pass instances at
-    two tile widths, plus a weighted multi-pass instance (whose fused stages
-    run as their two parts) against the oracle."""
+    """The default program (forward: cost layers fused into mixer chunks,
+    RX bits applied in registers during tile loads and stores) against the
+    same program with separate stages, on multi-pass instances at several
+    tile widths and workgroup sizes, plus a weighted multi-pass instance
+    (whose fused stages run as their two parts) against the oracle."""
     import torch
     rng = np.random.default_rng(8 + mode)
     graphs = [er(n, 0.5, 80 + n) for n in (7, 11, 14, 17)]
@@ -145,10 +146,12 @@ def test_fused_cost_stages_match_separate_stages(E, mode):
     p = 3
     beta = rng.uniform(0, np.pi / 2, size=(len(graphs), p))
     gamma = rng.uniform(0, np.pi, size=(len(graphs), p))
-    for T in (None, 8):
-        print("fused vs separate, mode", mode, "tile bits", T, flush=True)
-        va, ga = E.QAOABatch(graphs, p, mode=mode, tile_bits=T).evaluate(beta, gamma)
-        vb, gb = E.QAOABatch(graphs, p, mode=mode, tile_bits=T, fuse=False).evaluate(beta, gamma)
+    configs = [(None, 512), (8, 512)] if mode == 1 else [(None, 512), (13, 256), (10, 256), (8, 512)]
+    for T, block in configs:
+        print("fused vs separate, mode", mode, "tile bits", T, "block", block, flush=True)
+        va, ga = E.QAOABatch(graphs, p, mode=mode, tile_bits=T, block=block).evaluate(beta, gamma)
+        vb, gb = E.QAOABatch(graphs, p, mode=mode, tile_bits=T, block=block,
+                             fuse=False).evaluate(beta, gamma)
         assert torch.max(torch.abs(va - vb)).item() <= 1e-12
         if mode == 1:
             assert torch.max(torch.abs(ga - gb)).item() <= 1e-12

@@ -16,11 +16,11 @@ from oracle import qf_oracle as O
 from quantumflow_amd import qaoa_engine as E
 
 
-def emulate(n, edges, weights, beta, gamma, mode, T, fuse=True):
+def emulate(n, edges, weights, beta, gamma, mode, T, fuse=True, block=512):
     cuts = O.graph_cuts(n, edges, weights).reshape(-1)
     W = float(len(edges) if weights is None else np.sum(weights))
     p = len(beta)
-    prog = E.build_program(n, p, mode, T, fuse)
+    prog = E.build_program(n, p, mode, T, fuse, block)
     idx = np.arange(2 ** n)
     psi = lam = None
     value = 0.0
@@ -52,7 +52,11 @@ def emulate(n, edges, weights, beta, gamma, mode, T, fuse=True):
                 for b in bits:
                     seen[(layer, b)] = seen.get((layer, b), 0) + 1
                 psi = rx(psi, sorted(bits), beta[layer])
-            elif typ == E.ST_RX:
+            elif typ in (E.ST_RX, E.ST_RX_LOAD, E.ST_RX_STORE):
+                if typ == E.ST_RX_LOAD:
+                    assert stages[0][0] == typ and flags & E.PF_LOAD_PSI
+                if typ == E.ST_RX_STORE:
+                    assert stages[-1][0] == typ and flags & E.PF_STORE_PSI
                 for b in bits:
                     seen[(layer, b)] = seen.get((layer, b), 0) + 1
                 psi = rx(psi, sorted(bits), beta[layer])
@@ -91,6 +95,10 @@ def test_program_emulation_matches_oracle(n, T, p, fuse):
     assert np.max(np.abs(gg - sg)) < 1e-12
     v2, _, _ = emulate(n, edges, None, beta, gamma, E.MODE_FORWARD, T + 1, fuse)
     assert abs(v2 - ref) < 1e-12
+    # small workgroups put register-held RX bits inside these small tiles
+    for block in (2, 4, 8):
+        v3, _, _ = emulate(n, edges, None, beta, gamma, E.MODE_FORWARD, T, fuse, block)
+        assert abs(v3 - ref) < 1e-12
 
 
 def test_weighted_program_gradient():
