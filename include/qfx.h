@@ -297,7 +297,11 @@ typedef struct {
   /* {type, layer, tile-bit mask}; types: 1 init |+>, 2 cost layer, 3 RX on
    * the mask, 4 <C>, 5 lambda = C psi, 6 adjoint RX, 7 adjoint cost layer,
    * 8 cost layer then RX of the same layer (forward programs only; the
-   * phases ride in the first RX chunk; mask must be non-empty) */
+   * phases ride in the first RX chunk; mask must be non-empty), 10 / 11 RX
+   * applied while the tile is loaded / stored (forward programs only; the
+   * pass's first stage with flag 1 / last stage with flag 4; the bits a
+   * thread holds in registers then, the 4 tile bits above log2(block), cost
+   * no LDS sweep, any others run as an LDS chunk) */
   int32_t stage[QFX_QAOA_MAX_STAGES][3];
 } qfx_qaoa_pass;
 
