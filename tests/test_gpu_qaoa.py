@@ -273,3 +273,29 @@ def test_engine_values_on_standard_graphs(E):
     for i, g in enumerate(graphs):
         ref = O.qaoa_expectation(g.number_of_nodes(), list(g.edges()), beta[i], gamma[i])
         assert abs(val[i] - ref) <= 1e-10
+
+
+def test_program_runs_with_another_block_size(E):
+    """Forward programs move the RX bits a thread of their block size holds
+    in registers into tile loads and stores.  Run with another block size,
+    the kernel applies the bits it does not hold as LDS chunks, so results
+    match the unmerged program."""
+    import torch
+    rng = np.random.default_rng(12)
+    graphs = [er(n, 0.5, 120 + n) for n in (9, 14, 16)]
+    p = 2
+    beta = rng.uniform(0, np.pi / 2, size=(len(graphs), p))
+    gamma = rng.uniform(0, np.pi, size=(len(graphs), p))
+    ref, _ = E.QAOABatch(graphs, p, mode=E.MODE_FORWARD, tile_bits=13, block=512,
+                         fuse=False).evaluate(beta, gamma)
+    ref = ref.clone()
+    for built, run in ((512, 256), (256, 512), (256, 1024)):
+        print("program for block", built, "run with", run, flush=True)
+        b = E.QAOABatch(graphs, p, mode=E.MODE_FORWARD, tile_bits=13, block=built)
+        assert any(st[0] in (E.ST_RX_LOAD, E.ST_RX_STORE)
+                   for _, sts, _ in E.build_program(16, p, E.MODE_FORWARD, 13, True, built)
+                   for st in sts)
+        b._args.block = run
+        v, _ = b.evaluate(beta, gamma)
+        b.check()
+        assert torch.max(torch.abs(v - ref)).item() <= 1e-12
