@@ -150,6 +150,27 @@ int qfx_sample(void* stream, int64_t* out, const void* psi, int n,
 int qfx_histogram(void* stream, int64_t* counts, const int64_t* idx, int64_t m,
                   int n);
 
+/* ---- density matrices (quantumflow_amd/channels.py) --------------------
+ * rho: device complex128 [2^n x 2^n], row-major (row = ket index), the
+ * vectorised 2n-qubit tensor of the reference's Density
+ * (quantumflow/states.py:264-344).  Gates, Kraus operators and channels act
+ * on it through qfx_apply_* (rows by U, columns by conj(U)); these add the
+ * rest.  1 <= n <= QFX_DENSITY_MAX_QUBITS. */
+#define QFX_DENSITY_MAX_QUBITS 15
+/* rho = psi psi^H; replaces State.asdensity (quantumflow/states.py:162-165) */
+int qfx_density_outer(void* stream, void* rho, const void* psi, int n);
+/* rho = scale I; replaces mixed_density (quantumflow/states.py:322-326) */
+int qfx_density_identity(void* stream, void* rho, int n, double scale);
+/* out[i] = Re rho_ii (float64 [2^n]); replaces Density.probabilities
+ * (quantumflow/states.py:302-305) */
+int qfx_density_diag(void* stream, void* out, const void* rho, int n);
+/* out = sum_i rho_ii (one complex128, deterministic); replaces
+ * Density.trace (quantumflow/states.py:277-279) */
+int qfx_density_trace(void* stream, void* out, const void* rho, int n);
+/* y += w x over len complex128 elements; the weighted sum of
+ * Kraus.evolve (quantumflow/channels.py:74-80) */
+int qfx_density_axpy(void* stream, void* y, const void* x, double w, int64_t len);
+
 /* ---- fused circuit execution: LDS-tiled passes --------------------------
  * Replaces the per-gate loop of Circuit.run (quantumflow/circuits.py:87-99)
  * for runs of gates that act inside one tile bit set.  A pass loads every

@@ -666,3 +666,31 @@ def reset_qubits(tensor, axes):
 
 def all_bitstrings(n):
     return np.array(list(itertools.product([0, 1], repeat=n)), dtype=np.int64)
+
+
+# ---- density matrices -----------------------------------------------------
+# A density matrix is a 2^n x 2^n complex128 array, row = ket index (big-
+# endian over the qubit axes), the flattening of the reference's rank-2
+# QubitVector (quantumflow/states.py:264-344).
+
+def density_from_state(tensor):
+    """|psi><psi| (quantumflow/states.py:162-165)."""
+    v = np.asarray(tensor, dtype=CTYPE).reshape(-1)
+    return np.outer(v, v.conj())
+
+
+def density_evolve(rho, matrix, n, indices):
+    """U rho U^H for a K-qubit gate on axes ``indices`` (quantumflow/ops.py:
+    168-172, whose superoperator ops.py:243-252 is this map), from the full
+    2^n x 2^n operator (full_operator)."""
+    U = full_operator(matrix, n, indices)
+    return U @ np.asarray(rho, dtype=CTYPE) @ U.conj().T
+
+
+def kraus_evolve(rho, ops, n):
+    """sum_k w_k K_k rho K_k^H; ops = [(matrix, indices, weight)]
+    (quantumflow/channels.py:74-80)."""
+    out = np.zeros((2 ** n, 2 ** n), dtype=CTYPE)
+    for matrix, indices, w in ops:
+        out += w * density_evolve(rho, matrix, n, indices)
+    return out

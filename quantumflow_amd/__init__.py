@@ -17,6 +17,7 @@ from .gates import *  # noqa: F401,F403
 from .stdgates import *  # noqa: F401,F403
 from .stdops import *  # noqa: F401,F403
 from .circuits import *  # noqa: F401,F403
+from .channels import *  # noqa: F401,F403
 from .dagcircuit import *  # noqa: F401,F403
 from .measures import *  # noqa: F401,F403
 from . import qaoa  # noqa: F401

@@ -67,6 +67,11 @@ SIGNATURES = {
     'qfx_sample_workspace_bytes': (_i64, [_i]),
     'qfx_sample': (_i, [_vp, _vp, _vp, _i, _i64, ctypes.c_uint64, _vp]),
     'qfx_histogram': (_i, [_vp, _vp, _vp, _i64, _i]),
+    'qfx_density_outer': (_i, [_vp, _vp, _vp, _i]),
+    'qfx_density_identity': (_i, [_vp, _vp, _i, ctypes.c_double]),
+    'qfx_density_diag': (_i, [_vp, _vp, _vp, _i]),
+    'qfx_density_trace': (_i, [_vp, _vp, _vp, _i]),
+    'qfx_density_axpy': (_i, [_vp, _vp, _vp, ctypes.c_double, _i64]),
     'qfx_cut_table': (_i, [_vp, _vp, _i, _vp, _vp, _i]),
     'qfx_cut_counts': (_i, [_vp, _vp, _i, _vp, _i]),
     'qfx_qaoa_check': (_i, [_vp, _i, _vp, _i, _vp, _vp, _i, _i, _i64, _i64, _i64, _i64, _i64, _i,

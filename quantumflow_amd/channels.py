@@ -1,0 +1,239 @@
+"""Density matrices and Kraus channels on the GPU.
+
+Mirrors the reference's Density (quantumflow/states.py:264-344) and
+channels (quantumflow/channels.py:36-165).  A Density of N qubits holds its
+matrix as a [2]*2N complex128 device tensor: axes 0..N-1 are the row (ket)
+indices, N..2N-1 the column (bra) indices.  A K-qubit operator U acts as
+rho -> U rho U^H, which is U applied to the row axes and conj(U) to the
+column axes: two calls of the state-vector gate kernels on the 2N-axis
+tensor.  The reference builds a superoperator Channel for the same map
+(Gate.evolve, quantumflow/ops.py:168-172 -> Gate.aschannel, ops.py:243-252);
+the result is the same matrix.  Outer products, identity, diagonal, trace
+and the weighted sums of Kraus terms run in csrc/density.hip.
+"""
+
+from typing import Sequence, Union
+
+import numpy as np
+import torch
+
+from . import _native
+from . import backend as bk
+from .ops import Gate, Operation
+from .qubits import Qubits, qubits_count_tuple
+from .states import State
+
+__all__ = ['Density', 'mixed_density', 'random_density', 'Kraus', 'UnitaryMixture',
+           'Depolarizing', 'Damping', 'Dephasing', 'kraus_iscomplete']
+
+MAX_QUBITS = 15        # QFX_DENSITY_MAX_QUBITS
+
+
+def _ptr(t):
+    return _native.ctypes.c_void_p(t.data_ptr())
+
+
+def _check_n(n):
+    if not 1 <= n <= MAX_QUBITS:
+        raise ValueError('density matrices hold 1..%d qubits' % MAX_QUBITS)
+
+
+class Density(State):
+    """A density matrix of a mixed state (states.py:264-320).  ``tensor``:
+    the 2^N x 2^N matrix in any shape, host or device."""
+
+    def __init__(self, tensor, qubits: Qubits = None, memory=None) -> None:
+        if qubits is None:
+            tensor = bk.astensorproduct(tensor)
+            qubits = range(bk.rank(tensor) // 2)
+        super().__init__(tensor, qubits, memory)
+        _check_n(self.qubit_nb)
+
+    def _mat(self):
+        return bk.as_device(self.tensor)
+
+    def trace(self):
+        """sum_i rho_ii as a 0-dim complex128 device tensor (states.py:277-279)."""
+        out = torch.empty(1, dtype=torch.complex128, device=bk.DEVICE)
+        _native.call('qfx_density_trace', bk.stream_handle(), _ptr(out),
+                     _ptr(self._mat()), self.qubit_nb)
+        return out[0]
+
+    def normalize(self) -> 'Density':
+        """rho / trace(rho) (states.py:296-299)."""
+        tr = complex(self.trace().item())
+        out = torch.zeros_like(self._mat())
+        _native.call('qfx_density_axpy', bk.stream_handle(), _ptr(out),
+                     _ptr(self._mat()), 1.0 / tr.real, out.numel())
+        return Density(out.view([2] * (2 * self.qubit_nb)), self.qubits, self._memory)
+
+    def probabilities(self):
+        """Re rho_ii as a [2]*N float64 device tensor (states.py:302-305)."""
+        n = self.qubit_nb
+        out = torch.empty(2 ** n, dtype=torch.float64, device=bk.DEVICE)
+        _native.call('qfx_density_diag', bk.stream_handle(), _ptr(out),
+                     _ptr(self._mat()), n)
+        return out.view([2] * n)
+
+    def asoperator(self):
+        """The 2^N x 2^N matrix (a device view) (states.py:307-309)."""
+        return self._mat().view(2 ** self.qubit_nb, 2 ** self.qubit_nb)
+
+    def asdensity(self) -> 'Density':
+        return self
+
+    def relabel(self, qubits: Qubits) -> 'Density':
+        return Density(self.tensor, qubits, self._memory)
+
+    def update(self, memory) -> 'Density':
+        mem = self.memory
+        mem.update(memory)
+        return Density(self.tensor, self.qubits, mem)
+
+
+def mixed_density(qubits: Union[int, Qubits]) -> Density:
+    """I / 2^N, filled on the device (states.py:322-326)."""
+    N, qubits = qubits_count_tuple(qubits)
+    _check_n(N)
+    rho = torch.empty(4 ** N, dtype=torch.complex128, device=bk.DEVICE)
+    _native.call('qfx_density_identity', bk.stream_handle(), _ptr(rho), N, 2.0 ** -N)
+    return Density(rho.view([2] * (2 * N)), qubits)
+
+
+def random_density(qubits: Union[int, Qubits]) -> Density:
+    """A Hilbert-Schmidt random density (states.py:329-344): G G^H / tr for a
+    Ginibre matrix G drawn on the host (numpy's global generator)."""
+    N, qubits = qubits_count_tuple(qubits)
+    size = (2 ** N, 2 ** N)
+    g = (np.random.normal(size=size) + 1j * np.random.normal(size=size)) / np.sqrt(2.0)
+    m = g @ g.conj().T
+    m /= np.trace(m)
+    return Density(m.reshape([2] * (2 * N)), qubits)
+
+
+def state_asdensity(ket: State) -> Density:
+    """|psi><psi| from a state, on the device (states.py:162-165)."""
+    n = ket.qubit_nb
+    _check_n(n)
+    psi = bk.as_device(ket.tensor)
+    rho = torch.empty(4 ** n, dtype=torch.complex128, device=bk.DEVICE)
+    _native.call('qfx_density_outer', bk.stream_handle(), _ptr(rho), _ptr(psi), n)
+    return Density(rho.view([2] * (2 * n)), ket.qubits, ket._memory)
+
+
+def gate_evolve(gate: Gate, rho: Density) -> Density:
+    """U rho U^H (ops.py:168-172): U on the row axes of the gate's qubits,
+    conj(U) on their column axes."""
+    N = rho.qubit_nb
+    idx = [rho.qubits.index(q) for q in gate.qubits]
+    k = len(idx)
+    m = gate.matrix
+    t = bk.tensormul(m.reshape([2] * (2 * k)), rho.tensor, idx)
+    t = bk.tensormul(np.conj(m).reshape([2] * (2 * k)), t, [N + i for i in idx])
+    return Density(t, rho.qubits, rho._memory)
+
+
+class Kraus(Operation):
+    """A Kraus representation of a channel (channels.py:36-105):
+    rho -> sum_k w_k K_k rho K_k^H."""
+
+    def __init__(self, operators: Sequence[Gate], weights: Sequence[float] = None) -> None:
+        self.operators = list(operators)
+        if weights is None:
+            weights = [1.0] * len(self.operators)
+        self.weights = tuple(float(w) for w in weights)
+
+    @property
+    def qubits(self) -> Qubits:
+        return tuple(sorted(set(q for op in self.operators for q in op.qubits)))
+
+    def asgate(self) -> Gate:
+        raise TypeError('Not possible in general')
+
+    def evolve(self, rho: Density) -> Density:
+        """Weighted sum of the operators' evolutions (channels.py:74-80)."""
+        out = torch.zeros(4 ** rho.qubit_nb, dtype=torch.complex128, device=bk.DEVICE)
+        for op, w in zip(self.operators, self.weights):
+            term = bk.as_device(op.evolve(rho).tensor)
+            _native.call('qfx_density_axpy', bk.stream_handle(), _ptr(out), _ptr(term),
+                         w, out.numel())
+        return Density(out.view([2] * (2 * rho.qubit_nb)), rho.qubits, rho._memory)
+
+    def run(self, ket: State) -> State:
+        """One Kraus branch drawn with probability w_k |K_k psi|^2, the state
+        renormalised (channels.py:64-71)."""
+        res = [op.run(ket) for op in self.operators]
+        probs = np.array([float(np.abs(bk.evaluate(r.norm()))) * w
+                          for r, w in zip(res, self.weights)])
+        probs /= np.sum(probs)
+        pick = res[np.random.choice(len(res), p=probs)]
+        return pick.normalize()
+
+    @property
+    def H(self) -> 'Kraus':
+        return Kraus([op.H for op in self.operators], self.weights)
+
+
+class UnitaryMixture(Kraus):
+    """A convex mixture of unitaries (channels.py:108-125)."""
+
+    def asgate(self) -> Gate:
+        return self.operators[np.random.choice(len(self.operators), p=self.weights)]
+
+    def run(self, ket: State) -> State:
+        return self.asgate().run(ket)
+
+
+class Depolarizing(UnitaryMixture):
+    """I, X, Y, Z with weights 1-p, p/3, p/3, p/3 (channels.py:128-138)."""
+
+    def __init__(self, prob: float, q0) -> None:
+        from .stdgates import I, X, Y, Z
+        super().__init__([I(q0), X(q0), Y(q0), Z(q0)],
+                         [1 - prob, prob / 3.0, prob / 3.0, prob / 3.0])
+
+
+class Damping(Kraus):
+    """Amplitude damping: [[1, 0], [0, sqrt(1-p)]] and [[0, sqrt(p)], [0, 0]]
+    (channels.py:141-153)."""
+
+    def __init__(self, prob: float, q0) -> None:
+        k0 = Gate(np.array([[1.0, 0.0], [0.0, np.sqrt(1 - prob)]]), qubits=[q0])
+        k1 = Gate(np.array([[0.0, np.sqrt(prob)], [0.0, 0.0]]), qubits=[q0])
+        super().__init__([k0, k1])
+
+
+class Dephasing(UnitaryMixture):
+    """I and Z with weights 1-p/2, p/2 (channels.py:156-165)."""
+
+    def __init__(self, prob: float, q0) -> None:
+        from .stdgates import I, Z
+        super().__init__([I(q0), Z(q0)], [1 - prob / 2, prob / 2])
+
+
+def _embed(mat, pos, N):
+    """A 2^k x 2^k operator on the qubits at positions ``pos`` (position 0 =
+    most significant) as a 2^N x 2^N matrix (identity elsewhere)."""
+    k = len(pos)
+    others = [i for i in range(N) if i not in pos]
+    t = np.multiply.outer(mat.reshape([2] * (2 * k)),
+                          np.eye(2 ** (N - k)).reshape([2] * (2 * (N - k))))
+    # axes of t: operator out (k), operator in (k), identity out, identity in
+    out_ax = {p: j for j, p in enumerate(pos)}
+    out_ax.update({o: 2 * k + j for j, o in enumerate(others)})
+    in_ax = {p: k + j for j, p in enumerate(pos)}
+    in_ax.update({o: 2 * k + (N - k) + j for j, o in enumerate(others)})
+    perm = [out_ax[i] for i in range(N)] + [in_ax[i] for i in range(N)]
+    return np.transpose(t, perm).reshape(2 ** N, 2 ** N)
+
+
+def kraus_iscomplete(kraus: Kraus) -> bool:
+    """sum_k w_k K_k^H K_k == I on the channel's qubits (channels.py:200-217),
+    from the operators' host matrices."""
+    qubits = kraus.qubits
+    N = len(qubits)
+    total = np.zeros((2 ** N, 2 ** N), dtype=np.complex128)
+    for op, w in zip(kraus.operators, kraus.weights):
+        kk = op.matrix.conj().T @ op.matrix
+        total += w * _embed(kk, [qubits.index(q) for q in op.qubits], N)
+    return bool(np.allclose(total, np.eye(2 ** N)))

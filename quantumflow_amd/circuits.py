@@ -80,6 +80,15 @@ class Circuit(Operation):
             return None
         return out
 
+    def evolve(self, rho=None):
+        """Apply every element to a Density in order (circuits.py:102-109);
+        default |0...0><0...0| on the circuit's qubits."""
+        if rho is None:
+            rho = zero_state(qubits=self.qubits).asdensity()
+        for elem in self.elements:
+            rho = elem.evolve(rho)
+        return rho
+
     def run(self, ket: State = None, fused: bool = None) -> State:
         """Apply the circuit to ``ket`` (default |0...0> on the sorted
         qubits).  ``fused=None`` picks the fused executor for circuits on

@@ -164,6 +164,12 @@ class Gate(Operation):
             tensor = bk.tensormul(self.kind, ket.tensor, indices)
         return State(tensor, ket.qubits, ket.memory)
 
+    def evolve(self, rho):
+        """rho -> U rho U^H on a Density (ops.py:168-172): the gate kernels
+        on the density's row axes (U) and column axes (conj U)."""
+        from .channels import gate_evolve
+        return gate_evolve(self, rho)
+
     def __pow__(self, t: float) -> 'Gate':
         """Fractional matrix power (ops.py:175-184)."""
         N = self.qubit_nb

@@ -51,6 +51,11 @@ class State:
     def memory(self) -> dict:
         return defaultdict(int, self._memory)
 
+    def asdensity(self):
+        """|psi><psi| as a Density, formed on the device (states.py:162-165)."""
+        from .channels import state_asdensity
+        return state_asdensity(self)
+
     def update(self, memory: Dict[Any, Any]) -> 'State':
         mem = self.memory
         mem.update(memory)

@@ -1,0 +1,112 @@
+"""GPU parity of density matrices and Kraus channels against the oracle
+(complex128, tolerance 1e-12 on matrix entries)."""
+import numpy as np
+import pytest
+
+from oracle import qf_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def qf():
+    import quantumflow_amd as qf
+    assert qf.backend.gpu_available()
+    return qf
+
+
+def _mat(rho):
+    n = rho.qubit_nb
+    return np.asarray(rho.asoperator().cpu().numpy()).reshape(2 ** n, 2 ** n)
+
+
+def _rand_density(n, rng):
+    g = rng.normal(size=(2 ** n, 2 ** n)) + 1j * rng.normal(size=(2 ** n, 2 ** n))
+    m = g @ g.conj().T
+    return m / np.trace(m)
+
+
+def test_asdensity_trace_probabilities(qf):
+    rng = np.random.default_rng(1)
+    n = 6
+    psi = rng.normal(size=[2] * n) + 1j * rng.normal(size=[2] * n)
+    psi /= np.linalg.norm(psi)
+    print("asdensity n", n, flush=True)
+    rho = qf.State(psi).asdensity()
+    assert np.max(np.abs(_mat(rho) - O.density_from_state(psi))) <= 1e-12
+    assert abs(complex(rho.trace().item()) - 1.0) <= 1e-12
+    probs = rho.probabilities().cpu().numpy().reshape(-1)
+    assert np.max(np.abs(probs - np.abs(psi.reshape(-1)) ** 2)) <= 1e-12
+    mixed = qf.mixed_density(4)
+    assert np.max(np.abs(_mat(mixed) - np.eye(16) / 16)) == 0.0
+    half = qf.Density(_mat(rho).reshape([2] * (2 * n)) * 2.0)
+    assert np.max(np.abs(_mat(half.normalize()) - _mat(rho))) <= 1e-12
+
+
+@pytest.mark.parametrize("name,params,qubits", [
+    ("H", (), [3]), ("RX", (0.7,), [0]), ("CNOT", (), [4, 1]), ("ISWAP", (), [2, 5]),
+    ("CZ", (), [0, 5]), ("CCNOT", (), [5, 0, 3]), ("PHASE", (1.1,), [2])])
+def test_gate_evolve_matches_oracle(qf, name, params, qubits):
+    rng = np.random.default_rng(len(name) + len(qubits))
+    n = 6
+    m0 = _rand_density(n, rng)
+    print("evolve", name, qubits, flush=True)
+    rho = qf.Density(m0.reshape([2] * (2 * n)))
+    gate = getattr(qf, name)(*params, *qubits)
+    out = gate.evolve(rho)
+    ref = O.density_evolve(m0, O.gate_matrix(name, *params), n, qubits)
+    assert np.max(np.abs(_mat(out) - ref)) <= 1e-12
+
+
+def test_circuit_evolve_equals_pure_state_run(qf):
+    rng = np.random.default_rng(7)
+    n = 5
+    circ = qf.Circuit()
+    for _ in range(12):
+        a, b = rng.choice(n, size=2, replace=False)
+        circ += qf.RY(float(rng.uniform(0, 6)), int(a))
+        circ += qf.CNOT(int(a), int(b))
+    print("circuit evolve", flush=True)
+    rho = circ.evolve(qf.zero_state(n).asdensity())
+    ket = circ.run(qf.zero_state(n))
+    psi = ket.tensor.cpu().numpy()
+    assert np.max(np.abs(_mat(rho) - O.density_from_state(psi))) <= 1e-12
+
+
+def test_kraus_channels_match_oracle(qf):
+    rng = np.random.default_rng(9)
+    n = 4
+    m0 = _rand_density(n, rng)
+    rho = qf.Density(m0.reshape([2] * (2 * n)))
+    I2, X = np.eye(2), np.array([[0, 1], [1, 0]])
+    Y, Z = np.array([[0, -1j], [1j, 0]]), np.diag([1.0, -1.0])
+    p = 0.25
+    cases = [
+        (qf.Damping(p, 2), [(np.array([[1, 0], [0, np.sqrt(1 - p)]]), [2], 1.0),
+                            (np.array([[0, np.sqrt(p)], [0, 0]]), [2], 1.0)]),
+        (qf.Depolarizing(p, 0), [(I2, [0], 1 - p), (X, [0], p / 3), (Y, [0], p / 3),
+                                 (Z, [0], p / 3)]),
+        (qf.Dephasing(p, 3), [(I2, [3], 1 - p / 2), (Z, [3], p / 2)]),
+    ]
+    for chan, ops in cases:
+        print("channel", type(chan).__name__, flush=True)
+        out = chan.evolve(rho)
+        ref = O.kraus_evolve(m0, ops, n)
+        assert np.max(np.abs(_mat(out) - ref)) <= 1e-12
+        assert abs(complex(out.trace().item()) - 1.0) <= 1e-12
+    # closed form: damping sends |1><1| to diag(p, 1 - p)
+    one = qf.State(np.array([0.0, 1.0])).asdensity()
+    got = _mat(qf.Damping(p, 0).evolve(one))
+    assert np.max(np.abs(got - np.diag([p, 1 - p]))) <= 1e-15
+
+
+def test_density_entry_points_reject_bad_sizes(qf):
+    import torch
+    from quantumflow_amd import _native
+    lib = _native.load()
+    buf = torch.zeros(4, dtype=torch.complex128, device=qf.backend.DEVICE)
+    s = qf.backend.stream_handle()
+    ptr = _native.ctypes.c_void_p(buf.data_ptr())
+    assert lib.qfx_density_identity(s, ptr, 0, 1.0) == _native.QFX_ERANGE
+    assert lib.qfx_density_trace(s, ptr, ptr, 16) == _native.QFX_ERANGE
+    assert lib.qfx_density_outer(s, None, ptr, 1) == _native.QFX_EINVAL

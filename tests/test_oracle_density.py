@@ -1,0 +1,88 @@
+"""The oracle's density-matrix functions, pinned by closed forms and by the
+state-vector oracle (no GPU).
+
+* Pure states: evolving |psi><psi| by U equals |U psi><U psi| with U psi
+  from the state-vector oracle (tensormul, pinned in test_oracle.py).
+* Kraus channels on one qubit have textbook closed forms: amplitude damping
+  sends |1><1| to diag(p, 1 - p); depolarizing sends |0><0| to
+  diag(1 - 2p/3, 2p/3); dephasing scales the coherences of |+><+| by 1 - p.
+* Complete Kraus sets preserve trace and hermiticity; the maximally mixed
+  state is a fixed point of every unitary.
+"""
+import numpy as np
+import pytest
+
+from oracle import qf_oracle as O
+
+
+def _rand_unitary(k, rng):
+    m = rng.normal(size=(2 ** k, 2 ** k)) + 1j * rng.normal(size=(2 ** k, 2 ** k))
+    q, r = np.linalg.qr(m)
+    return q * (np.diag(r) / np.abs(np.diag(r)))
+
+
+def _rand_density(n, rng):
+    g = rng.normal(size=(2 ** n, 2 ** n)) + 1j * rng.normal(size=(2 ** n, 2 ** n))
+    m = g @ g.conj().T
+    return m / np.trace(m)
+
+
+@pytest.mark.parametrize("k,idx", [(1, [2]), (2, [3, 0]), (3, [1, 4, 2])])
+def test_pure_state_evolution_matches_state_vector_oracle(k, idx):
+    rng = np.random.default_rng(k)
+    n = 5
+    psi = rng.normal(size=[2] * n) + 1j * rng.normal(size=[2] * n)
+    psi /= np.linalg.norm(psi)
+    U = _rand_unitary(k, rng)
+    out = O.tensormul(U.reshape([2] * (2 * k)), psi, idx)
+    ref = O.density_from_state(out)
+    got = O.density_evolve(O.density_from_state(psi), U, n, idx)
+    assert np.max(np.abs(got - ref)) < 1e-13
+
+
+def test_damping_depolarizing_dephasing_closed_forms():
+    p = 0.3
+    one = np.diag([0.0, 1.0]).astype(complex)
+    damp = [(np.array([[1, 0], [0, np.sqrt(1 - p)]]), [0], 1.0),
+            (np.array([[0, np.sqrt(p)], [0, 0]]), [0], 1.0)]
+    assert np.allclose(O.kraus_evolve(one, damp, 1), np.diag([p, 1 - p]), atol=1e-15)
+    I2, X = np.eye(2), np.array([[0, 1], [1, 0]])
+    Y, Z = np.array([[0, -1j], [1j, 0]]), np.diag([1.0, -1.0])
+    zero = np.diag([1.0, 0.0]).astype(complex)
+    dep = [(I2, [0], 1 - p), (X, [0], p / 3), (Y, [0], p / 3), (Z, [0], p / 3)]
+    assert np.allclose(O.kraus_evolve(zero, dep, 1), np.diag([1 - 2 * p / 3, 2 * p / 3]),
+                       atol=1e-15)
+    plus = np.full((2, 2), 0.5, dtype=complex)
+    deph = [(I2, [0], 1 - p / 2), (Z, [0], p / 2)]
+    want = np.array([[0.5, 0.5 * (1 - p)], [0.5 * (1 - p), 0.5]])
+    assert np.allclose(O.kraus_evolve(plus, deph, 1), want, atol=1e-15)
+
+
+def test_trace_hermiticity_and_mixed_fixed_point():
+    rng = np.random.default_rng(3)
+    n = 4
+    rho = _rand_density(n, rng)
+    p = 0.2
+    ops = [(np.array([[1, 0], [0, np.sqrt(1 - p)]]), [1], 1.0),
+           (np.array([[0, np.sqrt(p)], [0, 0]]), [1], 1.0)]
+    out = O.kraus_evolve(rho, ops, n)
+    assert abs(np.trace(out) - 1.0) < 1e-13
+    assert np.max(np.abs(out - out.conj().T)) < 1e-14
+    mixed = np.eye(2 ** n) / 2 ** n
+    U = _rand_unitary(2, rng)
+    assert np.max(np.abs(O.density_evolve(mixed, U, n, [3, 1]) - mixed)) < 1e-15
+
+
+def test_kraus_completeness_on_the_host():
+    """kraus_iscomplete embeds K^H K with the oracle's full-operator
+    convention and accepts exactly the complete weighted sets."""
+    import quantumflow_amd as qf
+    from quantumflow_amd.channels import _embed
+    rng = np.random.default_rng(4)
+    m = rng.normal(size=(4, 4)) + 1j * rng.normal(size=(4, 4))
+    assert np.allclose(_embed(m, [2, 0], 3), O.full_operator(m, 3, [2, 0]))
+    assert qf.kraus_iscomplete(qf.Damping(0.3, 0))
+    assert qf.kraus_iscomplete(qf.Depolarizing(0.2, 1))
+    assert qf.kraus_iscomplete(qf.Dephasing(0.1, 0))
+    assert qf.kraus_iscomplete(qf.Kraus([qf.X(0), qf.CNOT(0, 1)], [0.5, 0.5]))
+    assert not qf.kraus_iscomplete(qf.Kraus([qf.X(0), qf.CNOT(0, 1)], [0.5, 0.4]))
