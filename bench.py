@@ -109,6 +109,51 @@ def fused_block_section(gates, qubits, work, widths=(3, 4, 5), reps=2):
     return out
 
 
+def density_section(n=13, ngates=40, seed=5):
+    """Density-matrix evolution: an n-qubit density (a 2n-axis tensor,
+    16 * 4^n bytes) through a random circuit of 1- and 2-qubit gates, then a
+    depolarizing layer on every qubit.  A gate is two gate-kernel passes
+    over the tensor (U on the row axes, conj U on the column axes); a
+    4-operator channel is four gate evolutions plus four weighted sums.
+    Synthetic data; HIP events around the whole sequence."""
+    import numpy as np
+    import torch
+    import quantumflow_amd as qf
+
+    rng = np.random.default_rng(seed)
+    gates = []
+    for _ in range(ngates):
+        if rng.random() < 0.5:
+            gates.append(qf.RX(float(rng.uniform(0, 6)), int(rng.integers(n))))
+        else:
+            a, b = rng.choice(n, size=2, replace=False)
+            gates.append(qf.CNOT(int(a), int(b)))
+    rho = qf.mixed_density(n)
+    rho = qf.RY(0.3, 0).evolve(rho)                  # warm the kernels
+    torch.cuda.synchronize()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for g in gates:
+        rho = g.evolve(rho)
+    gate_end = torch.cuda.Event(enable_timing=True)
+    gate_end.record()
+    for q in range(n):
+        rho = qf.Depolarizing(0.01, q).evolve(rho)
+    end.record()
+    torch.cuda.synchronize()
+    gate_ms = start.elapsed_time(gate_end)
+    chan_ms = gate_end.elapsed_time(end)
+    tr = complex(rho.trace().item())
+    nbytes = 16.0 * 4 ** n
+    return {"workload": "%d-qubit density (%.2f GiB), %d random RX/CNOT gates, then "
+                        "depolarizing on every qubit" % (n, nbytes / 2 ** 30, ngates),
+            "gate_evolutions_per_s": round(ngates / (gate_ms * 1e-3), 1),
+            "ms_per_gate_evolution": round(gate_ms / ngates, 4),
+            "gate_pass_GBs": round(2 * 2 * nbytes * ngates / (gate_ms * 1e-3) / 1e9, 1),
+            "ms_per_channel": round(chan_ms / n, 4),
+            "trace_after": round(tr.real, 12)}
+
+
 def qaoa_section(steps, plain_sample=24, grad_sample=4):
     """Batched QAOA (512 ER graphs, n 10..22, p=4: values; 64 3-regular
     graphs, n 16..24, p=8: values + adjoint gradients) against the plain
@@ -199,6 +244,7 @@ def main():
     ap.add_argument("--cpu-gates", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-qaoa", action="store_true")
+    ap.add_argument("--no-density", action="store_true")
     ap.add_argument("--no-mfma", action="store_true")
     ap.add_argument("--no-unfused", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -299,6 +345,8 @@ def main():
     if not args.no_qaoa:
         del state
         out["qaoa"] = qaoa_section(max(2, args.steps))
+    if not args.no_density:
+        out["density"] = density_section()
     if not args.no_cpu and args.cpu_gates > 0:
         del work
         torch.cuda.empty_cache()
