@@ -671,7 +671,8 @@ def all_bitstrings(n):
 # ---- density matrices -----------------------------------------------------
 # A density matrix is a 2^n x 2^n complex128 array, row = ket index (big-
 # endian over the qubit axes), the flattening of the reference's rank-2
-# QubitVector (quantumflow/states.py:264-344).
+# QubitVector (quantumflow/states.py:264-344).  Pinned in
+# tests/test_oracle_density.py (pure-state consistency, closed forms).
 
 def density_from_state(tensor):
     """|psi><psi| (quantumflow/states.py:162-165)."""
