@@ -112,10 +112,12 @@ def fused_block_section(gates, qubits, work, widths=(3, 4, 5), reps=2):
 def density_section(n=13, ngates=40, seed=5):
     """Density-matrix evolution: an n-qubit density (a 2n-axis tensor,
     16 * 4^n bytes) through a random circuit of 1- and 2-qubit gates, then a
-    depolarizing layer on every qubit.  A gate is two gate-kernel passes
-    over the tensor (U on the row axes, conj U on the column axes); a
-    4-operator channel is four gate evolutions plus four weighted sums.
-    Synthetic data; HIP events around the whole sequence."""
+    depolarizing layer on every qubit.  A gate is one gate-kernel pass over
+    the tensor (U (x) conj U on the row and column axes) or two when no
+    kernel takes that product; a 4-operator channel is four gate evolutions
+    plus four weighted sums.  Synthetic data; HIP events around the whole
+    sequence."""
+    from quantumflow_amd.channels import evolve_passes
     import numpy as np
     import torch
     import quantumflow_amd as qf
@@ -149,7 +151,9 @@ def density_section(n=13, ngates=40, seed=5):
                         "depolarizing on every qubit" % (n, nbytes / 2 ** 30, ngates),
             "gate_evolutions_per_s": round(ngates / (gate_ms * 1e-3), 1),
             "ms_per_gate_evolution": round(gate_ms / ngates, 4),
-            "gate_pass_GBs": round(2 * 2 * nbytes * ngates / (gate_ms * 1e-3) / 1e9, 1),
+            "passes_per_gate": round(sum(evolve_passes(g) for g in gates) / ngates, 3),
+            "gate_pass_GBs": round(2 * nbytes * sum(evolve_passes(g) for g in gates)
+                                   / (gate_ms * 1e-3) / 1e9, 1),
             "ms_per_channel": round(chan_ms / n, 4),
             "trace_after": round(tr.real, 12)}
 

@@ -121,15 +121,38 @@ def state_asdensity(ket: State) -> Density:
     return Density(rho.view([2] * (2 * n)), ket.qubits, ket._memory)
 
 
+def _one_pass_kind(m):
+    """The kernel class of U (x) conj(U) if one gate kernel takes it, else
+    None."""
+    k = int(round(np.log2(m.shape[0])))
+    if 2 * k > 5:
+        return None
+    kind = bk.classify(np.kron(m, np.conj(m)))
+    return kind if 2 * k <= 3 or kind.kind != kind.DENSE else None
+
+
+def evolve_passes(gate: Gate) -> int:
+    """HBM passes over the density that gate_evolve makes for this gate."""
+    return 1 if _one_pass_kind(gate.matrix) is not None else 2
+
+
 def gate_evolve(gate: Gate, rho: Density) -> Density:
     """U rho U^H (ops.py:168-172): U on the row axes of the gate's qubits,
-    conj(U) on their column axes."""
+    conj(U) on their column axes.  When U (x) conj(U), as one 2K-qubit gate
+    on both sets of axes, fits a single gate kernel (dense with 2K <= 3,
+    diagonal or monomial with 2K <= 5: every 1-qubit gate, CNOT, CZ, SWAP,
+    ISWAP, ...) the map is one pass over the density; otherwise two."""
     N = rho.qubit_nb
     idx = [rho.qubits.index(q) for q in gate.qubits]
     k = len(idx)
     m = gate.matrix
+    cols = [N + i for i in idx]
+    kind = _one_pass_kind(m)
+    if kind is not None:
+        t = bk.tensormul(kind, rho.tensor, idx + cols)
+        return Density(t, rho.qubits, rho._memory)
     t = bk.tensormul(m.reshape([2] * (2 * k)), rho.tensor, idx)
-    t = bk.tensormul(np.conj(m).reshape([2] * (2 * k)), t, [N + i for i in idx])
+    t = bk.tensormul(np.conj(m).reshape([2] * (2 * k)), t, cols)
     return Density(t, rho.qubits, rho._memory)
 
 
