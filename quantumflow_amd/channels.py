@@ -173,8 +173,33 @@ class Kraus(Operation):
     def asgate(self) -> Gate:
         raise TypeError('Not possible in general')
 
+    def superoperator(self):
+        """sum_k w_k K_k (x) conj(K_k) on the channel's qubits (rows then
+        columns): the channel's action on vectorised densities, as a
+        4^K x 4^K host matrix (the map of Kraus.aschannel, channels.py:
+        52-62)."""
+        qubits = self.qubits
+        K = len(qubits)
+        S = np.zeros((4 ** K, 4 ** K), dtype=np.complex128)
+        for op, w in zip(self.operators, self.weights):
+            m = _embed(op.matrix, [qubits.index(q) for q in op.qubits], K)
+            S += w * np.kron(m, np.conj(m))
+        return S
+
     def evolve(self, rho: Density) -> Density:
-        """Weighted sum of the operators' evolutions (channels.py:74-80)."""
+        """Weighted sum of the operators' evolutions (channels.py:74-80).
+        When one gate kernel takes the channel's superoperator (1-qubit
+        channels: a dense 2-qubit gate on the row and column axes), that is
+        one pass over the density; otherwise each operator's evolution is
+        accumulated by weighted sums."""
+        qubits = self.qubits
+        if 2 * len(qubits) <= 5:
+            kind = bk.classify(self.superoperator())
+            if 2 * len(qubits) <= 3 or kind.kind != kind.DENSE:
+                N = rho.qubit_nb
+                idx = [rho.qubits.index(q) for q in qubits]
+                t = bk.tensormul(kind, rho.tensor, idx + [N + i for i in idx])
+                return Density(t, rho.qubits, rho._memory)
         out = torch.zeros(4 ** rho.qubit_nb, dtype=torch.complex128, device=bk.DEVICE)
         for op, w in zip(self.operators, self.weights):
             term = bk.as_device(op.evolve(rho).tensor)

@@ -100,6 +100,21 @@ def test_kraus_channels_match_oracle(qf):
     assert np.max(np.abs(got - np.diag([p, 1 - p]))) <= 1e-15
 
 
+def test_two_qubit_kraus_channel_matches_oracle(qf):
+    """A channel on two qubits (its superoperator is a dense 4-qubit gate,
+    which no single kernel takes): operator evolutions summed by weights."""
+    rng = np.random.default_rng(11)
+    n = 5
+    m0 = _rand_density(n, rng)
+    rho = qf.Density(m0.reshape([2] * (2 * n)))
+    chan = qf.Kraus([qf.RX(0.4, 3), qf.CNOT(3, 1), qf.H(1)], [0.5, 0.3, 0.2])
+    out = chan.evolve(rho)
+    ops = [(O.gate_matrix('RX', 0.4), [3], 0.5), (O.gate_matrix('CNOT'), [3, 1], 0.3),
+           (O.gate_matrix('H'), [1], 0.2)]
+    ref = O.kraus_evolve(m0, ops, n)
+    assert np.max(np.abs(_mat(out) - ref)) <= 1e-12
+
+
 def test_density_entry_points_reject_bad_sizes(qf):
     import torch
     from quantumflow_amd import _native

@@ -86,3 +86,17 @@ def test_kraus_completeness_on_the_host():
     assert qf.kraus_iscomplete(qf.Dephasing(0.1, 0))
     assert qf.kraus_iscomplete(qf.Kraus([qf.X(0), qf.CNOT(0, 1)], [0.5, 0.5]))
     assert not qf.kraus_iscomplete(qf.Kraus([qf.X(0), qf.CNOT(0, 1)], [0.5, 0.4]))
+
+
+def test_superoperator_equals_kraus_sum():
+    """Kraus.superoperator applied to vec(rho) (row-major) equals the
+    oracle's sum of K rho K^H, for a channel whose operators act on
+    different qubit subsets."""
+    import quantumflow_amd as qf
+    rng = np.random.default_rng(6)
+    chan = qf.Kraus([qf.RX(0.4, 3), qf.CNOT(3, 1), qf.H(1)], [0.5, 0.3, 0.2])
+    rho = _rand_density(2, rng)                 # qubits (1, 3) in that order
+    got = (chan.superoperator() @ rho.reshape(-1)).reshape(4, 4)
+    ops = [(O.gate_matrix('RX', 0.4), [1], 0.5), (O.gate_matrix('CNOT'), [1, 0], 0.3),
+           (O.gate_matrix('H'), [0], 0.2)]
+    assert np.max(np.abs(got - O.kraus_evolve(rho, ops, 2))) < 1e-14
