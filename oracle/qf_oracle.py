@@ -695,3 +695,17 @@ def kraus_evolve(rho, ops, n):
     for matrix, indices, w in ops:
         out += w * density_evolve(rho, matrix, n, indices)
     return out
+
+
+def measure_density(rho, n, axis, rand):
+    """Measure.evolve (quantumflow/stdops.py:67-82) with the uniform draw
+    ``rand``: outcome 0 when rand < sum |P0 rho P0|^2 (the projected
+    density's norm(), its squared Frobenius norm), the kept projection
+    divided by its trace.  Returns (bit, rho)."""
+    p0 = np.diag([1.0, 0.0])
+    p1 = np.diag([0.0, 1.0])
+    r0 = density_evolve(rho, p0, n, [axis])
+    if rand < float(np.sum(np.abs(r0) ** 2)):
+        return 0, r0 / np.trace(r0)
+    r1 = density_evolve(rho, p1, n, [axis])
+    return 1, r1 / np.trace(r1)

@@ -61,6 +61,22 @@ class Measure(Operation):
             ket = ket.update({self.cbit: bit})
         return ket
 
+    def evolve(self, rho):
+        """Measure a Density (stdops.py:67-82): P0 rho P0 and P1 rho P1 are
+        one gate-kernel pass each.  As in the reference, the outcome 0 is
+        drawn with the value of the projected density's norm() (its squared
+        Frobenius norm), and the kept projection is divided by its trace."""
+        from .gates import P0, P1
+        r0 = P0(self.qubit).evolve(rho)
+        prob_zero = float(np.abs(bk.evaluate(r0.norm())))
+        if np.random.random() < prob_zero:
+            bit, rho = 0, r0.normalize()
+        else:
+            bit, rho = 1, P1(self.qubit).evolve(rho).normalize()
+        if self.cbit is not None:
+            rho = rho.update({self.cbit: bit})
+        return rho
+
 
 class Reset(Operation):
     """Reset qubits (all of the state's when none are given) to |0>
@@ -86,6 +102,10 @@ class Reset(Operation):
             tensor = bk.tensormul(kind, tensor, [ket.qubits.index(q)])
         return State(tensor, ket.qubits, ket.memory).normalize()
 
+    def evolve(self, rho):
+        """Not defined for densities, as in the reference (stdops.py:113-115)."""
+        raise TypeError('Not yet implemented')
+
 
 class Barrier(Operation):
     """No-op that marks a scheduling barrier (stdops.py:129-148)."""
@@ -102,6 +122,9 @@ class Barrier(Operation):
 
     def run(self, ket: State) -> State:
         return ket
+
+    def evolve(self, rho):
+        return rho
 
 
 class If(Operation):
@@ -121,6 +144,12 @@ class If(Operation):
         if ket.memory[self.condition] == self.value:
             ket = self.element.run(ket)
         return ket
+
+    def evolve(self, rho):
+        """stdops.py:165-168."""
+        if rho.memory[self.condition] == self.value:
+            rho = self.element.evolve(rho)
+        return rho
 
 
 class Move(Operation):

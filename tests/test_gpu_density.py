@@ -125,3 +125,25 @@ def test_density_entry_points_reject_bad_sizes(qf):
     assert lib.qfx_density_identity(s, ptr, 0, 1.0) == _native.QFX_ERANGE
     assert lib.qfx_density_trace(s, ptr, ptr, 16) == _native.QFX_ERANGE
     assert lib.qfx_density_outer(s, None, ptr, 1) == _native.QFX_EINVAL
+
+
+def test_measure_reset_barrier_if_on_densities(qf):
+    """Measure.evolve against the oracle with the same uniform draws;
+    Reset.evolve raises TypeError as in the reference; Barrier passes the
+    density through; If evolves on the recorded bit."""
+    rng = np.random.default_rng(13)
+    n = 4
+    m0 = _rand_density(n, rng)
+    for seed in range(6):
+        np.random.seed(seed)
+        rho = qf.Measure(2, 'c').evolve(qf.Density(m0.reshape([2] * (2 * n))))
+        np.random.seed(seed)
+        bit, ref = O.measure_density(m0, n, 2, np.random.random())
+        assert rho.memory['c'] == bit
+        assert np.max(np.abs(_mat(rho) - ref)) <= 1e-12
+    with pytest.raises(TypeError):
+        qf.Reset(0).evolve(rho)
+    assert qf.Barrier(0, 1).evolve(rho) is rho
+    flipped = qf.If(qf.X(0), 'c', rho.memory['c']).evolve(rho)
+    ref = O.density_evolve(_mat(rho), O.gate_matrix('X'), n, [0])
+    assert np.max(np.abs(_mat(flipped) - ref)) <= 1e-12

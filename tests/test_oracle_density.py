@@ -100,3 +100,20 @@ def test_superoperator_equals_kraus_sum():
     ops = [(O.gate_matrix('RX', 0.4), [1], 0.5), (O.gate_matrix('CNOT'), [1, 0], 0.3),
            (O.gate_matrix('H'), [0], 0.2)]
     assert np.max(np.abs(got - O.kraus_evolve(rho, ops, 2))) < 1e-14
+
+
+def test_measure_density_closed_forms():
+    """diag(p, 1-p): P0 rho P0 = diag(p, 0), whose squared Frobenius norm is
+    p^2, so draws below p^2 give |0><0| and draws above give |1><1|; a
+    coherent |+><+| loses its coherence either way."""
+    p = 0.6
+    rho = np.diag([p, 1 - p]).astype(complex)
+    bit, out = O.measure_density(rho, 1, 0, p * p - 1e-9)
+    assert bit == 0 and np.allclose(out, np.diag([1.0, 0.0]))
+    bit, out = O.measure_density(rho, 1, 0, p * p + 1e-9)
+    assert bit == 1 and np.allclose(out, np.diag([0.0, 1.0]))
+    plus = np.full((4, 4), 0.25, dtype=complex)      # |++><++|
+    bit, out = O.measure_density(plus, 2, 1, 0.01)
+    assert bit == 0
+    want = np.zeros((4, 4)); want[np.ix_([0, 2], [0, 2])] = 0.5
+    assert np.allclose(out, want)
