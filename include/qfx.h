@@ -170,6 +170,12 @@ int qfx_density_trace(void* stream, void* out, const void* rho, int n);
 /* y += w x over len complex128 elements; the weighted sum of
  * Kraus.evolve (quantumflow/channels.py:74-80) */
 int qfx_density_axpy(void* stream, void* y, const void* x, double w, int64_t len);
+/* out (device complex128 [2^m x 2^m], m = popcount(keep_mask)) = the
+ * partial trace of rho over the index bits outside keep_mask (bit b <->
+ * axis n-1-b), kept axes in their order; replaces QubitVector.partial_trace
+ * (quantumflow/qubits.py:201-229).  keep_mask != 0, keep_mask < 2^n. */
+int qfx_density_partial_trace(void* stream, void* out, const void* rho, int n,
+                              uint32_t keep_mask);
 
 /* ---- fused circuit execution: LDS-tiled passes --------------------------
  * Replaces the per-gate loop of Circuit.run (quantumflow/circuits.py:87-99)

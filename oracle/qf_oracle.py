@@ -709,3 +709,26 @@ def measure_density(rho, n, axis, rand):
         return 0, r0 / np.trace(r0)
     r1 = density_evolve(rho, p1, n, [axis])
     return 1, r1 / np.trace(r1)
+
+
+def density_partial_trace(rho, n, axes):
+    """Trace out the qubit axes ``axes`` of an n-qubit density, the others in
+    their order (quantumflow/qubits.py:201-229), by an explicit sum over the
+    traced indices."""
+    keep = [a for a in range(n) if a not in axes]
+    t = np.asarray(rho, dtype=CTYPE).reshape([2] * (2 * n))
+    m = len(keep)
+    out = np.zeros((2 ** m, 2 ** m), dtype=CTYPE)
+    for r in itertools.product([0, 1], repeat=m):
+        for c in itertools.product([0, 1], repeat=m):
+            acc = 0j
+            for u in itertools.product([0, 1], repeat=n - m):
+                row = [0] * n
+                col = [0] * n
+                for j, a in enumerate(keep):
+                    row[a], col[a] = r[j], c[j]
+                for j, a in enumerate(axes):
+                    row[a] = col[a] = u[j]
+                acc += t[tuple(row + col)]
+            out[int(''.join(map(str, r)) or '0', 2), int(''.join(map(str, c)) or '0', 2)] = acc
+    return out

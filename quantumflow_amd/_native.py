@@ -72,6 +72,7 @@ SIGNATURES = {
     'qfx_density_diag': (_i, [_vp, _vp, _vp, _i]),
     'qfx_density_trace': (_i, [_vp, _vp, _vp, _i]),
     'qfx_density_axpy': (_i, [_vp, _vp, _vp, ctypes.c_double, _i64]),
+    'qfx_density_partial_trace': (_i, [_vp, _vp, _vp, _i, ctypes.c_uint32]),
     'qfx_cut_table': (_i, [_vp, _vp, _i, _vp, _vp, _i]),
     'qfx_cut_counts': (_i, [_vp, _vp, _i, _vp, _i]),
     'qfx_qaoa_check': (_i, [_vp, _i, _vp, _i, _vp, _vp, _i, _i, _i64, _i64, _i64, _i64, _i64, _i,

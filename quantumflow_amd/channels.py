@@ -82,6 +82,25 @@ class Density(State):
     def asdensity(self) -> 'Density':
         return self
 
+    def partial_trace(self, qubits: Qubits) -> 'Density':
+        """Trace out ``qubits``; the others keep their order (states.py:
+        281-285, qubits.py:201-229).  ValueError for a qubit not held or for
+        tracing out every qubit."""
+        new_qubits = list(self.qubits)
+        for q in qubits:
+            new_qubits.remove(q)
+        if not new_qubits:
+            raise ValueError('Cannot remove all qubits with partial_trace.')
+        n = self.qubit_nb
+        keep = 0
+        for q in new_qubits:
+            keep |= 1 << (n - 1 - self.qubits.index(q))
+        m = len(new_qubits)
+        out = torch.empty(4 ** m, dtype=torch.complex128, device=bk.DEVICE)
+        _native.call('qfx_density_partial_trace', bk.stream_handle(), _ptr(out),
+                     _ptr(self._mat()), n, keep)
+        return Density(out.view([2] * (2 * m)), new_qubits, self._memory)
+
     def relabel(self, qubits: Qubits) -> 'Density':
         return Density(self.tensor, qubits, self._memory)
 

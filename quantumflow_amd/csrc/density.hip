@@ -11,6 +11,9 @@
 //   qfx_density_trace     sum rho_ii        (Density.trace, states.py:277-279)
 //   qfx_density_axpy      y += w x          (the weighted sum of Kraus.evolve,
 //                                            channels.py:74-80)
+//   qfx_density_partial_trace  trace over the qubits outside keep_mask
+//                                           (QubitVector.partial_trace,
+//                                            qubits.py:201-229)
 #include "qfx_common.h"
 
 namespace {
@@ -91,6 +94,41 @@ __global__ __launch_bounds__(kBlock) void axpy_kernel(double2* __restrict__ y,
   }
 }
 
+// out[r', c'] = sum_u rho[r, c], r = deposit(r', keep) | d(u), c =
+// deposit(c', keep) | d(u), d(u) = the u-th index over the traced bits
+// (stepped by one carry-add).  One thread per output element, terms in a
+// fixed order.
+__device__ __forceinline__ uint64_t deposit(uint64_t v, uint64_t mask) {
+  uint64_t r = 0;
+  for (uint64_t bit = 1; mask; bit <<= 1) {
+    const uint64_t low = mask & (0 - mask);
+    if (v & bit) r |= low;
+    mask &= mask - 1;
+  }
+  return r;
+}
+
+__global__ __launch_bounds__(kBlock) void ptrace_kernel(double2* __restrict__ out,
+                                                       const double2* __restrict__ rho, int n,
+                                                       uint32_t keep, int m) {
+  const uint64_t dimk = (uint64_t)1 << m, dim = (uint64_t)1 << n, len = dimk * dimk;
+  const uint64_t tr = (dim - 1) & ~(uint64_t)keep;
+  const uint64_t nt = (uint64_t)1 << (n - m);
+  for (uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x; t < len;
+       t += (uint64_t)gridDim.x * kBlock) {
+    const uint64_t r0 = deposit(t >> m, keep), c0 = deposit(t & (dimk - 1), keep);
+    double re = 0.0, im = 0.0;
+    uint64_t d = 0;
+    for (uint64_t u = 0; u < nt; ++u) {
+      const double2 x = rho[(r0 | d) * dim + (c0 | d)];
+      re += x.x;
+      im += x.y;
+      d = ((d | ~tr) + 1) & tr;
+    }
+    out[t] = double2{re, im};
+  }
+}
+
 int check_n(int n, const char* what) {
   if (n < 1 || n > QFX_DENSITY_MAX_QUBITS) return qfx::set_error(QFX_ERANGE, what);
   return QFX_OK;
@@ -142,6 +180,19 @@ int qfx_density_axpy(void* stream, void* y, const void* x, double w, int64_t len
                      qfx::as_stream(stream), static_cast<double2*>(y),
                      static_cast<const double2*>(x), w, (uint64_t)len);
   return qfx::check_launch("axpy_kernel");
+}
+
+int qfx_density_partial_trace(void* stream, void* out, const void* rho, int n,
+                              uint32_t keep_mask) {
+  if (!out || !rho) return qfx::set_error(QFX_EINVAL, "qfx_density_partial_trace: null pointer");
+  if (int rc = check_n(n, "qfx_density_partial_trace: n out of range")) return rc;
+  if (keep_mask == 0 || (keep_mask >> n) != 0)
+    return qfx::set_error(QFX_EINVAL, "qfx_density_partial_trace: bad keep mask");
+  const int m = __builtin_popcount(keep_mask);
+  hipLaunchKernelGGL(ptrace_kernel, dim3(grid_for((uint64_t)1 << (2 * m))), dim3(kBlock), 0,
+                     qfx::as_stream(stream), static_cast<double2*>(out),
+                     static_cast<const double2*>(rho), n, keep_mask, m);
+  return qfx::check_launch("ptrace_kernel");
 }
 
 }  // extern "C"

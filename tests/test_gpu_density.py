@@ -147,3 +147,18 @@ def test_measure_reset_barrier_if_on_densities(qf):
     flipped = qf.If(qf.X(0), 'c', rho.memory['c']).evolve(rho)
     ref = O.density_evolve(_mat(rho), O.gate_matrix('X'), n, [0])
     assert np.max(np.abs(_mat(flipped) - ref)) <= 1e-12
+
+
+@pytest.mark.parametrize("traced", [[0], [5], [1, 4], [0, 2, 3], [5, 4, 3, 2, 1]])
+def test_partial_trace_matches_oracle(qf, traced):
+    rng = np.random.default_rng(20 + len(traced))
+    n = 6
+    m0 = _rand_density(n, rng)
+    print("partial trace over", traced, flush=True)
+    rho = qf.Density(m0.reshape([2] * (2 * n)), qubits=list('abcdef'))
+    out = rho.partial_trace([rho.qubits[a] for a in traced])
+    assert list(out.qubits) == [q for i, q in enumerate('abcdef') if i not in traced]
+    ref = O.density_partial_trace(m0, n, traced)
+    assert np.max(np.abs(_mat(out) - ref)) <= 1e-12
+    with pytest.raises(ValueError):
+        rho.partial_trace(list('abcdef'))

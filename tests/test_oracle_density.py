@@ -117,3 +117,20 @@ def test_measure_density_closed_forms():
     assert bit == 0
     want = np.zeros((4, 4)); want[np.ix_([0, 2], [0, 2])] = 0.5
     assert np.allclose(out, want)
+
+
+def test_partial_trace_closed_forms():
+    """tr_B(rho_A (x) rho_B) = rho_A for any axes split; the Bell state
+    traces to I/2; the trace of the reduced density is that of the whole."""
+    rng = np.random.default_rng(8)
+    ra, rb = _rand_density(1, rng), _rand_density(2, rng)
+    rho = np.kron(ra, rb)                     # axes 0 | 1, 2
+    assert np.allclose(O.density_partial_trace(rho, 3, [1, 2]), ra, atol=1e-14)
+    assert np.allclose(O.density_partial_trace(rho, 3, [0]), rb, atol=1e-14)
+    bell = np.zeros(4, dtype=complex)
+    bell[0] = bell[3] = 1 / np.sqrt(2)
+    b = O.density_from_state(bell)
+    assert np.allclose(O.density_partial_trace(b, 2, [0]), np.eye(2) / 2, atol=1e-15)
+    assert np.allclose(O.density_partial_trace(b, 2, [1]), np.eye(2) / 2, atol=1e-15)
+    r4 = _rand_density(4, rng)
+    assert abs(np.trace(O.density_partial_trace(r4, 4, [1, 3])) - 1.0) < 1e-13
