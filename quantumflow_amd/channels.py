@@ -23,8 +23,8 @@ from .ops import Gate, Operation
 from .qubits import Qubits, qubits_count_tuple
 from .states import State
 
-__all__ = ['Density', 'mixed_density', 'random_density', 'Kraus', 'UnitaryMixture',
-           'Depolarizing', 'Damping', 'Dephasing', 'kraus_iscomplete']
+__all__ = ['Density', 'mixed_density', 'random_density', 'Channel', 'Kraus',
+           'UnitaryMixture', 'Depolarizing', 'Damping', 'Dephasing', 'kraus_iscomplete']
 
 MAX_QUBITS = 15        # QFX_DENSITY_MAX_QUBITS
 
@@ -175,6 +175,107 @@ def gate_evolve(gate: Gate, rho: Density) -> Density:
     return Density(t, rho.qubits, rho._memory)
 
 
+class Channel(Operation):
+    """A channel as its superoperator (ops.py:265-412): a [2]*4N tensor with
+    axes (rows out, columns out, rows in, columns in), i.e. the 4^N x 4^N
+    matrix that maps a vectorised density (rows then columns).  For a gate
+    it is U (x) conj(U) (Gate.aschannel, ops.py:243-252).  ``evolve`` and
+    ``@`` run it as one 2N-qubit gate on the gate kernels."""
+
+    def __init__(self, tensor, qubits, params=None, name=None) -> None:
+        _, qubits = qubits_count_tuple(qubits)
+        self._qubits = tuple(qubits)
+        N = len(self._qubits)
+        if isinstance(tensor, torch.Tensor):
+            tensor = tensor.detach().cpu().numpy()
+        m = np.asarray(tensor, dtype=np.complex128)
+        if m.size != 16 ** N:
+            raise ValueError('Incompatibility between tensor and qubits')
+        self._mat = m.reshape(4 ** N, 4 ** N)
+        self.params = params
+        self._name = name if name is not None else self.__class__.__name__
+
+    @property
+    def name(self) -> str:
+        return self._name
+
+    @property
+    def qubits(self) -> Qubits:
+        return self._qubits
+
+    @property
+    def qubit_nb(self) -> int:
+        return len(self._qubits)
+
+    @property
+    def tensor(self):
+        """The superoperator as a [2]*4N device tensor (ops.py:285-288)."""
+        return bk.astensorproduct(self._mat.reshape([2] * (4 * self.qubit_nb)))
+
+    def asoperator(self) -> np.ndarray:
+        """The 4^N x 4^N superoperator matrix (host)."""
+        return self._mat
+
+    def relabel(self, qubits: Qubits) -> 'Channel':
+        return Channel(self._mat, qubits, self.params, self._name)
+
+    @property
+    def sharp(self) -> 'Channel':
+        """Swap the two bra super-indices (ops.py:313-335)."""
+        N = self.qubit_nb
+        t = self._mat.reshape([2 ** N] * 4).transpose(0, 2, 1, 3)
+        return Channel(t.reshape(4 ** N, 4 ** N), self.qubits)
+
+    def choi(self) -> np.ndarray:
+        """The Choi matrix (ops.py:337-342)."""
+        return self.sharp._mat.reshape([4 ** self.qubit_nb] * 2)
+
+    def run(self, ket: State) -> State:
+        raise TypeError()  # not possible in general (ops.py:351-352)
+
+    def asgate(self) -> Gate:
+        raise TypeError()  # not possible in general (ops.py:365-366)
+
+    def aschannel(self) -> 'Channel':
+        return self
+
+    def evolve(self, rho: Density) -> Density:
+        """The superoperator on the channel's row and column axes of the
+        density, one gate-kernel pass (ops.py:354-363)."""
+        N = rho.qubit_nb
+        idx = [rho.qubits.index(q) for q in self.qubits]
+        kind = bk.classify(self._mat)
+        t = bk.tensormul(kind, rho.tensor, idx + [N + i for i in idx])
+        return Density(t, rho.qubits, rho._memory)
+
+    def __add__(self, other) -> 'Channel':
+        if isinstance(other, Channel):
+            if not self.qubits == other.qubits:
+                raise ValueError("Qubits must be identical")
+            return Channel(self._mat + other._mat, self.qubits)
+        raise NotImplementedError()
+
+    def __mul__(self, other) -> 'Channel':
+        return Channel(self._mat * other, self.qubits)
+
+    def __matmul__(self, other: 'Channel') -> 'Channel':
+        """This channel after ``other`` (whose qubits hold this one's): this
+        superoperator on the output axes of the other's tensor, as a gate on
+        a 4N-axis state (ops.py:384-398)."""
+        if not isinstance(other, Channel):
+            raise NotImplementedError()
+        N = other.qubit_nb
+        idx = [other.qubits.index(q) for q in self.qubits]
+        t = bk.tensormul(bk.classify(self._mat), other.tensor, idx + [N + i for i in idx])
+        return Channel(t, other.qubits)
+
+
+def gate_aschannel(gate: Gate) -> Channel:
+    """U (x) conj(U) in (row, column) order (ops.py:243-252)."""
+    m = gate.matrix
+    return Channel(np.kron(m, np.conj(m)), gate.qubits)
+
+
 class Kraus(Operation):
     """A Kraus representation of a channel (channels.py:36-105):
     rho -> sum_k w_k K_k rho K_k^H."""
@@ -191,6 +292,10 @@ class Kraus(Operation):
 
     def asgate(self) -> Gate:
         raise TypeError('Not possible in general')
+
+    def aschannel(self) -> Channel:
+        """The channel's superoperator (channels.py:52-62)."""
+        return Channel(self.superoperator(), self.qubits)
 
     def superoperator(self):
         """sum_k w_k K_k (x) conj(K_k) on the channel's qubits (rows then

@@ -164,6 +164,11 @@ class Gate(Operation):
             tensor = bk.tensormul(self.kind, ket.tensor, indices)
         return State(tensor, ket.qubits, ket.memory)
 
+    def aschannel(self):
+        """The gate's superoperator U (x) conj(U) as a Channel (ops.py:243-252)."""
+        from .channels import gate_aschannel
+        return gate_aschannel(self)
+
     def evolve(self, rho):
         """rho -> U rho U^H on a Density (ops.py:168-172): the gate kernels
         on the density's row axes (U) and column axes (conj U)."""

@@ -162,3 +162,29 @@ def test_partial_trace_matches_oracle(qf, traced):
     assert np.max(np.abs(_mat(out) - ref)) <= 1e-12
     with pytest.raises(ValueError):
         rho.partial_trace(list('abcdef'))
+
+
+def test_channel_evolve_and_composition(qf):
+    """Channel.evolve runs the superoperator as one gate on the row and
+    column axes (dense 2-qubit, monomial 4-qubit and, for a 2-qubit Kraus
+    channel, dense 4-qubit with the matrix in device memory); composition
+    by @ equals evolving by each channel in turn."""
+    rng = np.random.default_rng(17)
+    n = 5
+    m0 = _rand_density(n, rng)
+    rho = qf.Density(m0.reshape([2] * (2 * n)))
+    for gate, name, params, qubits in ((qf.RX(0.9, 2), 'RX', (0.9,), [2]),
+                                       (qf.CNOT(4, 0), 'CNOT', (), [4, 0])):
+        print("channel of", name, flush=True)
+        out = gate.aschannel().evolve(rho)
+        ref = O.density_evolve(m0, O.gate_matrix(name, *params), n, qubits)
+        assert np.max(np.abs(_mat(out) - ref)) <= 1e-12
+    chan = qf.Kraus([qf.RX(0.4, 3), qf.CNOT(3, 1), qf.H(1)], [0.5, 0.3, 0.2])
+    out = chan.aschannel().evolve(rho)
+    ops = [(O.gate_matrix('RX', 0.4), [3], 0.5), (O.gate_matrix('CNOT'), [3, 1], 0.3),
+           (O.gate_matrix('H'), [1], 0.2)]
+    assert np.max(np.abs(_mat(out) - O.kraus_evolve(m0, ops, n))) <= 1e-12
+    a, b = qf.Depolarizing(0.1, 1).aschannel(), qf.Damping(0.2, 1).aschannel()
+    both = (a @ b).evolve(rho)
+    step = qf.Depolarizing(0.1, 1).evolve(qf.Damping(0.2, 1).evolve(rho))
+    assert np.max(np.abs(_mat(both) - _mat(step))) <= 1e-12

@@ -134,3 +134,26 @@ def test_partial_trace_closed_forms():
     assert np.allclose(O.density_partial_trace(b, 2, [1]), np.eye(2) / 2, atol=1e-15)
     r4 = _rand_density(4, rng)
     assert abs(np.trace(O.density_partial_trace(r4, 4, [1, 3])) - 1.0) < 1e-13
+
+
+def test_channel_superoperator_host_forms():
+    """Gate.aschannel is U (x) conj(U); its Choi matrix is rank one with
+    trace 2^N (a unitary channel); sharp is an involution; the channel of a
+    Kraus set applied to vec(rho) equals the oracle's Kraus sum."""
+    import quantumflow_amd as qf
+    rng = np.random.default_rng(10)
+    g = qf.CNOT(0, 1)
+    ch = g.aschannel()
+    U = O.gate_matrix('CNOT')
+    assert np.allclose(ch.asoperator(), np.kron(U, U.conj()))
+    choi = ch.choi()
+    assert np.linalg.matrix_rank(choi, tol=1e-10) == 1
+    assert abs(np.trace(choi) - 4.0) < 1e-12
+    assert np.allclose(ch.sharp.sharp.asoperator(), ch.asoperator())
+    damp = qf.Damping(0.3, 0).aschannel()
+    rho = _rand_density(1, rng)
+    ops = [(np.array([[1, 0], [0, np.sqrt(0.7)]]), [0], 1.0),
+           (np.array([[0, np.sqrt(0.3)], [0, 0]]), [0], 1.0)]
+    got = (damp.asoperator() @ rho.reshape(-1)).reshape(2, 2)
+    assert np.allclose(got, O.kraus_evolve(rho, ops, 1))
+    assert np.allclose((damp + damp * 0.5).asoperator(), 1.5 * damp.asoperator())
