@@ -24,7 +24,8 @@ from .qubits import Qubits, qubits_count_tuple
 from .states import State
 
 __all__ = ['Density', 'mixed_density', 'random_density', 'Channel', 'Kraus',
-           'UnitaryMixture', 'Depolarizing', 'Damping', 'Dephasing', 'kraus_iscomplete']
+           'UnitaryMixture', 'Depolarizing', 'Damping', 'Dephasing', 'kraus_iscomplete',
+           'join_channels', 'channel_to_kraus']
 
 MAX_QUBITS = 15        # QFX_DENSITY_MAX_QUBITS
 
@@ -409,3 +410,39 @@ def kraus_iscomplete(kraus: Kraus) -> bool:
         kk = op.matrix.conj().T @ op.matrix
         total += w * _embed(kk, [qubits.index(q) for q in op.qubits], N)
     return bool(np.allclose(total, np.eye(2 ** N)))
+
+
+def join_channels(*channels: Channel) -> Channel:
+    """One channel on the union of the channels' (disjoint) qubits, in the
+    given order (channels.py:170-176): the tensor product of the
+    superoperators with the row and column super-indices regrouped."""
+    mats = [c.asoperator() for c in channels]
+    ns = [c.qubit_nb for c in channels]
+    qubits = [q for c in channels for q in c.qubits]
+    S = np.ones((1, 1), dtype=np.complex128)
+    N = 0
+    for m, k in zip(mats, ns):
+        # S: (ro, co, ri, ci) over N qubits; m: the same over k qubits
+        a = S.reshape([2 ** N] * 4)
+        b = m.reshape([2 ** k] * 4)
+        t = np.einsum('abcd,efgh->aebfcgdh', a, b)
+        N += k
+        S = t.reshape(4 ** N, 4 ** N)
+    return Channel(S, qubits)
+
+
+def channel_to_kraus(chan: Channel) -> Kraus:
+    """Kraus operators of a channel from the eigenvectors of its Choi matrix
+    (channels.py:179-205): K_i = sqrt(l_i) * eigenvector i as a 2^N x 2^N
+    matrix, for the non-zero eigenvalues l_i (a CP map has them >= 0)."""
+    N = chan.qubit_nb
+    choi = chan.choi()
+    evals, evecs = np.linalg.eigh((choi + choi.conj().T) / 2)
+    if np.any(evals < -1e-10):
+        raise ValueError('channel is not completely positive')
+    ops = []
+    for i in range(len(evals)):
+        if not np.isclose(evals[i], 0.0):
+            mat = evecs[:, i].reshape(2 ** N, 2 ** N) * np.sqrt(evals[i])
+            ops.append(Gate(mat, qubits=chan.qubits))
+    return Kraus(ops)
