@@ -188,3 +188,20 @@ def test_channel_evolve_and_composition(qf):
     both = (a @ b).evolve(rho)
     step = qf.Depolarizing(0.1, 1).evolve(qf.Damping(0.2, 1).evolve(rho))
     assert np.max(np.abs(_mat(both) - _mat(step))) <= 1e-12
+
+
+def test_kraus_run_on_states(qf):
+    """Kraus.run draws a branch with probability w_k |K_k psi|^2 and
+    renormalises: full damping sends |1> to |0> on every draw, and a
+    depolarizing channel with p = 0 leaves the state unchanged."""
+    one = qf.State(np.array([0.0, 1.0]))
+    for seed in range(3):
+        np.random.seed(seed)
+        out = qf.Damping(1.0, 0).run(one)
+        assert np.allclose(out.tensor.cpu().numpy(), [1.0, 0.0], atol=1e-15)
+    rng = np.random.default_rng(2)
+    psi = rng.normal(size=[2] * 3) + 1j * rng.normal(size=[2] * 3)
+    psi /= np.linalg.norm(psi)
+    ket = qf.State(psi)
+    out = qf.Depolarizing(0.0, 1).run(ket)
+    assert np.max(np.abs(out.tensor.cpu().numpy() - psi)) <= 1e-15
