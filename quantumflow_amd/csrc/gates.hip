@@ -173,7 +173,7 @@ int launch_dense(hipStream_t s, double2* out, const double2* in, int n, const in
 
 template <int K>
 int launch_sparse_k(hipStream_t s, double2* out, const double2* in, int n, const SparseArg& a) {
-  constexpr int G = K == 1 ? 4 : (K == 2 ? 2 : 1);
+  constexpr int G = K == 1 ? 4 : ((K == 2 || K == 4) ? 2 : 1);
   const uint64_t ngroups = (uint64_t)1 << (n - K);
   const uint64_t blocks = (ngroups + kBlock * G - 1) / (kBlock * G);
   hipLaunchKernelGGL((sparse_kernel<K, G>), dim3((unsigned)blocks), dim3(kBlock), 0, s, out, in,
