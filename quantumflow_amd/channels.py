@@ -1,15 +1,17 @@
 """Density matrices and Kraus channels on the GPU.
 
-Mirrors the reference's Density (quantumflow/states.py:264-344) and
-channels (quantumflow/channels.py:36-165).  A Density of N qubits holds its
+Mirrors the reference's Density (quantumflow/states.py:264-344), Channel
+(quantumflow/ops.py:265-412) and channels (quantumflow/channels.py:36-217).  A Density of N qubits holds its
 matrix as a [2]*2N complex128 device tensor: axes 0..N-1 are the row (ket)
 indices, N..2N-1 the column (bra) indices.  A K-qubit operator U acts as
-rho -> U rho U^H, which is U applied to the row axes and conj(U) to the
-column axes: two calls of the state-vector gate kernels on the 2N-axis
-tensor.  The reference builds a superoperator Channel for the same map
-(Gate.evolve, quantumflow/ops.py:168-172 -> Gate.aschannel, ops.py:243-252);
-the result is the same matrix.  Outer products, identity, diagonal, trace
-and the weighted sums of Kraus terms run in csrc/density.hip.
+rho -> U rho U^H, which is U on the row axes and conj(U) on the column
+axes.  Where a gate kernel takes U (x) conj(U) as one 2K-qubit gate on both
+sets of axes (the superoperator; Gate.aschannel, quantumflow/ops.py:
+243-252), that is one pass of the state-vector gate kernels over the
+2N-axis tensor; otherwise U and conj(U) are two passes.  1-qubit Kraus
+channels and Channel objects run their superoperator the same way.  Outer
+products, identity, diagonal, trace, partial trace and the weighted sums
+of wider Kraus channels run in csrc/density.hip.
 """
 
 from typing import Sequence, Union
