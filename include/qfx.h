@@ -351,7 +351,11 @@ typedef struct {
   const void* val_pool;   /* float64 cut values */
   void* partials;         /* float64 [nslots][width] */
   void* out;              /* float64 [ninst][width] */
-  void* sync;             /* >= 8 bytes of scratch: barrier counter + error flag */
+  void* sync;             /* >= 12 bytes: int32 barrier counter, this launch's
+                             error flag, sticky error flag.  The owner zeroes
+                             all three before the first launch; a launch
+                             resets the first two and leaves the third, which
+                             any grid-barrier timeout sets. */
   int32_t ninst, nk, width, mode, block, max_items_per_k;
   int32_t tile_bits;      /* largest pass tile (LDS per workgroup: 2^tile_bits
                              amplitudes, twice that backward); 0: 13 forward,
@@ -374,7 +378,9 @@ int qfx_qaoa_check(const qfx_qaoa_inst* insts, int ninst,
  * into out.  Capturable into a HIP graph. */
 int qfx_qaoa_run(void* stream, const qfx_qaoa_args* args);
 
-/* After the stream completes: *failed = 1 if a grid barrier timed out. */
+/* After the stream completes: *failed = 1 if a grid barrier timed out in
+ * the last launch or in any earlier launch since the owner zeroed the
+ * sticky flag.  A barrier times out after 10 s of device wall-clock time. */
 int qfx_qaoa_status(const void* sync, int* failed);
 
 /* ---- MLP angle proposer kernels (fp32) ----------------------------------

@@ -417,15 +417,26 @@ def graph_cuts_loop(n, edges, weights=None):
 def graph_cuts(n, edges, weights=None):
     """Vectorised quantumflow/qaoa.py:68-81: C(z) = sum over edges, in edge
     order, of w_e [z_i != z_j], node i <-> axis i (bit n-1-i of the flat
-    index).  Summation order per index equals the reference loop's, so the
-    table is bitwise identical to ``graph_cuts_loop``."""
+    index).  Summation order per index equals the reference loop's (adding
+    w * 0 leaves a partial sum unchanged), so the table is bitwise identical
+    to ``graph_cuts_loop``."""
     idx = np.arange(2 ** n, dtype=np.int64)
-    diag = np.zeros(2 ** n, dtype=np.double)
-    for e, (q0, q1) in enumerate(edges):
-        w = 1 if weights is None else weights[e]
-        b0 = (idx >> (n - 1 - q0)) & 1
-        b1 = (idx >> (n - 1 - q1)) & 1
-        diag += np.where(b0 != b1, w, 0)
+    bits = {}
+
+    def bit(q):
+        if q not in bits:
+            bits[q] = ((idx >> (n - 1 - q)) & 1).astype(np.uint8)
+        return bits[q]
+
+    if weights is None:
+        count = np.zeros(2 ** n, dtype=np.uint16)
+        for (q0, q1) in edges:
+            count += bit(q0) ^ bit(q1)
+        diag = count.astype(np.double)
+    else:
+        diag = np.zeros(2 ** n, dtype=np.double)
+        for e, (q0, q1) in enumerate(edges):
+            diag += weights[e] * (bit(q0) ^ bit(q1))
     return diag.reshape([2] * n)
 
 
@@ -449,14 +460,16 @@ def qaoa_expectation(n, edges, beta, gamma, weights=None):
     return expectation(ket, graph_cuts(n, edges, weights))
 
 
-def qaoa_state_fast(n, edges, beta, gamma, weights=None):
+def qaoa_state_fast(n, edges, beta, gamma, weights=None, cuts=None):
     """Same state as ``qaoa_state`` computed with the cost layer as one
     diagonal: prod_e ZZ(-w gamma/pi) = exp(i gamma W/2) exp(-i gamma C(z)),
     W = sum of weights (from the ZZ matrix at stdgates.py:944-968).  Mixer
     and H are applied qubit by qubit with tensormul.  Agrees with
     ``qaoa_state`` to rounding; used for sizes where the gate-by-gate
-    oracle is slow."""
-    cuts = graph_cuts(n, edges, weights).reshape(-1)
+    oracle is slow.  ``cuts``: the graph_cuts table, if already built."""
+    if cuts is None:
+        cuts = graph_cuts(n, edges, weights)
+    cuts = cuts.reshape(-1)
     wsum = float(len(edges) if weights is None else np.sum(weights))
     ket = np.full(2 ** n, 1.0 / np.sqrt(2.0 ** n), dtype=CTYPE)
     ket = ket.reshape([2] * n)
@@ -469,9 +482,32 @@ def qaoa_state_fast(n, edges, beta, gamma, weights=None):
     return ket
 
 
-def qaoa_expectation_fast(n, edges, beta, gamma, weights=None):
-    ket = qaoa_state_fast(n, edges, beta, gamma, weights)
-    return expectation(ket, graph_cuts(n, edges, weights))
+def qaoa_expectation_fast(n, edges, beta, gamma, weights=None, cuts=None):
+    if cuts is None:
+        cuts = graph_cuts(n, edges, weights)
+    ket = qaoa_state_fast(n, edges, beta, gamma, weights, cuts=cuts)
+    return expectation(ket, cuts)
+
+
+def qaoa_value_torch(n, edges, beta, gamma, device='cpu'):
+    """<C> as a torch complex128 computation that autograd can differentiate
+    (the stored-intermediates gradient the reference gets from TensorFlow
+    autograd, examples/qaoa_maxcut.py:53-64).  beta, gamma: float64 tensors
+    (requires_grad as needed); cost layer as in ``qaoa_state_fast``, RX on
+    every qubit by tensordot.  Unit weights."""
+    import torch
+    cuts = torch.from_numpy(graph_cuts(n, edges).reshape(-1)).to(device)
+    wsum = float(len(edges))
+    psi = torch.full((2 ** n,), 2.0 ** (-n / 2), dtype=torch.complex128, device=device)
+    for l in range(len(beta)):
+        psi = psi * torch.exp(1j * gamma[l] * (wsum / 2 - cuts))
+        c, s = torch.cos(beta[l] / 2), torch.sin(beta[l] / 2)
+        m = torch.stack([torch.stack([c + 0j, -1j * s]), torch.stack([-1j * s, c + 0j])])
+        t = psi.reshape([2] * n)
+        for q in range(n):
+            t = torch.movedim(torch.tensordot(m, t, dims=([1], [q])), 0, q)
+        psi = t.reshape(-1)
+    return torch.sum(torch.abs(psi) ** 2 * cuts)
 
 
 def qaoa_grad_fd(n, edges, beta, gamma, h=1e-6, order=2, fast=True,

@@ -140,20 +140,60 @@ def fcast(value: float) -> TensorLike:
     return value
 
 
+# Host arrays of gate size (2^k x 2^k, k <= 5) are classified when they are
+# uploaded, and the class rides on the device tensor (with the tensor's
+# version counter, so an in-place change forgets it): the reference's
+# Gate.run passes such a device tensor to tensormul for every application
+# (quantumflow/ops.py:161-166), which then needs no device-to-host copy.
+_KIND_MAX_ELEMS = 4 ** 5
+
+
+def _remember_kind(t, kind):
+    t._qfx_kind = (t._version, kind)
+    return t
+
+
+def _known_kind(t):
+    rec = getattr(t, '_qfx_kind', None)
+    if rec is not None and rec[0] == t._version:
+        return rec[1]
+    return None
+
+
+def _gate_kind_of_host(arr):
+    size = arr.size
+    if 4 <= size <= _KIND_MAX_ELEMS and size & (size - 1) == 0 and \
+            int(math.log2(size)) % 2 == 0:
+        dim = int(round(math.sqrt(size)))
+        return GateKind(arr.reshape(dim, dim))
+    return None
+
+
 def astensor(array: TensorLike) -> BKTensor:
     """Convert to a complex128 device tensor (shape preserved)."""
     _ensure_device()
     if isinstance(array, torch.Tensor):
-        return array.to(device=DEVICE, dtype=CTYPE)
-    arr = np.asarray(array, dtype=_NP_CTYPE)
-    return torch.from_numpy(np.ascontiguousarray(arr)).to(DEVICE)
+        t = array.to(device=DEVICE, dtype=CTYPE)
+        if t is not array or array.device.type != 'cuda':
+            kind = _known_kind(array) if array.device.type == 'cuda' else None
+            if kind is None and array.device.type != 'cuda':
+                kind = _gate_kind_of_host(array.detach().numpy().astype(_NP_CTYPE))
+            if kind is not None:
+                _remember_kind(t, kind)
+        return t
+    arr = np.ascontiguousarray(np.asarray(array, dtype=_NP_CTYPE))
+    t = torch.from_numpy(arr).to(DEVICE)
+    kind = _gate_kind_of_host(arr)
+    return _remember_kind(t, kind) if kind is not None else t
 
 
 def astensorproduct(array: TensorLike) -> BKTensor:
     """astensor, reshaped to [2]*N (numpybk.py:105-112)."""
     tensor = astensor(array)
     n = int(math.log2(tensor.numel()))
-    return tensor.reshape([2] * n)
+    out = tensor.reshape([2] * n)
+    kind = _known_kind(tensor)
+    return _remember_kind(out, kind) if kind is not None and out is not tensor else out
 
 
 def evaluate(tensor: BKTensor) -> TensorLike:
@@ -348,9 +388,12 @@ def tensormul(tensor0: BKTensor, tensor1: BKTensor,
               indices: typing.List[int]) -> BKTensor:
     """Contract gate ``tensor0`` (rank 2K, [ket_out..., bra_in...]) into
     ``tensor1`` (rank N) on axes ``indices``; returns a new [2]*N device
-    tensor (numpybk.py:159-214).  ``tensor0`` may be a host array or a
-    GateKind (no device round trip) or a device tensor (copied to the host:
-    a synchronising call)."""
+    tensor (numpybk.py:159-214).  ``tensor0`` may be a GateKind, a host
+    array (classified on the host) or a device tensor.  A device tensor made
+    by ``astensor``/``astensorproduct`` from a host array carries its class;
+    any other device tensor is applied as a dense matrix read from device
+    memory.  Neither copies anything to the host."""
+    dev_matrix = None
     if isinstance(tensor0, GateKind):
         kind = tensor0
         k = kind.k
@@ -360,13 +403,24 @@ def tensormul(tensor0: BKTensor, tensor1: BKTensor,
     n = rank(tensor1)
     assert k == len(indices)
     if kind is None:
-        kind = classify(_host(tensor0).reshape(2 ** k, 2 ** k))
+        if isinstance(tensor0, torch.Tensor) and tensor0.device.type == 'cuda':
+            kind = _known_kind(tensor0)
+            if kind is None:
+                if k > 12:
+                    raise ValueError('tensormul: gates of more than 12 qubits are not supported')
+                dev_matrix = as_device(tensor0)
+        else:
+            kind = classify(_host(tensor0).reshape(2 ** k, 2 ** k))
     state = as_device(tensor1)
     if n != int(math.log2(state.numel())):
         raise ValueError('tensormul: state is not a product tensor')
     out = torch.empty_like(state)
     bits = [n - 1 - int(i) for i in indices]
-    apply_gate(out, state, n, bits, kind)
+    if dev_matrix is not None:
+        _native.call('qfx_apply_matrix_dev', stream_handle(), _ptr(out), _ptr(state), n, k,
+                     _native.int_array(bits), _ptr(dev_matrix))
+    else:
+        apply_gate(out, state, n, bits, kind)
     return out.view([2] * n)
 
 

@@ -51,7 +51,12 @@ constexpr int kMaxT = 13;
 constexpr int kMaxPLayers = QFX_QAOA_MAX_LAYERS;
 constexpr int kAccum = 1 + 2 * kMaxPLayers;  // value, d/dbeta, d/dgamma
 constexpr int kMaxEdgesTab = 512;  // cut counts 0..kMaxEdgesTab-1 tabulated
-constexpr unsigned kSpinLimit = 1u << 22;
+// A grid barrier gives up after kBarrierSeconds of wall-clock time.  One
+// phase of the largest program qfx_qaoa_check accepts (n <= 30: a 16 GiB
+// state read and written once) streams in well under 0.1 s, so a timeout
+// means workgroups that never arrive, not a slow phase.
+constexpr int kBarrierSeconds = 10;
+constexpr int kMaxDevices = 64;
 
 __device__ __forceinline__ unsigned swz(unsigned e) {
   return e ^ (((e >> 3) ^ (e >> 6) ^ (e >> 9) ^ (e >> 12)) & 7u);
@@ -566,20 +571,25 @@ __device__ void run_item(const qfx_qaoa_inst& in, const qfx_qaoa_pass& ps, uint3
   __syncthreads();
 }
 
-// Grid barrier: monotone counter, bounded spin; on timeout sets *err and
-// returns false (every wave then leaves the kernel).
-__device__ bool grid_barrier(unsigned* counter, unsigned target, int* err) {
+// Grid barrier: monotone counter, wait bounded by the device's constant-rate
+// wall clock (limit_ticks); on timeout it sets this launch's error flag
+// (err[0], which makes every waiting workgroup leave too) and the sticky flag
+// (err[1], cleared only by the batch's owner), and returns false.
+__device__ bool grid_barrier(unsigned* counter, unsigned target, int* err,
+                             uint64_t limit_ticks) {
   __syncthreads();
   __shared__ int s_ok;
   if (threadIdx.x == 0) {
     __threadfence();
     atomicAdd(counter, 1u);
-    unsigned spins = 0;
+    const uint64_t t0 = wall_clock64();
     int ok = 1;
     while (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
       __builtin_amdgcn_s_sleep(8);
-      if (++spins > kSpinLimit || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+      if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
+          wall_clock64() - t0 > limit_ticks) {
         atomicExch(err, 1);
+        atomicOr(err + 1, 1);
         ok = 0;
         break;
       }
@@ -597,7 +607,7 @@ __global__ __launch_bounds__(BS) void qaoa_kernel(
     const qfx_qaoa_item* __restrict__ items, const int* __restrict__ kstart, int nk,
     const double* __restrict__ angles, v2d* __restrict__ psi_pool, v2d* __restrict__ lam_pool,
     const uint16_t* __restrict__ cnt_pool, const double* __restrict__ val_pool,
-    double* __restrict__ partials, int width, unsigned* sync, int* err) {
+    double* __restrict__ partials, int width, unsigned* sync, int* err, uint64_t limit_ticks) {
   extern __shared__ v2d lds[];
   __shared__ double red[16];
   __shared__ double accum[kAccum];
@@ -610,7 +620,8 @@ __global__ __launch_bounds__(BS) void qaoa_kernel(
                         lam_pool, cnt_pool, val_pool,
                         partials + (uint64_t)item.partial * width, width, lds, red, accum, tab);
     }
-    if (k + 1 < nk && !grid_barrier(sync, (unsigned)(k + 1) * gridDim.x, err)) return;
+    if (k + 1 < nk && !grid_barrier(sync, (unsigned)(k + 1) * gridDim.x, err, limit_ticks))
+      return;
   }
 }
 
@@ -642,37 +653,55 @@ __global__ void zero_sync_kernel(unsigned* sync, int* err) {
   }
 }
 
+// Persistent grid of one instantiation on one device: every workgroup that
+// fits the device at once (occupancy x CUs for this block size, mode and LDS
+// size), so all of them are resident together and the grid barrier cannot
+// wait on a workgroup that has not started.  Cached per device and LDS size.
+struct GridCache {
+  int grid = 0;
+  size_t lds = 0;
+  uint64_t limit_ticks = 0;
+};
+
 template <int BS, bool BWD>
 int launch_qaoa(hipStream_t s, const qfx_qaoa_args& a, int T) {
   const size_t lds = (sizeof(v2d) << T) * (BWD ? 2 : 1);
-  static int grid = 0;
-  static size_t grid_lds = 0;
-  if (grid == 0 || grid_lds != lds) {
-    int rc = qfx::check_hip(hipFuncSetAttribute((const void*)qaoa_kernel<BS, BWD>,
-                                                hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                (int)lds),
-                            "hipFuncSetAttribute");
+  static GridCache cache[kMaxDevices];
+  int dev = 0;
+  int rc = qfx::check_hip(hipGetDevice(&dev), "hipGetDevice");
+  if (rc) return rc;
+  if (dev < 0 || dev >= kMaxDevices)
+    return qfx::set_error(QFX_ERANGE, "qfx_qaoa_run: device index out of range");
+  GridCache& gc = cache[dev];
+  if (gc.grid == 0 || gc.lds != lds) {
+    rc = qfx::check_hip(hipFuncSetAttribute((const void*)qaoa_kernel<BS, BWD>,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
+                        "hipFuncSetAttribute");
     if (rc) return rc;
-    int per_cu = 0, dev = 0, cus = 0;
+    int per_cu = 0, cus = 0, rate_khz = 0;
     rc = qfx::check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(
                             &per_cu, (const void*)qaoa_kernel<BS, BWD>, BS, lds),
                         "hipOccupancyMaxActiveBlocksPerMultiprocessor");
     if (rc) return rc;
     if (per_cu < 1) return qfx::set_error(QFX_ERANGE, "qfx_qaoa_run: kernel does not fit a CU");
-    rc = qfx::check_hip(hipGetDevice(&dev), "hipGetDevice");
-    if (rc) return rc;
     rc = qfx::check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev),
                         "hipDeviceGetAttribute");
     if (rc) return rc;
-    grid = per_cu * cus;
-    grid_lds = lds;
+    rc = qfx::check_hip(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, dev),
+                        "hipDeviceGetAttribute");
+    if (rc) return rc;
+    if (rate_khz <= 0) return qfx::set_error(QFX_EKERNEL, "qfx_qaoa_run: no wall-clock rate");
+    gc.grid = per_cu * cus;
+    gc.lds = lds;
+    gc.limit_ticks = (uint64_t)rate_khz * 1000u * kBarrierSeconds;
   }
+  const int grid = gc.grid;
   int g = grid;
   if (a.max_items_per_k > 0 && a.max_items_per_k < g) g = a.max_items_per_k;
   auto* sync = static_cast<unsigned*>(a.sync);
   auto* err = reinterpret_cast<int*>(sync + 1);
   hipLaunchKernelGGL(zero_sync_kernel, dim3(1), dim3(64), 0, s, sync, err);
-  int rc = qfx::check_launch("zero_sync_kernel");
+  rc = qfx::check_launch("zero_sync_kernel");
   if (rc) return rc;
   hipLaunchKernelGGL((qaoa_kernel<BS, BWD>), dim3(g), dim3(BS), lds, s,
                      static_cast<const qfx_qaoa_inst*>(a.insts),
@@ -681,7 +710,7 @@ int launch_qaoa(hipStream_t s, const qfx_qaoa_args& a, int T) {
                      a.nk, static_cast<const double*>(a.angles), static_cast<v2d*>(a.psi_pool),
                      static_cast<v2d*>(a.lam_pool), static_cast<const uint16_t*>(a.cnt_pool),
                      static_cast<const double*>(a.val_pool), static_cast<double*>(a.partials),
-                     a.width, sync, err);
+                     a.width, sync, err, gc.limit_ticks);
   rc = qfx::check_launch("qaoa_kernel");
   if (rc) return rc;
   const int nout = a.ninst * a.width;
@@ -808,12 +837,12 @@ int qfx_qaoa_run(void* stream, const qfx_qaoa_args* args) {
 
 int qfx_qaoa_status(const void* sync, int* failed) {
   if (!sync || !failed) return qfx::set_error(QFX_EINVAL, "qfx_qaoa_status: null pointer");
-  int e = 0;
-  int rc = qfx::check_hip(hipMemcpy(&e, static_cast<const unsigned*>(sync) + 1, sizeof(int),
+  int e[2] = {0, 0};
+  int rc = qfx::check_hip(hipMemcpy(e, static_cast<const unsigned*>(sync) + 1, sizeof(e),
                                     hipMemcpyDeviceToHost),
                           "hipMemcpy");
   if (rc) return rc;
-  *failed = e;
+  *failed = (e[0] | e[1]) != 0;
   return QFX_OK;
 }
 

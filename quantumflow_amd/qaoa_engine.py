@@ -409,12 +409,13 @@ class QAOABatch:
         self._graph.replay()
 
     def check(self):
-        """Synchronise and raise if a grid barrier timed out."""
+        """Synchronise and raise if a grid barrier timed out in any launch
+        since the batch was built (the kernel's error word is sticky: a
+        later launch does not clear it)."""
         failed = ctypes.c_int(0)
         torch.cuda.synchronize()
         _native.call('qfx_qaoa_status', ctypes.c_void_p(self.sync.data_ptr()), ctypes.byref(failed))
-        if failed.value:
-            raise RuntimeError('qaoa engine: grid barrier timed out')
+        raise_if_failed(failed.value)
 
     def evaluate(self, beta, gamma, graph=False):
         """(values [ninst], grads [ninst, 2p] or None) as device tensors."""
@@ -429,6 +430,13 @@ class QAOABatch:
         return out[:, 0], out[:, 1:]
 
 
+def raise_if_failed(flag):
+    """The engine's status word: nonzero means a grid barrier gave up, and
+    the values and gradients of that launch are partial."""
+    if flag:
+        raise RuntimeError('qaoa engine: grid barrier timed out; results are partial')
+
+
 class QAOAExpectation(torch.autograd.Function):
     """<C>(beta, gamma) for every instance of a QAOABatch; the backward pass
     uses the engine's adjoint gradients (computed in the same launch)."""
@@ -436,6 +444,7 @@ class QAOAExpectation(torch.autograd.Function):
     @staticmethod
     def forward(ctx, beta, gamma, batch):
         val, grad = batch.evaluate(beta.detach(), gamma.detach())
+        batch.check()
         ctx.p = batch.p
         ctx.save_for_backward(grad.clone() if grad is not None else val.new_zeros(0))
         return val.clone().to(beta.device)
@@ -490,6 +499,7 @@ def optimize(batch, beta0, gamma0, learning_rate=0.01, max_steps=10000, block=10
             gamma = gamma + upd[:, batch.p:]
             steps += active.to(torch.int64)
         val, _ = batch.evaluate(beta, gamma, graph=use_graph)
+        batch.check()           # every launch of the block, before its values are used
         ratio = torch.where(active, val / maxcut, ratio)
         done = (ratio - last) < min_gain
         active = active & ~done

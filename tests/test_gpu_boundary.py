@@ -1,0 +1,117 @@
+"""The drop-in boundary driven exactly as the reference drives its backend.
+
+QuantumFlow's Gate holds its matrix as a backend tensor made by
+``bk.astensorproduct`` (quantumflow/ops.py:101-123 -> qubits.py:97-121),
+``Gate.run`` passes that tensor and the state's to ``bk.tensormul``
+(ops.py:161-166), and the readouts are ``bk.absolute`` squared
+(states.py:113-119), ``bk.sum(bk.real(bk.astensorproduct(diag)) * probs)``
+(states.py:131-147) and ``bk.inner`` (qubits.py:180-182).  These tests make
+those calls on this backend, check them against the oracle, and check that
+none of them synchronises with the host (torch's sync debug mode raises on
+a device-to-host copy).
+"""
+import numpy as np
+import pytest
+
+from oracle import qf_oracle as O
+
+# quantumflow/backend/__init__.py:25-35 (asserted by tests/test_backend.py:21-58)
+REFERENCE_ALL = [
+    'BKTensor', 'CTYPE', 'DEVICE', 'FTYPE', 'MAX_QUBITS', 'TENSOR',
+    'TL', 'TensorLike', 'absolute', 'arccos', 'astensor',
+    'ccast', 'cis', 'conj', 'cos', 'diag', 'evaluate', 'exp', 'fcast',
+    'gpu_available', 'imag', 'inner', 'minimum',
+    'outer', 'matmul',
+    'rank', 'real', 'reshape', 'set_random_seed', 'sin',
+    'sqrt', 'sum', 'tensormul', 'trace', 'transpose',
+    'getitem', 'astensorproduct', 'productdiag',
+    'EINSUM_SUBSCRIPTS', 'einsum',
+    '__version__', '__name__']
+
+
+def test_backend_exports_reference_names():
+    import quantumflow_amd.backend as bk
+    missing = [nm for nm in REFERENCE_ALL if not hasattr(bk, nm)]
+    assert not missing, missing
+    for nm in REFERENCE_ALL:
+        if not nm.startswith('__'):
+            assert nm in bk.__all__, nm
+    assert bk.BACKEND == 'rocm'
+
+
+@pytest.fixture(scope="module")
+def bk():
+    import quantumflow_amd.backend as bk
+    assert bk.gpu_available()
+    return bk
+
+
+def _gates(n, seed):
+    from quantumflow_amd.workloads import random_mixed_circuit
+    return random_mixed_circuit(n, 6, seed=seed)
+
+
+@pytest.mark.gpu
+def test_reference_call_sequence_no_sync(bk):
+    import torch
+    n = 10
+    desc = _gates(n, 7)
+    # gate tensors made once, as Gate.__init__ does
+    gts = [(bk.astensorproduct(O.gate_matrix(name, *params)), qs) for name, params, qs in desc]
+    diag = np.random.default_rng(3).normal(size=2 ** n)
+    ket0 = np.zeros([2] * n, dtype=complex)
+    ket0[(0,) * n] = 1.0
+    ket = bk.astensorproduct(ket0)
+    diag_t = bk.astensorproduct(diag)
+    torch.cuda.synchronize()
+    torch.cuda.set_sync_debug_mode("error")
+    try:
+        for t, qs in gts:                          # Gate.run, ops.py:161-166
+            ket = bk.tensormul(t, ket, list(qs))
+        probs = bk.absolute(ket)                   # states.py:118-119
+        probs = probs * probs
+        expv = bk.sum(bk.real(diag_t) * probs)     # states.py:146-147
+        nrm = bk.absolute(bk.inner(ket, ket))      # qubits.py:180-182
+    finally:
+        torch.cuda.set_sync_debug_mode(0)
+    ref = O.run_circuit(desc, qubits=tuple(range(n)))
+    got = bk.evaluate(ket)
+    assert np.max(np.abs(got - ref)) <= 1e-12
+    assert abs(bk.evaluate(expv) - O.expectation(ref, diag.reshape([2] * n))) <= 1e-12
+    assert abs(bk.evaluate(nrm) - 1.0) <= 1e-12
+
+
+@pytest.mark.gpu
+def test_device_tensor_without_class_runs_dense(bk):
+    # a device gate tensor made on the device (as Gate @ Gate results are)
+    # carries no class and is applied as a dense matrix from device memory
+    import torch
+    n = 9
+    rng = np.random.default_rng(5)
+    for k in (1, 2, 3, 4):
+        m = rng.normal(size=(2 ** k, 2 ** k)) + 1j * rng.normal(size=(2 ** k, 2 ** k))
+        t = torch.from_numpy(m).to('cuda').reshape([2] * (2 * k)) * 1.0   # made on device
+        assert bk.rank(t) == 2 * k
+        psi = rng.normal(size=[2] * n) + 1j * rng.normal(size=[2] * n)
+        idx = list(rng.choice(n, size=k, replace=False))
+        torch.cuda.synchronize()
+        torch.cuda.set_sync_debug_mode("error")
+        try:
+            out = bk.tensormul(t, bk.astensorproduct(psi), idx)
+        finally:
+            torch.cuda.set_sync_debug_mode(0)
+        ref = O.tensormul(O.astensorproduct(m), psi, idx)
+        assert np.max(np.abs(bk.evaluate(out) - ref)) <= 1e-12
+
+
+@pytest.mark.gpu
+def test_inplace_change_forgets_class(bk):
+    import torch
+    x = bk.astensorproduct(O.gate_matrix('X'))
+    psi = np.zeros([2] * 7, dtype=complex)
+    psi[(0,) * 7] = 1.0
+    x.mul_(1j)                                     # no longer the permutation X
+    out = bk.evaluate(bk.tensormul(x, bk.astensorproduct(psi), [3]))
+    ref = O.tensormul(1j * O.astensorproduct(O.gate_matrix('X')), psi, [3])
+    assert np.max(np.abs(out - ref)) <= 1e-15
+    torch.cuda.synchronize()

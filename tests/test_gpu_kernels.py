@@ -239,3 +239,54 @@ def test_tile_pass_of_only_absorbed_swaps(bk):
         assert any(p[1][2] == 0 for p in plan.passes)
         got = plan.run(qf.State(psi)).vec.asarray()
         assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("kind", ["diagonal", "monomial"])
+def test_sparse_4q_gates_two_groups_per_lane(bk, kind):
+    """4- and 5-qubit diagonal and monomial gates on states large enough
+    that every lane of the sparse kernel owns two amplitude groups (n = 13
+    and 15: 2^(n-k) groups exceed the block times two), in place and out of
+    place, against the oracle."""
+    import torch
+    rng = np.random.default_rng(22)
+    for n in (13, 15):
+        psi = rand_state(n, rng)
+        for k in (4, 5):
+            d = 2 ** k
+            if kind == "diagonal":
+                m = np.diag(np.exp(1j * rng.uniform(0, 2 * np.pi, size=d)))
+            else:
+                perm = rng.permutation(d)
+                m = np.zeros((d, d), dtype=complex)
+                m[np.arange(d), perm] = np.exp(1j * rng.uniform(0, 2 * np.pi, size=d))
+            assert bk.classify(m).kind == (bk.GateKind.DIAGONAL if kind == "diagonal"
+                                           else bk.GateKind.MONOMIAL)
+            for trial in range(3):
+                axes = [int(a) for a in rng.permutation(n)[:k]]
+                ref = O.tensormul(m.reshape([2] * (2 * k)), psi, axes)
+                got = gpu_tensormul(bk, m, psi, axes)
+                assert np.max(np.abs(got - ref)) <= TOL, (n, k, axes)
+                bits = [n - 1 - a for a in axes]
+                work = bk.as_device(bk.astensorproduct(psi)).clone()
+                bk.apply_gate(work, work, n, bits, bk.classify(m), inplace_sparse=True)
+                got_ip = work.cpu().numpy().reshape([2] * n)
+                assert np.max(np.abs(got_ip - ref)) <= TOL, (n, k, axes, "in place")
+                torch.cuda.synchronize()
+
+
+def test_density_cnot_7q(bk):
+    """A 7-qubit density (a 14-axis tensor, 2^10 four-qubit groups of the
+    U (x) conj U monomial pass, two per lane) through CNOT gates, against
+    the oracle."""
+    import quantumflow_amd as qf
+    rng = np.random.default_rng(23)
+    n = 7
+    g = rng.normal(size=(2 ** n, 2 ** n)) + 1j * rng.normal(size=(2 ** n, 2 ** n))
+    m = g @ g.conj().T
+    m /= np.trace(m)
+    rho = qf.Density(m.reshape([2] * (2 * n)))
+    for (c, t) in ((0, 6), (5, 2), (3, 4)):
+        m = O.density_evolve(m, O.gate_matrix('CNOT'), n, [c, t])
+        rho = qf.CNOT(c, t).evolve(rho)
+        got = rho.asoperator().cpu().numpy().reshape(2 ** n, 2 ** n)
+        assert np.array_equal(got, m), (c, t)

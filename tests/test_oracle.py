@@ -444,6 +444,33 @@ def test_qaoa_gradients_agree():
     assert np.max(np.abs(sg - fg2)) < 1e-7
 
 
+def test_qaoa_torch_autograd_vs_shift_rule():
+    # the torch restatement: value = the gate-by-gate oracle, autograd
+    # gradient = the exact per-occurrence parameter-shift rule
+    import torch
+    n = 6
+    edges = [(0, 1), (1, 2), (2, 3), (3, 4), (4, 5), (5, 0), (0, 3)]
+    beta = RNG.uniform(0, pi / 2, size=3)
+    gamma = RNG.uniform(0, pi, size=3)
+    b = torch.tensor(beta, requires_grad=True)
+    g = torch.tensor(gamma, requires_grad=True)
+    v = O.qaoa_value_torch(n, edges, b, g)
+    v.backward()
+    assert abs(v.item() - O.qaoa_expectation(n, edges, beta, gamma)) < 1e-12
+    sb, sg = O.qaoa_grad_shift(n, edges, beta, gamma)
+    assert np.max(np.abs(b.grad.numpy() - sb)) < 1e-12
+    assert np.max(np.abs(g.grad.numpy() - sg)) < 1e-12
+
+
+def test_qaoa_fast_with_given_cuts():
+    n = 8
+    edges = [(i, (i * 3 + 1) % n) for i in range(n) if i != (i * 3 + 1) % n]
+    beta, gamma = RNG.uniform(0, 1, size=2), RNG.uniform(0, 1, size=2)
+    cuts = O.graph_cuts(n, edges)
+    assert O.qaoa_expectation_fast(n, edges, beta, gamma, cuts=cuts) == \
+        O.qaoa_expectation_fast(n, edges, beta, gamma)
+
+
 def test_linear_ramp():
     b, g = O.linear_ramp(4, 0.8)
     assert np.allclose(b + g, 0.8)

@@ -157,3 +157,62 @@ def test_tile_sets_cover():
             sets = E.tile_sets(n, T)
             assert set().union(*map(set, sets)) == set(range(n))
             assert all(len(s) == min(T, n) for s in sets)
+
+
+class _FakeBatch:
+    """A CPU stand-in for QAOABatch: <C> = -|x|^2 per instance and a status
+    word that a chosen launch sets and no later launch clears (the kernel's
+    sticky error flag)."""
+    mode = E.MODE_BACKWARD
+
+    def __init__(self, ninst, p, fail_at):
+        import torch
+        self.ninst, self.p = ninst, p
+        self.angles = torch.zeros(ninst * 2 * p, dtype=torch.float64)
+        self.maxcut = np.ones(ninst)
+        self.fail_at, self.launches, self.flag, self.checks = fail_at, 0, 0, 0
+
+    def evaluate(self, beta, gamma, graph=False):
+        import torch
+        self.launches += 1
+        if self.launches == self.fail_at:
+            self.flag = 1
+        x = torch.cat([torch.as_tensor(beta), torch.as_tensor(gamma)], dim=1)
+        return 1.0 - (x * x).sum(1), -2 * x
+
+    def check(self):
+        self.checks += 1
+        E.raise_if_failed(self.flag)
+
+
+def test_status_flag_raises():
+    E.raise_if_failed(0)
+    with pytest.raises(RuntimeError, match="grid barrier"):
+        E.raise_if_failed(1)
+
+
+def test_optimize_sees_a_timeout_from_inside_a_block():
+    # the failing launch is the 3rd of the first 10-step block; later
+    # launches do not clear the flag, and the block-end check raises
+    b = _FakeBatch(3, 2, fail_at=3)
+    with pytest.raises(RuntimeError, match="grid barrier"):
+        E.optimize(b, np.full((3, 2), 0.5), np.full((3, 2), 0.5), use_graph=False)
+    assert b.checks == 1 and b.launches == 11
+    ok = _FakeBatch(3, 2, fail_at=-1)
+    ratio, beta, gamma, steps = E.optimize(ok, np.full((3, 2), 0.5), np.full((3, 2), 0.5),
+                                           use_graph=False, learning_rate=0.1)
+    assert ok.checks >= 1 and np.all(ratio.numpy() > 0.9)
+
+
+def test_autograd_forward_checks_status():
+    import torch
+    b = _FakeBatch(2, 1, fail_at=1)
+    beta = torch.full((2, 1), 0.1, dtype=torch.float64, requires_grad=True)
+    gamma = torch.full((2, 1), 0.2, dtype=torch.float64, requires_grad=True)
+    with pytest.raises(RuntimeError, match="grid barrier"):
+        E.qaoa_expectation(beta, gamma, b)
+    ok = _FakeBatch(2, 1, fail_at=-1)
+    v = E.qaoa_expectation(beta, gamma, ok)
+    v.sum().backward()
+    assert ok.checks == 1
+    assert torch.allclose(beta.grad, -2 * beta.detach())
