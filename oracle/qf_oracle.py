@@ -494,7 +494,9 @@ def qaoa_value_torch(n, edges, beta, gamma, device='cpu'):
     (the stored-intermediates gradient the reference gets from TensorFlow
     autograd, examples/qaoa_maxcut.py:53-64).  beta, gamma: float64 tensors
     (requires_grad as needed); cost layer as in ``qaoa_state_fast``, RX on
-    every qubit by tensordot.  Unit weights."""
+    every qubit as the elementwise pair update of RX(beta) = [[c, -is],
+    [-is, c]] on axis q (state viewed as [2^q, 2, 2^(n-q-1)]).  Unit
+    weights."""
     import torch
     cuts = torch.from_numpy(graph_cuts(n, edges).reshape(-1)).to(device)
     wsum = float(len(edges))
@@ -502,11 +504,10 @@ def qaoa_value_torch(n, edges, beta, gamma, device='cpu'):
     for l in range(len(beta)):
         psi = psi * torch.exp(1j * gamma[l] * (wsum / 2 - cuts))
         c, s = torch.cos(beta[l] / 2), torch.sin(beta[l] / 2)
-        m = torch.stack([torch.stack([c + 0j, -1j * s]), torch.stack([-1j * s, c + 0j])])
-        t = psi.reshape([2] * n)
         for q in range(n):
-            t = torch.movedim(torch.tensordot(m, t, dims=([1], [q])), 0, q)
-        psi = t.reshape(-1)
+            t = psi.reshape(2 ** q, 2, 2 ** (n - q - 1))
+            a0, a1 = t[:, 0, :], t[:, 1, :]
+            psi = torch.stack([c * a0 - 1j * s * a1, c * a1 - 1j * s * a0], dim=1).reshape(-1)
     return torch.sum(torch.abs(psi) ** 2 * cuts)
 
 

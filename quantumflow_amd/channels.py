@@ -22,7 +22,7 @@ import torch
 from . import _native
 from . import backend as bk
 from .ops import Gate, Operation
-from .qubits import Qubits, qubits_count_tuple
+from .qubits import QubitVector, Qubits, qubits_count_tuple
 from .states import State
 
 __all__ = ['Density', 'mixed_density', 'random_density', 'Channel', 'Kraus',
@@ -106,6 +106,12 @@ class Density(State):
 
     def relabel(self, qubits: Qubits) -> 'Density':
         return Density(self.tensor, qubits, self._memory)
+
+    def permute(self, qubits: Qubits) -> 'Density':
+        """The same density with its axes reordered to ``qubits`` (row and
+        column axes alike), still a Density (states.py:287-290)."""
+        vec = self.vec.permute(qubits)
+        return Density(vec.tensor, vec.qubits, self._memory)
 
     def update(self, memory) -> 'Density':
         mem = self.memory
@@ -221,6 +227,59 @@ class Channel(Operation):
 
     def relabel(self, qubits: Qubits) -> 'Channel':
         return Channel(self._mat, qubits, self.params, self._name)
+
+    @property
+    def vec(self) -> QubitVector:
+        """The superoperator as a rank-4 QubitVector on the device
+        (ops.py:272)."""
+        return QubitVector(self.tensor, self.qubits, rank=4)
+
+    def permute(self, qubits: Qubits) -> 'Channel':
+        """The channel with its axes reordered to ``qubits`` in all four
+        super-indices (ops.py:303-306)."""
+        qubits = tuple(qubits)
+        N = self.qubit_nb
+        idx = [self.qubits.index(q) for q in qubits]
+        perm = [r * N + i for r in range(4) for i in idx]
+        t = self._mat.reshape([2] * (4 * N)).transpose(perm)
+        return Channel(t.reshape(4 ** N, 4 ** N), qubits)
+
+    @property
+    def H(self) -> 'Channel':
+        """Conjugate transpose of the superoperator (ops.py:308-310)."""
+        return Channel(self._mat.conj().T, self.qubits)
+
+    def chi(self) -> np.ndarray:
+        """The chi (process) matrix as the reference defines it: the sharp
+        transpose flattened to a 4^N x 4^N matrix, i.e. equal to the Choi
+        matrix (ops.py:344-349)."""
+        return self.choi()
+
+    def trace(self) -> complex:
+        """Trace of the 4^N x 4^N superoperator matrix (ops.py:400-403,
+        qubits.py:185-197)."""
+        return complex(np.trace(self._mat))
+
+    def partial_trace(self, qubits: Qubits) -> 'Channel':
+        """Sum over the traced qubits with all four of their super-index
+        axes tied to one index (qubits.py:201-227, einsum with the traced
+        qubit's subscript repeated in every super-index); the kept axes stay
+        in their order."""
+        new_qubits = list(self.qubits)
+        for q in qubits:
+            new_qubits.remove(q)
+        if not new_qubits:
+            raise ValueError('Cannot remove all qubits with partial_trace.')
+        N = self.qubit_nb
+        traced = {self.qubits.index(q) for q in qubits}
+        letters = list(range(4 * N))
+        for i in traced:
+            for r in range(1, 4):
+                letters[r * N + i] = i
+        out = [r * N + i for r in range(4) for i in range(N) if i not in traced]
+        t = np.einsum(self._mat.reshape([2] * (4 * N)), letters, out)
+        M = len(new_qubits)
+        return Channel(t.reshape(4 ** M, 4 ** M), new_qubits)
 
     @property
     def sharp(self) -> 'Channel':

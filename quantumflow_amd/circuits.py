@@ -126,6 +126,15 @@ class Circuit(Operation):
             gate = elem.asgate() @ gate
         return gate
 
+    def aschannel(self):
+        """The circuit's superoperator: the identity channel on its qubits
+        composed with each element's channel in order (circuits.py:122-127);
+        each composition is one gate-kernel pass over the superoperator."""
+        chan = identity_gate(self.qubits).aschannel()
+        for elem in self.elements:
+            chan = elem.aschannel() @ chan
+        return chan
+
     @property
     def H(self) -> 'Circuit':
         return Circuit([elem.H for elem in self.elements[::-1]])

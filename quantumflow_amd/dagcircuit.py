@@ -63,8 +63,19 @@ class DAGCircuit(Operation):
             ket = elem.run(ket)
         return ket
 
+    def evolve(self, rho):
+        """Every operation applied to a Density in topological order
+        (dagcircuit.py:86-89)."""
+        for elem in self:
+            rho = elem.evolve(rho)
+        return rho
+
     def asgate(self) -> Gate:
         return Circuit(self).asgate()
+
+    def aschannel(self):
+        """The superoperator of the operations in order (dagcircuit.py:94-95)."""
+        return Circuit(self).aschannel()
 
     def _depths(self, ops):
         """Layer index (0-based) of each op: 1 + max over its wire

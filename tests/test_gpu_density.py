@@ -205,3 +205,59 @@ def test_kraus_run_on_states(qf):
     ket = qf.State(psi)
     out = qf.Depolarizing(0.0, 1).run(ket)
     assert np.max(np.abs(out.tensor.cpu().numpy() - psi)) <= 1e-15
+
+
+def _oracle_circuit_evolve(m, n, circ_desc):
+    for name, params, qubits in circ_desc:
+        m = O.density_evolve(m, O.gate_matrix(name, *params), n, qubits)
+    return m
+
+
+_CIRC = [("H", (), [0]), ("CNOT", (), [0, 1]), ("RX", (0.3,), [2]), ("CCNOT", (), [2, 0, 1]),
+         ("ISWAP", (), [1, 2]), ("T", (), [1]), ("CZ", (), [2, 0])]
+
+
+def test_circuit_aschannel_and_dag_evolve(qf):
+    """Circuit.aschannel (circuits.py:122-127) and DAGCircuit.evolve /
+    aschannel (dagcircuit.py:86-96) on a 3-qubit density against the oracle's
+    gate-by-gate evolution; the reference's CCNOT known answer
+    (tests/test_dagcircuit.py:76-94)."""
+    rng = np.random.default_rng(41)
+    n = 3
+    m0 = _rand_density(n, rng)
+    ref = _oracle_circuit_evolve(m0, n, _CIRC)
+    circ = qf.Circuit([getattr(qf, nm)(*p, *q) for nm, p, q in _CIRC])
+    rho = qf.Density(m0.reshape([2] * (2 * n)))
+    chan = circ.aschannel()
+    assert chan.qubits == (0, 1, 2)
+    assert np.max(np.abs(_mat(chan.evolve(rho)) - ref)) <= 1e-12
+    dag = qf.DAGCircuit(circ)
+    assert np.max(np.abs(_mat(dag.evolve(rho)) - ref)) <= 1e-12
+    assert np.max(np.abs(_mat(dag.aschannel().evolve(rho)) - ref)) <= 1e-12
+    # the superoperator of a unitary circuit is U (x) conj(U)
+    u = O.circuit_unitary(_CIRC, (0, 1, 2))
+    assert np.max(np.abs(chan.asoperator() - np.kron(u, u.conj()))) <= 1e-12
+    # reference KAT: CCNOT against its decomposition, as a DAG
+    rho1 = qf.CCNOT(0, 1, 2).evolve(rho)
+    dag = qf.DAGCircuit(qf.ccnot_circuit([0, 1, 2]))
+    assert np.max(np.abs(_mat(dag.evolve(rho)) - _mat(rho1))) <= 1e-12
+    assert np.max(np.abs(_mat(dag.aschannel().evolve(rho)) - _mat(rho1))) <= 1e-12
+
+
+def test_density_permute_stays_a_density(qf):
+    rng = np.random.default_rng(42)
+    n = 4
+    m0 = _rand_density(n, rng)
+    rho = qf.Density(m0.reshape([2] * (2 * n)), qubits=("a", "b", "c", "d"))
+    order = ("c", "a", "d", "b")
+    out = rho.permute(order)
+    assert isinstance(out, qf.Density)
+    assert out.qubits == order
+    perm = [2, 0, 3, 1]
+    want = m0.reshape([2] * (2 * n)).transpose(perm + [n + p for p in perm])
+    assert np.array_equal(_mat(out), want.reshape(2 ** n, 2 ** n))
+    assert abs(complex(out.trace().item()) - 1.0) <= 1e-12
+    pt = out.partial_trace(["a"])
+    ref = O.density_partial_trace(m0, n, [0])
+    sub = ref.reshape([2] * 6).transpose([1, 2, 0, 4, 5, 3]).reshape(8, 8)   # (b,c,d) -> (c,d,b)
+    assert np.max(np.abs(_mat(pt) - sub)) <= 1e-12
