@@ -37,6 +37,11 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "28q c128 gates/s as % HBM roofline; QAOA value+grad/s vs plain, same GPU"
 
 
+def note(msg):
+    """A progress line on stderr (the JSON result is the only stdout line)."""
+    print("# " + msg, file=sys.stderr, flush=True)
+
+
 def cpu_baseline(desc, n, ngates):
     """Time the oracle (numpy einsum per gate) on the first ``ngates`` gates
     of the circuit at the full size, on this host."""
@@ -158,10 +163,12 @@ def density_section(n=13, ngates=40, seed=5):
             "trace_after": round(tr.real, 12)}
 
 
-def qaoa_section(steps, plain_sample=24, grad_sample=4):
+def qaoa_section(steps, grad_compare_n=18):
     """Batched QAOA (512 ER graphs, n 10..22, p=4: values; 64 3-regular
     graphs, n 16..24, p=8: values + adjoint gradients) against the plain
-    per-gate GPU path on samples of the same instances."""
+    per-gate GPU path run on exactly the same instances, one at a time; and
+    one n=18, p=8 instance's value + gradient by stored-intermediates torch
+    autograd on the GPU against the engine (peak memory and time)."""
     import numpy as np
     import torch
     from quantumflow_amd import qaoa as Q
@@ -191,22 +198,30 @@ def qaoa_section(steps, plain_sample=24, grad_sample=4):
     dt = timed(batch.replay, steps)
     batch.check()
     vals = batch.out.view(-1).cpu().numpy()
-    tp = timed(lambda: [Q.plain_value(graphs[i], beta[i], gamma[i]) for i in range(plain_sample)], 1)
-    plain_vals = [Q.plain_value(graphs[i], beta[i], gamma[i]).item() for i in range(plain_sample)]
+    del batch
+    plain_vals = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(len(graphs)):
+        plain_vals.append(Q.plain_value(graphs[i], beta[i], gamma[i]))
+    torch.cuda.synchronize()
+    tp = time.perf_counter() - t0
+    plain_vals = torch.stack(plain_vals).cpu().numpy()
     out["forward"] = {
         "workload": "512 ER graphs n=10..22 p_edge=0.3..0.6, p=4, complex128",
         "batched_instances_per_s": round(512 / dt, 1), "batched_ms": round(dt * 1e3, 3),
         "setup_s (programs + cut tables, once per graph set)": round(setup, 3),
-        "plain_instances_per_s": round(plain_sample / tp, 2),
-        "plain_sample": plain_sample,
-        "speedup_vs_plain": round((512 / dt) / (plain_sample / tp), 1),
-        "max_abs_diff_vs_plain": float(np.max(np.abs(vals[:plain_sample] - np.array(plain_vals)))),
+        "plain_instances_per_s": round(len(graphs) / tp, 2), "plain_s": round(tp, 3),
+        "plain_sample": "all 512 instances, one at a time (same instances as the batch)",
+        "speedup_vs_plain": round((512 / dt) / (len(graphs) / tp), 1),
+        "max_abs_diff_vs_plain": float(np.max(np.abs(vals - plain_vals))),
     }
     # values + gradients: 64 3-regular instances, p=8
     inst = regular_instances(64, 16, 24, 8, seed=12)
     graphs = [g for g, _, _ in inst]
     beta = np.array([b for _, b, _ in inst])
     gamma = np.array([g for _, _, g in inst])
+    torch.cuda.synchronize()
     torch.cuda.reset_peak_memory_stats()
     base = torch.cuda.memory_allocated()
     batch = E.QAOABatch(graphs, 8, mode=E.MODE_BACKWARD)
@@ -216,25 +231,219 @@ def qaoa_section(steps, plain_sample=24, grad_sample=4):
     batch.check()
     peak = torch.cuda.max_memory_allocated() - base
     res = batch.out.view(64, -1).cpu().numpy()
-    errs, tps = [], 0.0
-    for i in range(grad_sample):
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
+    del batch
+    errs = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(len(graphs)):
         v, g = Q.plain_value_and_grad(graphs[i], beta[i], gamma[i])
-        torch.cuda.synchronize()
-        tps += time.perf_counter() - t0
         errs.append(float(np.max(np.abs(g.cpu().numpy() - res[i, 1:]))))
         errs.append(abs(v.item() - res[i, 0]))
+    torch.cuda.synchronize()
+    tps = time.perf_counter() - t0
     stored = sum((len(g) + 8 * (g.number_of_edges() + len(g))) * 16 * 2 ** len(g) for g in graphs)
     out["value_and_grad"] = {
         "workload": "64 random 3-regular graphs n=16..24 (even), p=8, complex128",
         "batched_value_grad_per_s": round(64 / dt, 2), "batched_ms": round(dt * 1e3, 3),
-        "plain_value_grad_per_s": round(grad_sample / tps, 3), "plain_sample": grad_sample,
-        "speedup_vs_plain": round((64 / dt) / (grad_sample / tps), 1),
+        "plain_value_grad_per_s": round(len(graphs) / tps, 3), "plain_s": round(tps, 3),
+        "plain_sample": "all 64 instances, one at a time: per-gate forward, per-gate adjoint sweep",
+        "speedup_vs_plain": round((64 / dt) / (len(graphs) / tps), 1),
         "max_abs_diff_vs_plain_adjoint": max(errs),
         "engine_peak_bytes": int(peak),
-        "stored_intermediates_bytes (one state per gate, as autograd keeps)": int(stored),
+        "stored_intermediates_bytes_estimate (one state per gate)": int(stored),
     }
+    # one instance: stored-intermediates autograd vs the engine, measured
+    k = next(i for i, g in enumerate(graphs) if len(g) == grad_compare_n)
+    g1 = [graphs[k]]
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    torch.cuda.reset_peak_memory_stats()
+    base = torch.cuda.memory_allocated()
+    t0 = time.perf_counter()
+    av, ag = Q.autograd_value_and_grad(graphs[k], beta[k], gamma[k])
+    torch.cuda.synchronize()
+    t_auto = time.perf_counter() - t0
+    peak_auto = torch.cuda.max_memory_allocated() - base
+    av, ag = av.item(), ag.cpu().numpy()
+    torch.cuda.empty_cache()
+    torch.cuda.reset_peak_memory_stats()
+    base = torch.cuda.memory_allocated()
+    one = E.QAOABatch(g1, 8, mode=E.MODE_BACKWARD)
+    ev, eg = one.evaluate(beta[k:k + 1], gamma[k:k + 1])
+    one.check()
+    t_eng = timed(lambda: one.evaluate(beta[k:k + 1], gamma[k:k + 1]), 5)
+    peak_eng = torch.cuda.max_memory_allocated() - base
+    ev, eg = ev[0].item(), eg[0].cpu().numpy()
+    del one
+    out["stored_intermediates_backward"] = {
+        "workload": "one random 3-regular graph n=%d (|E|=%d), p=8: value + 16-entry gradient"
+                    % (grad_compare_n, graphs[k].number_of_edges()),
+        "autograd_peak_bytes": int(peak_auto), "autograd_s": round(t_auto, 4),
+        "engine_peak_bytes": int(peak_eng), "engine_s": round(t_eng, 5),
+        "memory_ratio": round(peak_auto / max(peak_eng, 1), 1),
+        "max_abs_diff_grad": float(np.max(np.abs(ag - eg))),
+        "abs_diff_value": abs(av - ev),
+    }
+    return out
+
+
+def config1_section(reps=20):
+    """The 12-qubit, 20-layer rotation circuit (360 gates) from |0...0>:
+    the state is 64 KiB, so this is launch- and latency-bound: microseconds
+    per circuit and gates/s, on the per-gate path and the fused executor
+    (one or two passes), plus all <Z_i>."""
+    import torch
+    import quantumflow_amd as qf
+    from quantumflow_amd import fusion
+    from quantumflow_amd.workloads import layered_rotation_circuit, build_gates
+    n = 12
+    gates = build_gates(layered_rotation_circuit(n, 20, seed=1))
+    qubits = tuple(range(n))
+    circ = qf.Circuit(gates)
+    out = {"workload": "12-qubit layered rotation circuit, 20 layers, %d gates, seed 1" % len(gates)}
+
+    def per_circuit(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    dt = per_circuit(lambda: circ.run(qf.zero_state(n), fused=False).expectation_z())
+    out["per_gate_us_per_circuit"] = round(dt * 1e6, 1)
+    out["per_gate_gates_per_s"] = round(len(gates) / dt, 1)
+    plan = fusion.plan_for(gates, qubits)
+    work = qf.backend.as_device(qf.zero_state(n).tensor).reshape(-1).clone()
+
+    def fused():
+        work.zero_()
+        work[0] = 1.0
+        plan.run_inplace(work)
+        qf.State(work.view([2] * n), qubits).expectation_z()
+    dt = per_circuit(fused)
+    out["fused_us_per_circuit"] = round(dt * 1e6, 1)
+    out["fused_gates_per_s"] = round(len(gates) / dt, 1)
+    out["fused_passes"] = plan.n_passes
+    return out
+
+
+def gops_section(sizes=(16, 20, 24, 28), gates=100, reps=3):
+    """The reference's GOPS harness (tools/benchmark.py:31-53): H on every
+    qubit, then X, T, CNOT triples on random pairs, GOPS = 100 / seconds.
+    Gate by gate through Gate.run (the reference's call pattern: each gate
+    returns a new state) and through Circuit.run's fused executor."""
+    import torch
+    import quantumflow_amd as qf
+    from quantumflow_amd.workloads import gops_circuit, build_gates
+    out = {}
+    for n in sizes:
+        gl = build_gates(gops_circuit(n, gates, seed=n))
+        circ = qf.Circuit(gl)
+
+        def per_gate():
+            ket = qf.zero_state(n)
+            for g in gl:
+                ket = g.run(ket)
+            return ket
+
+        def fused():
+            return circ.run(qf.zero_state(n))
+        row = {}
+        for name, fn in (("per_gate", per_gate), ("fused", fused)):
+            fn()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                fn()
+            torch.cuda.synchronize()
+            dt = (time.perf_counter() - t0) / reps
+            row[name + "_gops"] = round(gates / dt, 1)
+        out["n%d" % n] = row
+        torch.cuda.empty_cache()
+    return out
+
+
+def _qaoa_cpu_value(args):
+    """One config-3 instance's <C> by the numpy oracle (one process)."""
+    from oracle import qf_oracle as O
+    n, edges, beta, gamma = args
+    t0 = time.perf_counter()
+    O.qaoa_expectation_fast(n, edges, beta, gamma)
+    return n, time.perf_counter() - t0
+
+
+def cpu_extra_baselines(desc20, desc24):
+    """CPU legs beyond the headline sample, on this host with the oracle:
+    config 1 (12 qubits, whole circuit, 1 thread); config 2 at 20 qubits
+    (whole circuit) and 24 qubits (first 48 gates), numpy einsum per gate,
+    1 thread; config 3 values on all cores (a process per instance) over 4
+    instances per qubit count, extrapolated to the 512-instance mix by the
+    per-n mean times; config 4 value + gradient by torch complex128 autograd
+    on the CPU (intra-op threads = cores) for two n=16 instances."""
+    import multiprocessing as mp
+    import numpy as np
+    import torch
+    from oracle import qf_oracle as O
+    from quantumflow_amd.workloads import (layered_rotation_circuit, qaoa_batch_instances,
+                                           regular_instances)
+    cores = len(os.sched_getaffinity(0))
+    procs = min(16, cores)
+    out = {"host_cpu_count": os.cpu_count(), "affinity": cores}
+
+    def run(desc, n, ngates):
+        ket = np.zeros(2 ** n, dtype=np.complex128)
+        ket[0] = 1.0
+        ket = ket.reshape([2] * n)
+        qubits = tuple(range(n))
+        t0 = time.perf_counter()
+        for g in desc[:ngates]:
+            ket = O.run_gate(ket, qubits, O.gate_matrix(g[0], *g[1]), g[2])
+        return time.perf_counter() - t0
+
+    d1 = layered_rotation_circuit(12, 20, seed=1)
+    t = run(d1, 12, len(d1))
+    out["config1_12q"] = {"us_per_circuit": round(t * 1e6, 1),
+                          "gates_per_s": round(len(d1) / t, 1), "threads": 1}
+    t = run(desc20, 20, len(desc20))
+    out["config2_20q_full_circuit"] = {"s": round(t, 3), "gates": len(desc20),
+                                       "gates_per_s": round(len(desc20) / t, 2), "threads": 1}
+    t = run(desc24, 24, 48)
+    out["config2_24q_first48"] = {"s": round(t, 3), "gates": 48,
+                                  "gates_per_s": round(48 / t, 3), "threads": 1}
+    inst = qaoa_batch_instances(512, 10, 22, 4, seed=11)
+    by_n = {}
+    for g, b, gm in inst:
+        by_n.setdefault(len(g), []).append((len(g), list(g.edges()), b, gm))
+    sample = [a for n in sorted(by_n) for a in by_n[n][:4]]
+    t0 = time.perf_counter()
+    with mp.get_context("fork").Pool(procs) as pool:
+        res = pool.map(_qaoa_cpu_value, sample, chunksize=1)
+    wall = time.perf_counter() - t0
+    mean_t = {}
+    for n, dt in res:
+        mean_t.setdefault(n, []).append(dt)
+    mean_t = {n: float(np.mean(v)) for n, v in mean_t.items()}
+    est = sum(mean_t[len(g)] for g, _, _ in inst) / procs
+    out["config3_all_cores"] = {
+        "processes": procs, "sample": "%d instances (4 per n=10..22), wall %.2f s"
+        % (len(sample), wall),
+        "sample_instances_per_s": round(len(sample) / wall, 2),
+        "estimated_512_batch_instances_per_s": round(512 / est, 2)}
+    inst = regular_instances(64, 16, 24, 8, seed=12)
+    pick = [i for i, (g, _, _) in enumerate(inst) if len(g) == 16][:2]
+    torch.set_num_threads(procs)
+    t0 = time.perf_counter()
+    for i in pick:
+        g, b, gm = inst[i]
+        bt = torch.tensor(b, dtype=torch.float64, requires_grad=True)
+        gt = torch.tensor(gm, dtype=torch.float64, requires_grad=True)
+        O.qaoa_value_torch(16, list(g.edges()), bt, gt).backward()
+    t = time.perf_counter() - t0
+    out["config4_torch_autograd_cpu"] = {
+        "threads": procs, "sample": "%d instances at n=16, p=8" % len(pick),
+        "value_grad_per_s": round(len(pick) / t, 3)}
     return out
 
 
@@ -251,6 +460,8 @@ def main():
     ap.add_argument("--no-density", action="store_true")
     ap.add_argument("--no-mfma", action="store_true")
     ap.add_argument("--no-unfused", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the 12-qubit, GOPS and extra CPU sections")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one plain device copy of the state (4 GiB read + 4 GiB "
                          "written at 28 qubits) to scale HBM byte counters against")
@@ -328,34 +539,63 @@ def main():
     }
     if plan is not None:
         out["config"].update(plan.stats())
-    if not args.no_unfused:
+    # correctness of the timed path at this size: the fused plan and the
+    # unfused per-gate path, each once from |0...0>, compared amplitude by
+    # amplitude (the unfused run is also the unfused throughput figure)
+    note("timed %d steps: %.3f ms per step; checking fused vs unfused" % (args.steps, ms))
+    if plan is not None and not args.no_unfused:
+        work.zero_()
+        work[0] = 1.0
+        plan.run_inplace(work)
+        ref = torch.zeros_like(work)
+        ref[0] = 1.0
         unf = fusion.KernelTimer()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        fusion.apply_gates_inplace(gates, qubits, work, unf)
+        fusion.apply_gates_inplace(gates, qubits, ref, unf)
         torch.cuda.synchronize()
         dtu = time.perf_counter() - t0
+        err = float((work - ref).abs().max().item())
+        norm = float(qf.backend.norm(work))
         un_name, un_ms, _ = unf.dominant()
         out["unfused"] = {"gates_per_s": round(len(gates) / dtu, 2),
                           "dominant_kernel": un_name, "avg_ms": round(un_ms, 4),
                           "achieved_GBs": round(bytes_per_pass / (un_ms * 1e-3) / 1e9, 1)}
+        out["check"] = {"max_abs_diff_fused_vs_unfused": err, "norm_minus_1": norm - 1.0,
+                        "tolerance": 1e-10, "ok": bool(err <= 1e-10 and abs(norm - 1.0) <= 1e-10)}
+        del ref
+        torch.cuda.empty_cache()
     if args.copy_calibration:
         spare = torch.empty_like(work)
         spare.copy_(work)
         torch.cuda.synchronize()
         del spare
     if not args.no_mfma:
+        note("matrix-core block plans")
         out["fused_mfma_blocks"] = fused_block_section(gates, qubits, work)
+    if not args.no_extra:
+        note("12-qubit circuit and GOPS")
+        out["config1"] = config1_section()
+        out["gops"] = gops_section()
     if not args.no_qaoa:
         del state
+        note("QAOA batches and the plain path on the same instances")
         out["qaoa"] = qaoa_section(max(2, args.steps))
     if not args.no_density:
+        note("density evolution")
         out["density"] = density_section()
     if not args.no_cpu and args.cpu_gates > 0:
         del work
         torch.cuda.empty_cache()
+        note("CPU baselines")
         out["cpu_baseline"] = cpu_baseline(desc, n, args.cpu_gates)
+        if not args.no_extra:
+            out["cpu_baselines_extra"] = cpu_extra_baselines(
+                random_mixed_circuit(20, DEPTH, seed=SEED), random_mixed_circuit(24, DEPTH, seed=SEED))
     print(json.dumps(out), flush=True)
+    if "check" in out and not out["check"]["ok"]:
+        print("bench.py: fused and unfused states disagree (%s)" % out["check"], file=sys.stderr)
+        sys.exit(3)
 
 
 if __name__ == "__main__":

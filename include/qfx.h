@@ -54,7 +54,7 @@ const char* qfx_version(void);
 /* Initialise for the current device; returns QFX_ENODEV if it is not gfx950.
  * Safe to call more than once.  Fills (host) props[8] when props != NULL:
  * {CUs, LDS bytes per block, warp size, L2 bytes, clock kHz, total MiB,
- *  max threads per block, 0}. */
+ *  max threads per block, device wall-clock (s_memrealtime) rate kHz}. */
 int qfx_init(int64_t* props);
 
 /* ---- gate application (replaces numpybk.tensormul,
@@ -155,7 +155,7 @@ int qfx_histogram(void* stream, int64_t* counts, const int64_t* idx, int64_t m,
  * vectorised 2n-qubit tensor of the reference's Density
  * (quantumflow/states.py:264-344).  Gates, Kraus operators and channels act
  * on it through qfx_apply_* (rows by U, columns by conj(U)); these add the
- * rest.  1 <= n <= QFX_DENSITY_MAX_QUBITS. */
+ * rest.  0 <= n <= QFX_DENSITY_MAX_QUBITS (1 <= n for the partial trace). */
 #define QFX_DENSITY_MAX_QUBITS 15
 /* rho = psi psi^H; replaces State.asdensity (quantumflow/states.py:162-165) */
 int qfx_density_outer(void* stream, void* rho, const void* psi, int n);
@@ -243,6 +243,10 @@ typedef struct {
 #define QFX_TILE_REG_IO 8 /* flags: persistent pass whose last op is a register
                              phase storing straight to HBM; needs
                              2^tile_bits == 16 * block, block <= 512 */
+#define QFX_TILE_REG_IN 16 /* flags: persistent pass whose first op is a register
+                              phase loading straight from HBM (no LDS round
+                              trip before it); needs 2^tile_bits == 16 * block,
+                              block <= 512; combines with QFX_TILE_REG_IO */
 
 /* Fills the derived `lds` words of every register phase in ops[0..nops-1]
  * (host array) from its tile bits and maps; zeroes them for other kinds.
@@ -266,11 +270,12 @@ int qfx_tile_check_ops(int T, int block, int flags, const qfx_tile_op* ops /*hos
  * list's ncoef coefficients (indices in the ops are relative to coef).  A
  * pass without QFX_TILE_WIDE or QFX_TILE_MFMA takes at most 768
  * coefficients: it stages them in LDS.  Tile bit j is loaded from index bit
- * tile_pos[j] and stored to index bit store_pos[j] (a permutation of
- * tile_pos; NULL: the same), so a pass can end with a relabelling of its
- * qubits (the SWAP gates it absorbed). */
+ * tile_pos[j].  The pass runs in place and stores source index bit b to
+ * index bit dest[b] (NULL: the identity): dest may permute the tile
+ * positions among themselves (a relabelling of the pass's qubits, the SWAP
+ * gates it absorbed) and must keep every other index bit. */
 int qfx_tile_pass(void* stream, void* state, int n, int tile_bits,
-                  const int* tile_pos /*host*/, const int* store_pos /*host or NULL*/,
+                  const int* tile_pos /*host*/, const int* dest /*host or NULL*/,
                   const void* ops, int nops, const void* coef, int ncoef,
                   const void* srcs, int block, int flags);
 
@@ -362,6 +367,12 @@ typedef struct {
                              12 backward.  Must be what qfx_qaoa_check
                              accepted the program with. */
   int32_t pad;
+  void* timing;           /* NULL, or uint64 [timing_len]: per global pass k
+                             and workgroup b, entries 2 (k grid + b) and +1
+                             get the device wall clock when b starts pass k
+                             and when it reaches the barrier after it (entries
+                             past timing_len are not written) */
+  int64_t timing_len;
 } qfx_qaoa_args;
 
 /* Host-side validation of a program (host copies of the device arrays);

@@ -16,6 +16,8 @@ from .ops import *  # noqa: F401,F403
 from .gates import *  # noqa: F401,F403
 from .stdgates import *  # noqa: F401,F403
 from .stdops import *  # noqa: F401,F403
+from .cbits import *  # noqa: F401,F403
+from .programs import *  # noqa: F401,F403
 from .circuits import *  # noqa: F401,F403
 from .channels import *  # noqa: F401,F403
 from .dagcircuit import *  # noqa: F401,F403

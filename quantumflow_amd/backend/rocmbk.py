@@ -54,6 +54,7 @@ pi = np.pi
 
 _NP_CTYPE = np.complex128
 _inited = set()
+_props = {}
 _workspace = {}
 
 
@@ -79,7 +80,15 @@ def _ensure_device():
         props = (_native.ctypes.c_int64 * 8)()
         _native.call('qfx_init', props)
         _inited.add(dev)
+        _props[dev] = list(props)
     return dev
+
+
+def device_props():
+    """qfx_init's properties of the current device: CUs, LDS bytes per
+    block, warp size, L2 bytes, clock kHz, total MiB, max threads per block,
+    wall-clock rate kHz."""
+    return list(_props[_ensure_device()])
 
 
 def stream_handle():

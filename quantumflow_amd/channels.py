@@ -37,8 +37,8 @@ def _ptr(t):
 
 
 def _check_n(n):
-    if not 1 <= n <= MAX_QUBITS:
-        raise ValueError('density matrices hold 1..%d qubits' % MAX_QUBITS)
+    if not 0 <= n <= MAX_QUBITS:
+        raise ValueError('density matrices hold 0..%d qubits' % MAX_QUBITS)
 
 
 class Density(State):

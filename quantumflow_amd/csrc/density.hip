@@ -129,8 +129,10 @@ __global__ __launch_bounds__(kBlock) void ptrace_kernel(double2* __restrict__ ou
   }
 }
 
-int check_n(int n, const char* what) {
-  if (n < 1 || n > QFX_DENSITY_MAX_QUBITS) return qfx::set_error(QFX_ERANGE, what);
+// A density of 0 qubits is the 1 x 1 matrix of a scalar state (the
+// reference's empty program evolves one); a partial trace needs a qubit.
+int check_n(int n, const char* what, int lo = 0) {
+  if (n < lo || n > QFX_DENSITY_MAX_QUBITS) return qfx::set_error(QFX_ERANGE, what);
   return QFX_OK;
 }
 
@@ -185,7 +187,7 @@ int qfx_density_axpy(void* stream, void* y, const void* x, double w, int64_t len
 int qfx_density_partial_trace(void* stream, void* out, const void* rho, int n,
                               uint32_t keep_mask) {
   if (!out || !rho) return qfx::set_error(QFX_EINVAL, "qfx_density_partial_trace: null pointer");
-  if (int rc = check_n(n, "qfx_density_partial_trace: n out of range")) return rc;
+  if (int rc = check_n(n, "qfx_density_partial_trace: n out of range", 1)) return rc;
   if (keep_mask == 0 || (keep_mask >> n) != 0)
     return qfx::set_error(QFX_EINVAL, "qfx_density_partial_trace: bad keep mask");
   const int m = __builtin_popcount(keep_mask);

@@ -69,7 +69,11 @@ int qfx_init(int64_t* props) {
     props[4] = p.clockRate;
     props[5] = (int64_t)(p.totalGlobalMem >> 20);
     props[6] = p.maxThreadsPerBlock;
-    props[7] = 0;
+    int wall_khz = 0;
+    rc = qfx::check_hip(hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, dev),
+                        "hipDeviceGetAttribute");
+    if (rc) return rc;
+    props[7] = wall_khz;
   }
   return QFX_OK;
 }

@@ -607,12 +607,16 @@ __global__ __launch_bounds__(BS) void qaoa_kernel(
     const qfx_qaoa_item* __restrict__ items, const int* __restrict__ kstart, int nk,
     const double* __restrict__ angles, v2d* __restrict__ psi_pool, v2d* __restrict__ lam_pool,
     const uint16_t* __restrict__ cnt_pool, const double* __restrict__ val_pool,
-    double* __restrict__ partials, int width, unsigned* sync, int* err, uint64_t limit_ticks) {
+    double* __restrict__ partials, int width, unsigned* sync, int* err, uint64_t limit_ticks,
+    uint64_t* __restrict__ timing, int64_t timing_len) {
   extern __shared__ v2d lds[];
   __shared__ double red[16];
   __shared__ double accum[kAccum];
   __shared__ v2d tab[kMaxEdgesTab];
   for (int k = 0; k < nk; ++k) {
+    const int64_t tslot = 2 * ((int64_t)k * gridDim.x + blockIdx.x);
+    const bool timed = timing != nullptr && tslot + 1 < timing_len && threadIdx.x == 0;
+    if (timed) timing[tslot] = wall_clock64();
     for (int it = kstart[k] + (int)blockIdx.x; it < kstart[k + 1]; it += gridDim.x) {
       const qfx_qaoa_item item = items[it];
       const qfx_qaoa_inst& in = insts[item.inst];
@@ -620,6 +624,7 @@ __global__ __launch_bounds__(BS) void qaoa_kernel(
                         lam_pool, cnt_pool, val_pool,
                         partials + (uint64_t)item.partial * width, width, lds, red, accum, tab);
     }
+    if (timed) timing[tslot + 1] = wall_clock64();
     if (k + 1 < nk && !grid_barrier(sync, (unsigned)(k + 1) * gridDim.x, err, limit_ticks))
       return;
   }
@@ -710,7 +715,8 @@ int launch_qaoa(hipStream_t s, const qfx_qaoa_args& a, int T) {
                      a.nk, static_cast<const double*>(a.angles), static_cast<v2d*>(a.psi_pool),
                      static_cast<v2d*>(a.lam_pool), static_cast<const uint16_t*>(a.cnt_pool),
                      static_cast<const double*>(a.val_pool), static_cast<double*>(a.partials),
-                     a.width, sync, err, gc.limit_ticks);
+                     a.width, sync, err, gc.limit_ticks, static_cast<uint64_t*>(a.timing),
+                     a.timing ? a.timing_len : (int64_t)0);
   rc = qfx::check_launch("qaoa_kernel");
   if (rc) return rc;
   const int nout = a.ninst * a.width;

@@ -12,6 +12,11 @@
 // (quantumflow/circuits.py:87-99) for runs of gates the planner
 // (quantumflow_amd/fusion.py) groups into passes.
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <set>
+#include <string>
+#include <tuple>
 
 #include "qfx_common.h"
 
@@ -32,9 +37,14 @@ constexpr int kMfmaMaxBlock = 512;    // largest block running matrix-core block
 // CU (2 x 76 KiB of the 160 KiB).
 constexpr int kMaxStaged = 768;
 
+// Where a pass loads and stores.  A pass is in place: it stores each tile
+// to the addresses it loaded, possibly with its tile bits permuted (tile bit
+// j is stored to index bit d[j], a permutation of s[0..T-1]: the qubit
+// relabelling of the SWAP gates the pass absorbed).
 struct TilePos {
-  int s[QFX_MAX_QUBITS];  // tile bit positions, ascending (where tile bit j loads from)
-  int d[QFX_MAX_QUBITS];  // where tile bit j is stored (a permutation of s[0..T-1])
+  int s[QFX_MAX_QUBITS];   // tile bit positions, ascending (where tile bit j loads from)
+  int d[QFX_MAX_QUBITS];   // where tile bit j is stored
+  int permuted;            // d differs from s somewhere
 };
 
 __device__ __forceinline__ unsigned ins0(unsigned g, int b) {
@@ -895,6 +905,131 @@ __global__ __launch_bounds__(BS) void tile_reg_kernel(v2d* state, int T, TilePos
   }
 }
 
+// ---- register-phase passes that load through their first phase ----------
+// A persistent pass (T = LB + 4) whose first op is a register phase loads
+// that phase's 16 amplitudes per lane straight from HBM into registers (the
+// phase's load map applied to global offsets, tile bit j <-> index bit
+// pos.s[j]), so the tile never takes the LDS write, barrier and LDS read
+// that would otherwise precede the phase.  The next tile's amplitudes are
+// in flight in `pf` while the current one is processed.  With RIO the last
+// register phase also stores straight to HBM; a pass of one such phase
+// then touches no LDS at all.  Lanes 0..7 of a wave own neighbouring
+// amplitudes when the phase's register bits avoid tile bits 0..2 (the
+// planner asks for that), so loads come in 128-byte runs.
+__device__ __forceinline__ void phase_gload(v2d (&v)[kRegs], const v2d* p, const PhaseGlobal& g) {
+  uint64_t a[kRegs];
+  xor_tree(a, g.c, g.b);
+#pragma unroll
+  for (int r = 0; r < kRegs; ++r) v[r] = ld_stream(p + a[r]);
+}
+
+// How a register-input pass ends: its tile goes back to HBM from LDS
+// (RIN_LDS_OUT), through its last register phase (RIN_REG_OUT), or the pass
+// is that one phase, loading and storing through registers (RIN_SINGLE).
+enum RinMode : int { RIN_LDS_OUT = 0, RIN_REG_OUT = 1, RIN_SINGLE = 2 };
+
+// Two waves per SIMD (at most 256 registers a lane, the other kernels'
+// budget) is what hides the LDS and memory latency; the hint keeps the
+// variants that hold the next tile and a phase's registers at once within it.
+template <int BS, int LB, int MODE>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void tile_rin_kernel(
+    v2d* state, int T, TilePos pos, const qfx_tile_op* __restrict__ ops, int nops,
+    const v2d* __restrict__ coef, int ncoef, const int* __restrict__ srcs, uint32_t ntiles) {
+  extern __shared__ v2d lds[];
+  const int tid = threadIdx.x;
+  v2d* cf = stage_coef<BS>(lds, T, coef, ncoef);
+  const qfx_tile_op& first = ops[0];
+  const unsigned f_egrp = phase_lb(first, (unsigned)tid);
+  const uint64_t f_lane = tile_state_off(f_egrp, T, pos.s);
+  const PhaseGlobal fg = phase_global(first, first.phase, T, pos.s);
+  uint64_t tmask = 0;
+#pragma unroll
+  for (int j = 0; j < kMaxTileBits; ++j)
+    if (j < T) tmask |= (uint64_t)1 << pos.s[j];
+  auto tile_base = [&](uint32_t t) {
+    uint64_t base = t;
+#pragma unroll
+    for (int j = 0; j < kMaxTileBits; ++j) {
+      if (j < T) {
+        const int b = pos.s[j];
+        base = ((base >> b) << (b + 1)) | (base & (((uint64_t)1 << b) - 1));
+      }
+    }
+    return base;
+  };
+  const uint64_t step = tile_base(gridDim.x);
+  v2d pf[kRegs];
+  uint32_t tile = blockIdx.x;       // the grid never exceeds ntiles
+  uint64_t base = tile_base(tile);
+  phase_gload(pf, state + base + f_lane, fg);
+  __syncthreads();                  // staged coefficients
+  if constexpr (MODE == RIN_SINGLE) {
+    const uint64_t s_lane = tile_state_off(f_egrp, T, pos.d);
+    const PhaseGlobal sg = phase_global(first, first.active, T, pos.d);
+    for (; tile < ntiles; tile += gridDim.x) {
+      const uint64_t nbase =
+          tile + gridDim.x < ntiles ? ((base | tmask) + step) & ~tmask : base;
+      v2d v[kRegs];
+#pragma unroll
+      for (int r = 0; r < kRegs; ++r) v[r] = pf[r];
+      phase_gload(pf, state + nbase + f_lane, fg);
+      run_subs(v, first, cf, srcs, base, f_egrp);
+      // No LDS is involved, but with a permuted store a wave writes
+      // amplitudes other waves of the workgroup load: every wave must hold
+      // this tile in registers (v) before any stores it.
+      if (pos.permuted) __syncthreads();
+      phase_gstore(v, state + base + s_lane, sg);
+      base = nbase;
+    }
+    return;
+  }
+  // store side of RIN_LDS_OUT: the LDS store loop
+  uint64_t soff = 0;
+#pragma unroll
+  for (int j = 0; j < LB; ++j) soff |= (uint64_t)((tid >> j) & 1) << pos.d[j];
+  const uint64_t sb0 = (uint64_t)1 << pos.d[LB], sb1 = (uint64_t)1 << pos.d[LB + 1];
+  const uint64_t sb2 = (uint64_t)1 << pos.d[LB + 2], sb3 = (uint64_t)1 << pos.d[LB + 3];
+  const int nmid_end = MODE == RIN_REG_OUT ? nops - 1 : nops;   // ops 1 .. nmid_end-1 from LDS
+  for (; tile < ntiles; tile += gridDim.x) {
+    // the last tile re-reads itself instead of branching around the loads
+    const uint64_t nbase =
+        tile + gridDim.x < ntiles ? ((base | tmask) + step) & ~tmask : base;
+    v2d v[kRegs];
+#pragma unroll
+    for (int r = 0; r < kRegs; ++r) v[r] = pf[r];
+    phase_gload(pf, state + nbase + f_lane, fg);
+    run_subs(v, first, cf, srcs, base, f_egrp);
+    phase_lds_store(v, lds, sw(f_egrp) << 4, phase_lds(first));
+    __syncthreads();
+    for (int i = 1; i < nmid_end; ++i) {
+      apply_op<BS, false, false>(lds, T, ops[i], cf, coef, srcs, base);
+      __syncthreads();
+    }
+    if constexpr (MODE == RIN_REG_OUT) {
+      // the last phase reuses the first phase's registers
+      const qfx_tile_op& last = ops[nops - 1];
+      const unsigned egrp = phase_lb(last, (unsigned)tid);
+      phase_lds_load(v, lds, sw(egrp) << 4, phase_lds(last));
+      run_subs(v, last, cf, srcs, base, egrp);
+      phase_gstore(v, state + base + tile_state_off(egrp, T, pos.d),
+                   phase_global(last, last.active, T, pos.d));
+    } else {
+      v2d* q0 = state + base + soff;
+      v2d* q1 = q0 + sb1;
+#pragma unroll
+      for (int u = 0; u < kPrefetch; u += 4) {
+        const uint64_t o = ((u & 4) ? sb2 : 0) + ((u & 8) ? sb3 : 0);
+        st_stream(q0 + o, lds[sw(tid + u * BS)]);
+        st_stream(q0 + o + sb0, lds[sw(tid + (u + 1) * BS)]);
+        st_stream(q1 + o, lds[sw(tid + (u + 2) * BS)]);
+        st_stream(q1 + o + sb0, lds[sw(tid + (u + 3) * BS)]);
+      }
+    }
+    base = nbase;
+    __syncthreads();                // LDS free for the next tile's first phase
+  }
+}
+
 // Register phases address LDS by absolute byte offsets (lds_byte), valid
 // only when the kernel's dynamic LDS array starts at address 0, i.e. the
 // kernel has no static LDS; also raises the dynamic LDS limit.
@@ -909,73 +1044,93 @@ int prepare_kernel(const void* fn, const char* name) {
                         "hipFuncSetAttribute");
 }
 
+// Per (kernel, device): the kernel's attributes are set once, and its
+// persistent grid is the workgroups of `block` lanes with `lds` bytes that
+// fit the device at once (occupancy x CUs), cached per LDS size.
+int kernel_grid(const void* fn, const char* name, int block, size_t lds, int* grid) {
+  static std::mutex mu;
+  static std::map<std::tuple<const void*, int, size_t>, int> grids;
+  static std::set<std::pair<const void*, int>> prepared;
+  int dev = 0;
+  int rc = qfx::check_hip(hipGetDevice(&dev), "hipGetDevice");
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!prepared.count({fn, dev})) {
+    rc = prepare_kernel(fn, name);
+    if (rc) return rc;
+    prepared.insert({fn, dev});
+  }
+  if (!grid) return QFX_OK;
+  auto key = std::make_tuple(fn, dev, lds);
+  auto it = grids.find(key);
+  if (it == grids.end()) {
+    int per_cu = 0, cus = 0;
+    rc = qfx::check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block, lds),
+                        "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+    if (rc) return rc;
+    rc = qfx::check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev),
+                        "hipDeviceGetAttribute");
+    if (rc) return rc;
+    it = grids.emplace(key, (per_cu > 0 ? per_cu : 1) * cus).first;
+  }
+  *grid = it->second;
+  return QFX_OK;
+}
+
 template <int BS, int LB, bool WIDE, bool MF>
 int launch_pass(hipStream_t s, v2d* state, int n, int T, const TilePos& pos,
                 const qfx_tile_op* ops, int nops, const v2d* coef, int ncoef, const int* srcs,
-                bool persistent, bool reg_io = false) {
+                bool persistent, bool reg_io, bool reg_in) {
   constexpr bool kStage = !WIDE && !MF;
   const size_t lds = sizeof(v2d) * ((size_t(1) << T) + (kStage ? ncoef : 0));
   // occupancy (persistent grid size) for the largest staged table
   const size_t lds_max = sizeof(v2d) * ((size_t(1) << T) + (kStage ? kMaxStaged : 0));
-  static bool attr_set = false;
-  static int occ[kMaxTileBits + 1];
-  if (!attr_set) {
-    int rc = prepare_kernel((const void*)tile_pass_kernel<BS, LB, WIDE, MF>, "tile_pass_kernel");
-    if (rc) return rc;
-    rc = prepare_kernel((const void*)tile_pipe_kernel<BS, LB, WIDE, MF>, "tile_pipe_kernel");
-    if (rc) return rc;
-    for (int t = 0; t <= kMaxTileBits; ++t) occ[t] = 0;
-    attr_set = true;
-  }
   const uint64_t tiles = (uint64_t)1 << (n - T);
+  int grid = 0;
+  int rc = QFX_OK;
   if constexpr (!WIDE && !MF && BS <= kRphaseMaxBlock) {
-    if (reg_io) {   // the caller checked T == LB + 4
-      static int occ_reg[kMaxTileBits + 1];
-      static bool reg_attr = false;
-      if (!reg_attr) {
-        int rc = prepare_kernel((const void*)tile_reg_kernel<BS, LB>, "tile_reg_kernel");
-        if (rc) return rc;
-        for (int t = 0; t <= kMaxTileBits; ++t) occ_reg[t] = 0;
-        reg_attr = true;
-      }
-      if (occ_reg[T] == 0) {
-        int per_cu = 0, dev = 0, cus = 0;
-        int rc = qfx::check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                                    &per_cu, (const void*)tile_reg_kernel<BS, LB>, BS, lds_max),
-                                "hipOccupancyMaxActiveBlocksPerMultiprocessor");
-        if (rc) return rc;
-        rc = qfx::check_hip(hipGetDevice(&dev), "hipGetDevice");
-        if (rc) return rc;
-        rc = qfx::check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev),
-                            "hipDeviceGetAttribute");
-        if (rc) return rc;
-        occ_reg[T] = (per_cu > 0 ? per_cu : 1) * cus;
-      }
-      const uint64_t grid = tiles < (uint64_t)occ_reg[T] ? tiles : (uint64_t)occ_reg[T];
-      hipLaunchKernelGGL((tile_reg_kernel<BS, LB>), dim3((unsigned)grid), dim3(BS), lds, s, state,
+    // the caller checked T == LB + 4 for these
+    if (reg_in) {
+      const int mode = !reg_io ? RIN_LDS_OUT : (nops == 1 ? RIN_SINGLE : RIN_REG_OUT);
+      const void* fn = mode == RIN_LDS_OUT   ? (const void*)tile_rin_kernel<BS, LB, RIN_LDS_OUT>
+                       : mode == RIN_SINGLE ? (const void*)tile_rin_kernel<BS, LB, RIN_SINGLE>
+                                            : (const void*)tile_rin_kernel<BS, LB, RIN_REG_OUT>;
+      rc = kernel_grid(fn, "tile_rin_kernel", BS, lds_max, &grid);
+      if (rc) return rc;
+      const uint64_t g = tiles < (uint64_t)grid ? tiles : (uint64_t)grid;
+      const uint32_t nt = (uint32_t)tiles;
+      if (mode == RIN_LDS_OUT)
+        hipLaunchKernelGGL((tile_rin_kernel<BS, LB, RIN_LDS_OUT>), dim3((unsigned)g), dim3(BS),
+                           lds, s, state, T, pos, ops, nops, coef, ncoef, srcs, nt);
+      else if (mode == RIN_SINGLE)
+        hipLaunchKernelGGL((tile_rin_kernel<BS, LB, RIN_SINGLE>), dim3((unsigned)g), dim3(BS),
+                           lds, s, state, T, pos, ops, nops, coef, ncoef, srcs, nt);
+      else
+        hipLaunchKernelGGL((tile_rin_kernel<BS, LB, RIN_REG_OUT>), dim3((unsigned)g), dim3(BS),
+                           lds, s, state, T, pos, ops, nops, coef, ncoef, srcs, nt);
+      return qfx::check_launch("tile_rin_kernel");
+    }
+    if (reg_io) {
+      rc = kernel_grid((const void*)tile_reg_kernel<BS, LB>, "tile_reg_kernel", BS, lds_max, &grid);
+      if (rc) return rc;
+      const uint64_t g = tiles < (uint64_t)grid ? tiles : (uint64_t)grid;
+      hipLaunchKernelGGL((tile_reg_kernel<BS, LB>), dim3((unsigned)g), dim3(BS), lds, s, state,
                          T, pos, ops, nops, coef, ncoef, srcs, (uint32_t)tiles);
       return qfx::check_launch("tile_reg_kernel");
     }
   }
   if (persistent && BS <= 512 && T - LB == 4) {
-    if (occ[T] == 0) {
-      int per_cu = 0, dev = 0, cus = 0;
-      int rc = qfx::check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                                  &per_cu, (const void*)tile_pipe_kernel<BS, LB, WIDE, MF>, BS, lds_max),
-                              "hipOccupancyMaxActiveBlocksPerMultiprocessor");
-      if (rc) return rc;
-      rc = qfx::check_hip(hipGetDevice(&dev), "hipGetDevice");
-      if (rc) return rc;
-      rc = qfx::check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev),
-                          "hipDeviceGetAttribute");
-      if (rc) return rc;
-      occ[T] = (per_cu > 0 ? per_cu : 1) * cus;
-    }
-    const uint64_t grid = tiles < (uint64_t)occ[T] ? tiles : (uint64_t)occ[T];
-    hipLaunchKernelGGL((tile_pipe_kernel<BS, LB, WIDE, MF>), dim3((unsigned)grid), dim3(BS), lds, s,
+    rc = kernel_grid((const void*)tile_pipe_kernel<BS, LB, WIDE, MF>, "tile_pipe_kernel", BS,
+                     lds_max, &grid);
+    if (rc) return rc;
+    const uint64_t g = tiles < (uint64_t)grid ? tiles : (uint64_t)grid;
+    hipLaunchKernelGGL((tile_pipe_kernel<BS, LB, WIDE, MF>), dim3((unsigned)g), dim3(BS), lds, s,
                        state, T, pos, ops, nops, coef, ncoef, srcs, (uint32_t)tiles);
     return qfx::check_launch("tile_pipe_kernel");
   }
+  rc = kernel_grid((const void*)tile_pass_kernel<BS, LB, WIDE, MF>, "tile_pass_kernel", BS, lds,
+                   nullptr);
+  if (rc) return rc;
   hipLaunchKernelGGL((tile_pass_kernel<BS, LB, WIDE, MF>), dim3((unsigned)tiles), dim3(BS), lds, s, state, T,
                      pos, ops, nops, coef, ncoef, srcs);
   return qfx::check_launch("tile_pass_kernel");
@@ -1022,8 +1177,16 @@ int qfx_tile_prepare_ops(qfx_tile_op* ops, int nops) {
 
 int qfx_tile_check_ops(int T, int block, int flags, const qfx_tile_op* ops, int nops,
                        int ncoef, const int* srcs, int nsrcs) {
-  if (flags & ~(QFX_TILE_WIDE | QFX_TILE_PERSISTENT | QFX_TILE_MFMA | QFX_TILE_REG_IO))
+  if (flags & ~(QFX_TILE_WIDE | QFX_TILE_PERSISTENT | QFX_TILE_MFMA | QFX_TILE_REG_IO |
+                QFX_TILE_REG_IN))
     return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: unknown flags");
+  if (flags & QFX_TILE_REG_IN) {
+    if ((flags & (QFX_TILE_WIDE | QFX_TILE_MFMA)) || block > kRphaseMaxBlock ||
+        (1 << T) != block * kRegs || nops < 1 || !ops || ops[0].kind != OP_RPHASE)
+      return qfx::set_error(QFX_EINVAL,
+                            "qfx_tile_check_ops: QFX_TILE_REG_IN needs a first register phase, "
+                            "16 amplitudes per lane, block <= 512");
+  }
   if (flags & QFX_TILE_REG_IO) {
     if ((flags & (QFX_TILE_WIDE | QFX_TILE_MFMA)) || block > kRphaseMaxBlock ||
         (1 << T) != block * kRegs || nops < 1 || !ops || ops[nops - 1].kind != OP_RPHASE)
@@ -1167,25 +1330,37 @@ int qfx_tile_check_ops(int T, int block, int flags, const qfx_tile_op* ops, int 
 }
 
 int qfx_tile_pass(void* stream, void* state, int n, int tile_bits, const int* tile_pos,
-                  const int* store_pos, const void* ops, int nops, const void* coef, int ncoef,
+                  const int* dest, const void* ops, int nops, const void* coef, int ncoef,
                   const void* srcs, int block, int flags) {
   int rc = check_pass(n, tile_bits, tile_pos, nops, block, "qfx_tile_pass");
   if (rc) return rc;
-  if (store_pos) {
-    for (int j = 0; j < tile_bits; ++j) {
-      int hits = 0;
-      for (int i = 0; i < tile_bits; ++i) hits += store_pos[i] == tile_pos[j];
-      if (hits != 1)
+  if (dest) {
+    // in place: tile bits permute among the tile positions, the others stay
+    uint64_t tmask = 0, seen = 0;
+    for (int j = 0; j < tile_bits; ++j) tmask |= (uint64_t)1 << tile_pos[j];
+    for (int b = 0; b < n; ++b) {
+      const bool tb = (tmask >> b) & 1u;
+      if (dest[b] < 0 || dest[b] >= n || ((seen >> dest[b]) & 1u) ||
+          (tb ? !((tmask >> dest[b]) & 1u) : dest[b] != b))
         return qfx::set_error(QFX_EINVAL,
-                              "qfx_tile_pass: store positions must permute the tile positions");
+                              "qfx_tile_pass: dest must permute the tile positions among "
+                              "themselves and keep every other index bit (the pass is in place)");
+      seen |= (uint64_t)1 << dest[b];
     }
   }
   const bool wide = (flags & QFX_TILE_WIDE) != 0;
   const bool pers = (flags & QFX_TILE_PERSISTENT) != 0;
   const bool mf = (flags & QFX_TILE_MFMA) != 0;
   const bool reg_io = (flags & QFX_TILE_REG_IO) != 0;
-  if (flags & ~(QFX_TILE_WIDE | QFX_TILE_PERSISTENT | QFX_TILE_MFMA | QFX_TILE_REG_IO))
+  const bool reg_in = (flags & QFX_TILE_REG_IN) != 0;
+  if (flags & ~(QFX_TILE_WIDE | QFX_TILE_PERSISTENT | QFX_TILE_MFMA | QFX_TILE_REG_IO |
+                QFX_TILE_REG_IN))
     return qfx::set_error(QFX_EINVAL, "qfx_tile_pass: unknown flags");
+  if (reg_in && (wide || mf || block > kRphaseMaxBlock || nops < 1 ||
+                 (1 << tile_bits) != block * kRegs))
+    return qfx::set_error(QFX_EINVAL,
+                          "qfx_tile_pass: QFX_TILE_REG_IN needs 16 amplitudes per lane, "
+                          "block <= 512, no wide ops or matrix-core blocks");
   if (reg_io && (wide || mf || block > kRphaseMaxBlock || nops < 1 ||
                  (1 << tile_bits) != block * kRegs))
     return qfx::set_error(QFX_EINVAL,
@@ -1204,7 +1379,8 @@ int qfx_tile_pass(void* stream, void* state, int n, int tile_bits, const int* ti
   std::memset(&pos, 0, sizeof(pos));
   for (int j = 0; j < tile_bits; ++j) {
     pos.s[j] = tile_pos[j];
-    pos.d[j] = store_pos ? store_pos[j] : tile_pos[j];
+    pos.d[j] = dest ? dest[tile_pos[j]] : tile_pos[j];
+    pos.permuted |= pos.d[j] != pos.s[j];
   }
   hipStream_t s = qfx::as_stream(stream);
   auto* st = static_cast<v2d*>(state);
@@ -1212,7 +1388,7 @@ int qfx_tile_pass(void* stream, void* state, int n, int tile_bits, const int* ti
   auto* c = static_cast<const v2d*>(coef);
   auto* sr = static_cast<const int*>(srcs);
 #define QFX_PASS(BS, LB, W, M) \
-  launch_pass<BS, LB, W, M>(s, st, n, tile_bits, pos, o, nops, c, ncoef, sr, pers, reg_io)
+  launch_pass<BS, LB, W, M>(s, st, n, tile_bits, pos, o, nops, c, ncoef, sr, pers, reg_io, reg_in)
   switch (block) {
     case 64:
       if (mf) return wide ? QFX_PASS(64, 6, true, true) : QFX_PASS(64, 6, false, true);

@@ -47,6 +47,7 @@ TILE_WIDE = 1
 TILE_PERSISTENT = 2
 TILE_MFMA = 4
 TILE_REG_IO = 8
+TILE_REG_IN = 16
 MFMA_MIN_BITS = 3
 
 
@@ -890,7 +891,7 @@ class HostPlan:
     index their pass's coefficient table."""
 
     def __init__(self, gates, qubits, tile_bits=MAX_TILE_BITS, low_bits=3, block=256,
-                 fuse_width=4, mode='reg', persistent=True, reg_io=True):
+                 fuse_width=4, mode='reg', persistent=True, reg_io=True, reg_in=True):
         qubits = tuple(qubits)
         n = len(qubits)
         self.n = n
@@ -909,6 +910,7 @@ class HostPlan:
         self.fuse_width = fuse_width
         self.persistent = persistent
         self.reg_io = reg_io
+        self.reg_in = reg_in
         self.gates = list(gates)
         bits_list = [tuple(n - 1 - qubits.index(q) for q in g.qubits) for g in gates]
         tileable = [len(b) <= 5 and len(b) <= T for b in bits_list]
@@ -929,13 +931,15 @@ class HostPlan:
         ops_host, coefs, srcs = [], [], []
         self.n_phases = 0
         self.n_blocks = 0
+        orders, holders = [], []   # per tile pass: tile bits, qubit each holds at the end
         for pos, members in planned:
             if pos is None:
                 g = gates[members[0]]
                 self.passes.append(('gate', (list(bits_list[members[0]]), g.kind)))
                 continue
-            pos_index = {b: j for j, b in enumerate(pos)}
             tset = set(pos)
+            pos = sorted(pos)
+            pos_index = {b: j for j, b in enumerate(pos)}
             ops = []
             for i in members:
                 bits = bits_list[i]
@@ -989,6 +993,12 @@ class HostPlan:
                         and (1 << T) == 16 * block and block <= 512
                         and min(last.t[j] for j in range(REG_BITS)) >= min(low_bits, 3)):
                     flags = TILE_REG_IO
+                # likewise a first register phase loads straight from HBM
+                first = ops_host[start] if len(ops_host) > start else None
+                if (first is not None and persistent and self.reg_in and first.kind == OP_RPHASE
+                        and (1 << T) == 16 * block and block <= 512
+                        and min(first.t[j] for j in range(REG_BITS)) >= min(low_bits, 3)):
+                    flags |= TILE_REG_IN
             else:
                 block = min(self.block, 256) if wide else self.block
                 width = self.fuse_width if block <= 256 else min(self.fuse_width, 3)
@@ -1000,8 +1010,24 @@ class HostPlan:
                     block = min(block, 256)
             if persistent:
                 flags |= TILE_PERSISTENT
-            self._add_tile_passes(pos, ops_host, start, coefs, c_lo, srcs, block, flags,
-                                  store_pos)
+            orders.append(pos)
+            holders.append({b: (store_pos[j] if store_pos is not None else b)
+                            for j, b in enumerate(pos)})
+            self._add_tile_passes(len(orders) - 1, ops_host, start, coefs, c_lo, srcs, block,
+                                  flags)
+        # each tile pass's store permutation: the last launch of a pass
+        # stores every tile bit to the index bit of the qubit it then holds
+        for i, (kind, payload) in enumerate(self.passes):
+            if kind != 'tile':
+                continue
+            k, start, nops, block, flags, c_lo, ncoef, last = payload
+            hold = holders[k]
+            dest = None
+            if last and any(hold[b] != b for b in orders[k]):
+                dest = list(range(n))
+                for b in orders[k]:
+                    dest[b] = hold[b]
+            self.passes[i] = ('tile', (orders[k], start, nops, block, flags, c_lo, ncoef, dest))
         self.n_passes = len(self.passes)
         self.n_ops = len(ops_host)
         self.ops_host, self.coefs, self.srcs = ops_host, coefs, srcs
@@ -1017,18 +1043,20 @@ class HostPlan:
             _native.check(lib.qfx_tile_check_ops(T, block, flags, arr, nops, ncoef,
                                                  src_arr, len(srcs)), 'qfx_tile_check_ops')
 
-    def _add_tile_passes(self, pos, ops_host, start, coefs, c_lo, srcs, block, flags,
-                         store_pos=None):
-        """Record the pass of ops_host[start:] with coefficient indices made
-        relative to the pass's table; a pass that would stage more than
-        MAX_STAGED coefficients is cut into consecutive passes on the same
-        tile bits (only the last one stores through store_pos)."""
+    def _add_tile_passes(self, k, ops_host, start, coefs, c_lo, srcs, block, flags):
+        """Record tile pass k (ops_host[start:]) with coefficient indices
+        made relative to the pass's table; a pass that would stage more than
+        MAX_STAGED coefficients is cut into consecutive launches on the same
+        tile bits, of which only the last stores to the next bit order (the
+        payload's last field).  The load positions and store permutations are
+        filled in once every pass's bit order is known."""
         ops = ops_host[start:]
         if not ops:
-            # only absorbed exchanges: the pass is their store permutation
-            if store_pos is not None:
-                self.passes.append(('tile', (pos, start, 0, block, flags & ~TILE_REG_IO, c_lo,
-                                             0, store_pos)))
+            # only absorbed exchanges (or a change of bit order): the pass is
+            # its store permutation
+            self.passes.append(('tile', (k, start, 0, block,
+                                         flags & ~(TILE_REG_IO | TILE_REG_IN), c_lo,
+                                         0, True)))
             return
         bounds = [_op_coef_start(o, srcs) for o in ops] + [len(coefs)]
         # an op without coefficients starts where the next op starts
@@ -1043,12 +1071,13 @@ class HostPlan:
                 b += 1
             for o in ops[a:b]:
                 _rebase_coef(o, srcs, bounds[a])
-            spos = store_pos if b == len(ops) else None
             fl = flags
             if fl & TILE_REG_IO and ops[b - 1].kind != OP_RPHASE:
                 fl &= ~TILE_REG_IO
-            self.passes.append(('tile', (pos, start + a, b - a, block, fl, bounds[a],
-                                         bounds[b] - bounds[a], spos)))
+            if fl & TILE_REG_IN and (a > 0 or ops[a].kind != OP_RPHASE):
+                fl &= ~TILE_REG_IN
+            self.passes.append(('tile', (k, start + a, b - a, block, fl, bounds[a],
+                                         bounds[b] - bounds[a], b == len(ops))))
             a = b
 
     def stats(self):

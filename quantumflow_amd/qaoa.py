@@ -125,3 +125,37 @@ def plain_value_and_grad(graph: nx.Graph, beta: Sequence, gamma: Sequence):
         bk.apply_gate(lam, lam, n, bits, inv, inplace_sparse=True)
         pos -= 1
     return value, grad
+
+
+def autograd_value_and_grad(graph: nx.Graph, beta: Sequence, gamma: Sequence):
+    """<C> and its gradient the way the reference gets it
+    (examples/qaoa_maxcut.py:53-64, TensorFlow autograd over the gate-level
+    circuit): every gate of ``qubo_circuit`` after the initial layer of H is
+    a torch complex128 operation on the device -- ZZ(-w gamma/pi) on (u, v)
+    multiplies by exp(i w gamma z_u z_v / 2), RX(beta) updates the amplitude
+    pairs of its axis -- and autograd keeps what each gate's backward needs
+    (the state entering it), so memory grows with the gate count.  The
+    stored-intermediates comparator of the adjoint engine; unit or 'weight'
+    edge weights, no node weights.  Returns (value, grad [2p]) as device
+    tensors."""
+    p = len(beta)
+    n = len(graph)
+    dev = bk.DEVICE
+    b = torch.tensor(np.asarray(beta, dtype=np.float64), device=dev, requires_grad=True)
+    g = torch.tensor(np.asarray(gamma, dtype=np.float64), device=dev, requires_grad=True)
+    cuts = graph_cuts(graph).reshape(-1)
+    idx = torch.arange(2 ** n, device=dev)
+    psi = torch.full((2 ** n,), 2.0 ** (-n / 2), dtype=torch.complex128, device=dev)
+    for layer in range(p):
+        for u, v in graph.edges():
+            w = float(graph[u][v].get('weight', 1.0))
+            zz = 1.0 - 2.0 * (((idx >> (n - 1 - u)) ^ (idx >> (n - 1 - v))) & 1).to(torch.float64)
+            psi = psi * torch.exp(1j * (0.5 * w) * g[layer] * zz)
+        c, s = torch.cos(b[layer] / 2), torch.sin(b[layer] / 2)
+        for q in range(n):
+            t = psi.reshape(2 ** q, 2, 2 ** (n - q - 1))
+            a0, a1 = t[:, 0, :], t[:, 1, :]
+            psi = torch.stack([c * a0 - 1j * s * a1, c * a1 - 1j * s * a0], dim=1).reshape(-1)
+    value = torch.sum((psi.real ** 2 + psi.imag ** 2) * cuts)
+    value.backward()
+    return value.detach(), torch.cat([b.grad, g.grad])

@@ -69,7 +69,8 @@ class QaoaArgs(ctypes.Structure):
                  'cnt_pool', 'val_pool', 'partials', 'out', 'sync')] + \
                [(nm, ctypes.c_int32) for nm in
                 ('ninst', 'nk', 'width', 'mode', 'block', 'max_items_per_k', 'tile_bits',
-                 'pad')]
+                 'pad')] + \
+               [('timing', ctypes.c_void_p), ('timing_len', ctypes.c_int64)]
 
 
 def tile_sets(n, T, low=None):
@@ -416,6 +417,43 @@ class QAOABatch:
         torch.cuda.synchronize()
         _native.call('qfx_qaoa_status', ctypes.c_void_p(self.sync.data_ptr()), ctypes.byref(failed))
         raise_if_failed(failed.value)
+
+    def pass_times(self, max_grid=4096):
+        """One launch with the kernel's wall-clock stamps on.  Per global pass
+        k: its items, its span (first workgroup start to last arrival at the
+        barrier after it) and the mean and largest time a workgroup spent on
+        its items, in microseconds (device wall clock, qfx_init's rate).
+        The gap between span and mean busy time is barrier wait and load
+        imbalance."""
+        dev = bk.DEVICE
+        khz = bk.device_props()[7]
+        if khz <= 0:
+            raise RuntimeError('no device wall-clock rate')
+        buf = torch.zeros(2 * self.nk * max_grid, dtype=torch.int64, device=dev)
+        a = self._args
+        a.timing, a.timing_len = buf.data_ptr(), buf.numel()
+        try:
+            self.launch()
+            torch.cuda.synchronize()
+        finally:
+            a.timing, a.timing_len = None, 0
+        self.check()
+        flat = buf.cpu().numpy().astype(np.int64)
+        # entries [k][workgroup][start, end] for the launch's grid
+        grid = int(np.count_nonzero(flat)) // (2 * self.nk)
+        st = flat[:2 * self.nk * grid].reshape(self.nk, grid, 2)
+        us = 1e3 / khz
+        ks = self._kstart.cpu().numpy()
+        out = []
+        for k in range(self.nk):
+            s0, s1 = st[k, :, 0], st[k, :, 1]
+            busy = (s1 - s0) * us
+            out.append({'pass': k, 'items': int(ks[k + 1] - ks[k]),
+                        'span_us': round(float(s1.max() - s0.min()) * us, 2),
+                        'busy_mean_us': round(float(np.mean(busy)), 2),
+                        'busy_max_us': round(float(np.max(busy)), 2)})
+        total = float(st[:, :, 1].max() - st[0, :, 0].min()) * us
+        return {'grid': grid, 'total_us': round(total, 2), 'passes': out}
 
     def evaluate(self, beta, gamma, graph=False):
         """(values [ninst], grads [ninst, 2p] or None) as device tensors."""

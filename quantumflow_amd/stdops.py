@@ -12,8 +12,9 @@ control and classical-memory operations the reference's own measurement tests
 use (stdops.py:129-170, :279-282).
 """
 
+import operator
 from math import sqrt
-from typing import Any, Hashable
+from typing import Any, Callable, Hashable
 
 import numpy as np
 
@@ -22,7 +23,8 @@ from .ops import Operation
 from .qubits import Qubit, Qubits
 from .states import State
 
-__all__ = ['Measure', 'Reset', 'Barrier', 'If', 'Move']
+__all__ = ['Measure', 'Reset', 'Barrier', 'If', 'Move', 'Neg', 'Not', 'And', 'Ior', 'Or',
+           'Xor', 'Add', 'Sub', 'Mul', 'Div', 'Exchange', 'EQ', 'GT', 'GE', 'LT', 'LE']
 
 _RESET = np.array([[1, 1], [0, 0]], dtype=np.complex128)
 
@@ -169,3 +171,155 @@ class Move(Operation):
     def run(self, ket: State) -> State:
         return ket.update({self.target: self._value(ket)})
 
+
+
+def _is_number(x) -> bool:
+    return isinstance(x, (bool, int, float, complex, np.number))
+
+
+class Neg(Operation):
+    """Negate a value in classical memory (stdops.py:173-182)."""
+
+    def __init__(self, target: Hashable) -> None:
+        self.target = target
+
+    def quil(self) -> str:
+        return '{} {}'.format(self.name.upper(), self.target)
+
+    def run(self, ket: State) -> State:
+        return ket.update({self.target: -ket.memory[self.target]})
+
+    def evolve(self, rho):
+        return self.run(rho)
+
+
+class Not(Operation):
+    """Logical NOT of a classical bit (stdops.py:185-194)."""
+
+    def __init__(self, target: Hashable) -> None:
+        self.target = target
+
+    def quil(self) -> str:
+        return '{} {}'.format(self.name.upper(), self.target)
+
+    def run(self, ket: State) -> State:
+        return ket.update({self.target: int(not ket.memory[self.target])})
+
+    def evolve(self, rho):
+        return self.run(rho)
+
+
+class _BinaryOp(Operation):
+    """target <- op(target, source), source an address or a constant
+    (stdops.py:199-229)."""
+    _op: Callable = None
+
+    def __init__(self, target: Hashable, source: Any) -> None:
+        self.target = target
+        self.source = source
+
+    def _source(self, ket):
+        return self.source if _is_number(self.source) else ket.memory[self.source]
+
+    def quil(self) -> str:
+        return '{} {} {}'.format(self.name.upper(), self.target, self.source)
+
+    def run(self, ket: State) -> State:
+        return ket.update({self.target: type(self)._op(ket.memory[self.target],
+                                                       self._source(ket))})
+
+    def evolve(self, rho):
+        return self.run(rho)
+
+
+class And(_BinaryOp):
+    """Logical AND into the target (stdops.py:232-234)."""
+    _op = operator.and_
+
+
+class Ior(_BinaryOp):
+    """Inclusive OR into the target (stdops.py:237-240)."""
+    _op = operator.or_
+
+
+class Or(_BinaryOp):
+    """Inclusive OR (Quil's deprecated name for IOR, stdops.py:243-246)."""
+    _op = operator.or_
+
+
+class Xor(_BinaryOp):
+    """Exclusive OR into the target (stdops.py:249-253)."""
+    _op = operator.xor
+
+
+class Add(_BinaryOp):
+    """target + source (stdops.py:256-258)."""
+    _op = operator.add
+
+
+class Sub(_BinaryOp):
+    """target - source (stdops.py:261-263)."""
+    _op = operator.sub
+
+
+class Mul(_BinaryOp):
+    """target * source (stdops.py:266-268)."""
+    _op = operator.mul
+
+
+class Div(_BinaryOp):
+    """target / source, true division (stdops.py:271-273)."""
+    _op = operator.truediv
+
+
+class Exchange(_BinaryOp):
+    """Swap two classical cells (stdops.py:285-290)."""
+
+    def run(self, ket: State) -> State:
+        mem = ket.memory
+        return ket.update({self.target: mem[self.source], self.source: mem[self.target]})
+
+
+class _Comparison(Operation):
+    """target <- op(left, right) on two addresses (stdops.py:295-311)."""
+    _op: Callable = None
+
+    def __init__(self, target: Hashable, left: Hashable, right: Hashable) -> None:
+        self.target = target
+        self.left = left
+        self.right = right
+
+    def quil(self) -> str:
+        return '{} {} {} {}'.format(self.name, self.target, self.left, self.right)
+
+    def run(self, ket: State) -> State:
+        mem = ket.memory
+        return ket.update({self.target: type(self)._op(mem[self.left], mem[self.right])})
+
+    def evolve(self, rho):
+        return self.run(rho)
+
+
+class EQ(_Comparison):
+    """left == right (stdops.py:314-316)."""
+    _op = operator.eq
+
+
+class GT(_Comparison):
+    """left > right (stdops.py:319-321)."""
+    _op = operator.gt
+
+
+class GE(_Comparison):
+    """left >= right (stdops.py:324-326)."""
+    _op = operator.ge
+
+
+class LT(_Comparison):
+    """left < right (stdops.py:329-331)."""
+    _op = operator.lt
+
+
+class LE(_Comparison):
+    """left <= right (stdops.py:334-336)."""
+    _op = operator.le

@@ -96,3 +96,16 @@ def regular_instances(count=64, n_lo=16, n_hi=24, p=8, seed=0, degree=3):
         beta = rng.uniform(0, np.pi / 2, size=p)
         out.append((g, beta, gamma))
     return out
+
+
+def gops_circuit(n=16, gates=100, seed=0):
+    """The gate-operations-per-second workload (tools/benchmark.py:31-46):
+    H on every qubit, then (gates - n) // 3 triples X(q0), T(q1),
+    CNOT(q0, q1) on a random pair of distinct qubits.  The reference draws
+    the pairs with Python's random.sample; here a seeded numpy generator."""
+    rng = np.random.default_rng(seed)
+    out = [('H', (), (q,)) for q in range(n)]
+    for _ in range((gates - n) // 3):
+        q0, q1 = (int(q) for q in rng.choice(n, size=2, replace=False))
+        out += [('X', (), (q0,)), ('T', (), (q1,)), ('CNOT', (), (q0, q1))]
+    return out

@@ -189,3 +189,19 @@ def test_channel_chi_h_permute(qf):
     sw = O.gate_matrix('SWAP')
     up = sw @ u @ sw
     assert np.allclose(perm.asoperator(), np.kron(up, up.conj()), atol=1e-14, rtol=0)
+
+
+def test_program_quil_text(qf):
+    # reference tests/test_programs.py:95-99, 153-157 and the DECLARE form
+    prog = qf.Program([qf.Call('BELL', params=[], qubits=[])])
+    assert str(prog) == 'BELL\n'
+    assert qf.Include('somefile.quil', qf.Program()).quil() == 'INCLUDE "somefile.quil"'
+    assert qf.Declare('ro', 'BIT', 2).quil() == 'DECLARE ro BIT [2]'
+    assert qf.Call('RX', [0.5], [3]).quil() == 'RX(0.5) 3'
+    assert qf.JumpWhen('a', qf.Register()[0]).quil() == 'JUMP-WHEN @a ro[0]'
+    assert qf.Program([qf.Label('x'), qf.Jump('x')]).quil() == 'LABEL @x\nJUMP @x\n'
+    r = qf.Register('r', 'INT')
+    assert r[1] == qf.Register('r', 'INT')[1] and hash(r[1]) == hash(qf.Register('r', 'INT')[1])
+    assert r[0] < r[1] and str(r[1]) == 'r[1]'
+    with pytest.raises(AssertionError):
+        qf.Register('x', 'FLOAT')

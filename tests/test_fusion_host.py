@@ -117,14 +117,15 @@ def unpack_mfma(slots, k):
     return r[:d, :d] + 1j * r[d:, :d]
 
 
-def apply_store(ket, n, pos, spos):
-    """A pass's store relabelling: tile bit pos[j] is written to index bit
-    spos[j] (None: unchanged)."""
-    if spos is None:
+def apply_store(ket, n, pos, dest):
+    """A pass's store permutation: index bit b is written to index bit
+    dest[b] (None: unchanged)."""
+    if dest is None:
         return ket
+    assert sorted(dest) == list(range(n))
     axes = list(range(n))
-    for p, q in zip(pos, spos):
-        axes[n - 1 - q] = n - 1 - p
+    for b in range(n):
+        axes[n - 1 - dest[b]] = n - 1 - b
     return np.transpose(ket, axes)
 
 
@@ -509,7 +510,11 @@ def test_pass_level_swap_relabel():
     for kind, payload in plan.passes:
         pos, start, nops, block, flags, c_lo, ncoef, spos = payload
         if spos is not None:
-            assert sorted(spos) == sorted(pos) and all(a == b for a, b in zip(pos, spos) if a < 3)
+            # tile bits permute among themselves, the others and the three
+            # lowest index bits (the coalesced runs) never move
+            assert sorted(spos) == list(range(n)) and all(spos[b] == b for b in range(3))
+            assert all(spos[b] == b for b in range(n) if b not in pos)
+            assert sorted(spos[b] for b in pos) == sorted(pos)
         ket = _emulate_ops(ket, n, pos, plan.ops_host[start:start + nops],
                            plan.coefs[c_lo:c_lo + ncoef], plan.srcs)
         ket = apply_store(ket, n, pos, spos)

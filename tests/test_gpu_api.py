@@ -160,6 +160,34 @@ def test_benchmark_circuit_20q(qf, mixed20, mode, width):
     assert err <= 1e-12
 
 
+@pytest.mark.parametrize("reg_in,reg_io", [(False, False), (True, False), (False, True),
+                                            (True, True)])
+def test_benchmark_circuit_20q_register_io(qf, mixed20, reg_in, reg_io):
+    """Passes whose first register phase loads straight from HBM and/or
+    whose last stores straight to HBM against the oracle (<= 1e-12) and,
+    bit for bit, against the plan that moves every tile through LDS (the
+    same arithmetic in the same order)."""
+    from quantumflow_amd import fusion
+    from quantumflow_amd.fusion import TilePlan
+    from quantumflow_amd.workloads import build_gates
+    desc, ref = mixed20
+    gates = build_gates(desc)
+    kw = dict(tile_bits=12, low_bits=3, block=256, fuse_width=1, mode='reg')
+    plan = TilePlan(gates, tuple(range(20)), reg_in=reg_in, reg_io=reg_io, **kw)
+    flags = [p[1][4] for p in plan.passes if p[0] == 'tile']
+    n_in = sum(bool(f & fusion.TILE_REG_IN) for f in flags)
+    n_io = sum(bool(f & fusion.TILE_REG_IO) for f in flags)
+    assert (n_in > 0) == reg_in and (n_io > 0) == reg_io
+    got = plan.run(qf.zero_state(20)).vec.asarray()
+    base = TilePlan(gates, tuple(range(20)), reg_in=False, reg_io=False, **kw)
+    want = base.run(qf.zero_state(20)).vec.asarray()
+    err = float(np.max(np.abs(got - ref)))
+    print("20q register I/O passes: in %d, out %d of %d, max |d amp| %.3g"
+          % (n_in, n_io, len(flags), err), flush=True)
+    assert err <= 1e-12
+    assert np.array_equal(got, want)
+
+
 def test_w16_kat(qf):
     # examples/state_prep_w16.py via tests/test_examples.py:29-32
     from tests.test_oracle import w16_gates

@@ -94,10 +94,11 @@ def test_device_tensor_without_class_runs_dense(bk):
         assert bk.rank(t) == 2 * k
         psi = rng.normal(size=[2] * n) + 1j * rng.normal(size=[2] * n)
         idx = list(rng.choice(n, size=k, replace=False))
+        psi_dev = bk.astensorproduct(psi)          # the host-to-device copy syncs
         torch.cuda.synchronize()
         torch.cuda.set_sync_debug_mode("error")
         try:
-            out = bk.tensormul(t, bk.astensorproduct(psi), idx)
+            out = bk.tensormul(t, psi_dev, idx)
         finally:
             torch.cuda.set_sync_debug_mode(0)
         ref = O.tensormul(O.astensorproduct(m), psi, idx)

@@ -122,9 +122,14 @@ def test_density_entry_points_reject_bad_sizes(qf):
     buf = torch.zeros(4, dtype=torch.complex128, device=qf.backend.DEVICE)
     s = qf.backend.stream_handle()
     ptr = _native.ctypes.c_void_p(buf.data_ptr())
-    assert lib.qfx_density_identity(s, ptr, 0, 1.0) == _native.QFX_ERANGE
+    assert lib.qfx_density_identity(s, ptr, -1, 1.0) == _native.QFX_ERANGE
     assert lib.qfx_density_trace(s, ptr, ptr, 16) == _native.QFX_ERANGE
     assert lib.qfx_density_outer(s, None, ptr, 1) == _native.QFX_EINVAL
+    assert lib.qfx_density_partial_trace(s, ptr, ptr, 0, 1) == _native.QFX_ERANGE
+    # a 0-qubit density is the 1 x 1 matrix of a scalar state
+    assert lib.qfx_density_identity(s, ptr, 0, 0.5) == 0
+    torch.cuda.synchronize()
+    assert complex(buf[0].item()) == 0.5
 
 
 def test_measure_reset_barrier_if_on_densities(qf):
