@@ -122,6 +122,16 @@ int qfx_qubit_probs(void* stream, void* out, const void* psi, int n, int axis,
 int qfx_norm2(void* stream, void* result, const void* psi, int64_t len,
               void* workspace);
 
+/* qfx_expect_diag without a workspace argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  Its block partials live in a 16 KiB array of the library's
+ * own device memory (one copy per GPU, held until the process ends), so it
+ * calls no allocator; len <= 2^34.  Calls that overlap on one GPU share
+ * that array and race: order them on one stream. */
+int qfx_expect_diag_noalloc(void* stream, void* result, const void* psi,
+                            const void* diag, int64_t len);
+
 /* ---- states ------------------------------------------------------------ */
 
 /* Qubit relabelling (QubitVector.permute, quantumflow/qubits.py:145-161):

@@ -58,6 +58,7 @@ SIGNATURES = {
     'qfx_abs': (_i, [_vp, _vp, _vp, _i64]),
     'qfx_abs2': (_i, [_vp, _vp, _vp, _i64]),
     'qfx_expect_diag': (_i, [_vp, _vp, _vp, _vp, _i64, _vp]),
+    'qfx_expect_diag_noalloc': (_i, [_vp, _vp, _vp, _vp, _i64]),
     'qfx_expect_z': (_i, [_vp, _vp, _vp, _i, _vp]),
     'qfx_norm2': (_i, [_vp, _vp, _vp, _i64, _vp]),
     'qfx_qubit_probs': (_i, [_vp, _vp, _vp, _i, _i, _vp]),

@@ -7,6 +7,9 @@
 // (quantumflow/qubits.py:180-182).  Reductions run in two stages with a fixed
 // work assignment (block partials, then one block in a fixed order), so a
 // result is bitwise reproducible for a given length.
+#include <mutex>
+#include <string>
+
 #include "qfx_common.h"
 
 namespace {
@@ -226,7 +229,61 @@ int run_reduce(hipStream_t s, void* result, const void* a, const void* b, int64_
 
 }  // namespace
 
+// ---- expectation without a caller workspace --------------------------------
+// The block partials of qfx_expect_diag_noalloc live in this file-scope
+// device array (16 KiB: one double2 per stage-1 block, at most
+// kMaxRedBlocks), which every GPU holds once for the life of the process.
+// Host code never passes the array's name to a kernel: in host code it is a
+// stand-in, not a device address.  Its address on each device comes from
+// hipGetSymbolAddress, taken once per device with that device current.
+// Every slot stage 2 reads is written by stage 1 of the same call, so the
+// array needs no clearing; calls that overlap on one GPU share it and race.
+static __device__ __align__(256) double2 scratch[kMaxRedBlocks];
+
+namespace {
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+
+int scratch_on_current_device(void** out, const char* fn) {
+  int dev = 0;
+  int rc = qfx::check_hip(hipGetDevice(&dev), "hipGetDevice");
+  if (rc) return rc;
+  if (dev < 0 || dev >= kMaxScratchDevices)
+    return qfx::set_error(QFX_ERANGE, std::string(fn) + ": device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    rc = qfx::check_hip(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress");
+    if (rc) return rc;
+    if (!p) return qfx::set_error(QFX_EHIP, std::string(fn) + ": no address for scratch");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return QFX_OK;
+}
+
+}  // namespace
+
 extern "C" {
+
+int qfx_expect_diag_noalloc(void* stream, void* result, const void* psi, const void* diag,
+                            int64_t len) {
+  if (len > ((int64_t)1 << QFX_MAX_QUBITS))
+    return qfx::set_error(QFX_ERANGE,
+                          "qfx_expect_diag_noalloc: len exceeds the maximum of 2^34 elements");
+  if (!result || !psi || !diag)
+    return qfx::set_error(QFX_EINVAL, "qfx_expect_diag_noalloc: null pointer");
+  if (len < 1) return qfx::set_error(QFX_ERANGE, "qfx_expect_diag_noalloc: empty input");
+  static_assert(sizeof(scratch) == (size_t)kMaxRedBlocks * sizeof(double2),
+                "scratch holds one partial per stage-1 block");
+  void* ws = nullptr;
+  int rc = scratch_on_current_device(&ws, "qfx_expect_diag_noalloc");
+  if (rc) return rc;
+  return run_reduce<1>(qfx::as_stream(stream), result, psi, diag, len, ws,
+                       "qfx_expect_diag_noalloc");
+}
 
 int64_t qfx_reduce_workspace_bytes(int64_t len) {
   int64_t z = 16;
