@@ -287,6 +287,48 @@ def qaoa_section(steps, grad_compare_n=18):
     return out
 
 
+def qaoa_batches_only(steps, pass_times=False):
+    """Only the two batched QAOA launches (graph replays), for profilers:
+    no plain-path comparison, no other section.  With ``pass_times``, one
+    more launch of each with the kernel's wall-clock stamps on: per global
+    pass, the span and the workgroups' mean and largest busy time; what the
+    spans hold beyond the mean busy time is barrier wait and imbalance."""
+    import numpy as np
+    import torch
+    from quantumflow_amd import qaoa_engine as E
+    from quantumflow_amd.workloads import qaoa_batch_instances, regular_instances
+    out = {}
+    for name, inst, p, mode in (
+            ("forward", qaoa_batch_instances(512, 10, 22, 4, seed=11), 4, E.MODE_FORWARD),
+            ("value_and_grad", regular_instances(64, 16, 24, 8, seed=12), 8, E.MODE_BACKWARD)):
+        graphs = [g for g, _, _ in inst]
+        batch = E.QAOABatch(graphs, p, mode=mode)
+        batch.set_angles(np.array([b for _, b, _ in inst]), np.array([g for _, _, g in inst]))
+        batch.capture()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            batch.replay()
+        torch.cuda.synchronize()
+        batch.check()
+        out[name] = {"instances": len(graphs), "ms_per_batch":
+                     round((time.perf_counter() - t0) / steps * 1e3, 3)}
+        if pass_times:
+            pt = batch.pass_times()
+            span = sum(q["span_us"] for q in pt["passes"])
+            busy = sum(q["busy_mean_us"] for q in pt["passes"])
+            out[name]["pass_times"] = {
+                "grid": pt["grid"], "global_passes": len(pt["passes"]),
+                "total_us": pt["total_us"], "sum_span_us": round(span, 1),
+                "sum_busy_mean_us": round(busy, 1),
+                "sum_busy_max_us": round(sum(q["busy_max_us"] for q in pt["passes"]), 1),
+                "barrier_and_imbalance_frac": round((span - busy) / span, 4),
+                "passes": pt["passes"]}
+        del batch
+        torch.cuda.empty_cache()
+    return out
+
+
 def config1_section(reps=20):
     """The 12-qubit, 20-layer rotation circuit (360 gates) from |0...0>:
     the state is 64 KiB, so this is launch- and latency-bound: microseconds
@@ -465,7 +507,24 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one plain device copy of the state (4 GiB read + 4 GiB "
                          "written at 28 qubits) to scale HBM byte counters against")
+    ap.add_argument("--only", choices=["qaoa"],
+                    help="run only the batched QAOA launches (for profilers) and print "
+                         "their timings")
+    ap.add_argument("--pass-times", action="store_true",
+                    help="with --only qaoa: also report each global pass's span and busy "
+                         "times from the kernel's wall-clock stamps")
     args = ap.parse_args()
+    if args.only == "qaoa":
+        res = {"qaoa_batches": qaoa_batches_only(max(1, args.steps), args.pass_times)}
+        if args.copy_calibration:
+            import torch
+            a = torch.zeros(2 ** 26, dtype=torch.complex128, device="cuda")   # 1 GiB
+            b = torch.empty_like(a)
+            b.copy_(a)
+            torch.cuda.synchronize()
+            res["copy_calibration_bytes"] = 2 ** 30
+        print(json.dumps(res), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(sharding is described in DESIGN.md); run with --gpus 1", file=sys.stderr)

@@ -24,7 +24,7 @@ from .qubits import Qubit, Qubits
 from .states import State
 
 __all__ = ['Measure', 'Reset', 'Barrier', 'If', 'Move', 'Neg', 'Not', 'And', 'Ior', 'Or',
-           'Xor', 'Add', 'Sub', 'Mul', 'Div', 'Exchange', 'EQ', 'GT', 'GE', 'LT', 'LE']
+           'Xor', 'Add', 'Sub', 'Mul', 'Div', 'Exchange', 'EQ', 'GT', 'GE', 'LT', 'LE', 'NE']
 
 _RESET = np.array([[1, 1], [0, 0]], dtype=np.complex128)
 
@@ -323,3 +323,8 @@ class LT(_Comparison):
 class LE(_Comparison):
     """left <= right (stdops.py:334-336)."""
     _op = operator.le
+
+
+class NE(_Comparison):
+    """left != right (stdops.py:339-341)."""
+    _op = operator.ne

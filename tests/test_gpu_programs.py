@@ -140,11 +140,13 @@ def test_classical_arithmetic(qf):
                        qf.Mul(r[0], 2), qf.Div(r[0], 3), qf.Neg(r[0]), qf.Exchange(r[0], r[1]),
                        qf.Move(r[2], 1), qf.And(r[2], 1), qf.Ior(r[2], 0), qf.Xor(r[2], 1),
                        qf.LT(r[3], r[1], r[0]), qf.GT(r[4], r[1], r[0]), qf.GE(r[5], r[0], r[0]),
-                       qf.LE(r[6], r[1], r[0]), qf.EQ(r[7], r[0], r[0])])
+                       qf.LE(r[6], r[1], r[0]), qf.EQ(r[7], r[0], r[0]),
+                       qf.NE(r[8], r[0], r[1]), qf.NE(r[9], r[0], r[0])])
     m = prog.run().memory
     assert m[r[1]] == -6.0 and m[r[0]] == 4
     assert m[r[2]] == 0
     assert m[r[3]] and not m[r[4]] and m[r[5]] and m[r[6]] and m[r[7]]
+    assert m[r[8]] and not m[r[9]]
 
 
 def test_program_evolve_density(qf):
