@@ -24,3 +24,4 @@ from .dagcircuit import *  # noqa: F401,F403
 from .measures import *  # noqa: F401,F403
 from . import qaoa  # noqa: F401
 from . import utils  # noqa: F401
+from . import forest  # noqa: F401

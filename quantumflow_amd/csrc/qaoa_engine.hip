@@ -602,7 +602,7 @@ __device__ bool grid_barrier(unsigned* counter, unsigned target, int* err,
 }
 
 template <int BS, bool BWD>
-__global__ __launch_bounds__(BS) void qaoa_kernel(
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void qaoa_kernel(
     const qfx_qaoa_inst* __restrict__ insts, const qfx_qaoa_pass* __restrict__ passes,
     const qfx_qaoa_item* __restrict__ items, const int* __restrict__ kstart, int nk,
     const double* __restrict__ angles, v2d* __restrict__ psi_pool, v2d* __restrict__ lam_pool,

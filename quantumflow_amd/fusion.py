@@ -28,7 +28,8 @@ Planning (host, once per circuit):
      one kernel launch per pass.
 """
 
-from collections import defaultdict
+import threading
+from collections import OrderedDict, defaultdict
 
 import numpy as np
 import torch
@@ -1151,13 +1152,37 @@ class TilePlan(HostPlan):
         return State(work.view([2] * self.n), self.qubits, ket.memory)
 
 
+# Plans of recent gate lists.  A plan depends only on its gates (their
+# matrices are fixed when a gate is built), qubits and configuration; the
+# entry keeps its gates alive, so the ids in its key cannot be reused by
+# other objects while it is cached.
+_PLAN_CACHE = OrderedDict()
+_PLAN_CACHE_SIZE = 8
+_plan_lock = threading.Lock()
+
+
 def plan_for(gates, qubits, **kwargs):
     """Fused execution plan for ``gates`` on a state with these qubits, or
-    None when the state is too small to tile (fewer than 6 qubits)."""
+    None when the state is too small to tile (fewer than 6 qubits).  The
+    last few plans are kept, so running the same gate objects again (a
+    circuit run repeatedly) skips the planner."""
     n = len(qubits)
     if n < MIN_TILE_BITS:
         return None
     from . import autotune
     cfg = autotune.choose(n, len(gates))
     cfg.update(kwargs)
-    return TilePlan(gates, qubits, **cfg)
+    gates = tuple(gates)
+    key = (tuple(id(g) for g in gates), tuple(qubits), tuple(sorted(cfg.items())),
+           str(bk.DEVICE))
+    with _plan_lock:
+        hit = _PLAN_CACHE.get(key)
+        if hit is not None:
+            _PLAN_CACHE.move_to_end(key)
+            return hit[1]
+    plan = TilePlan(gates, qubits, **cfg)
+    with _plan_lock:
+        _PLAN_CACHE[key] = (gates, plan)
+        while len(_PLAN_CACHE) > _PLAN_CACHE_SIZE:
+            _PLAN_CACHE.popitem(last=False)
+    return plan

@@ -456,7 +456,10 @@ class QAOABatch:
         return {'grid': grid, 'total_us': round(total, 2), 'passes': out}
 
     def evaluate(self, beta, gamma, graph=False):
-        """(values [ninst], grads [ninst, 2p] or None) as device tensors."""
+        """(values [ninst], grads [ninst, 2p] or None) as device tensors.
+        The launch is asynchronous: ``check()`` (which synchronises) raises
+        if any launch since the batch was built hit a barrier timeout, so
+        one call after a run of evaluations covers all of them."""
         self.set_angles(beta, gamma)
         if graph:
             self.replay()

@@ -168,8 +168,14 @@ class Move(Operation):
             return self.source
         return ket.memory[self.source]
 
+    def quil(self) -> str:
+        return 'MOVE {} {}'.format(self.target, self.source)
+
     def run(self, ket: State) -> State:
         return ket.update({self.target: self._value(ket)})
+
+    def evolve(self, rho):
+        return self.run(rho)
 
 
 

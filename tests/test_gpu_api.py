@@ -130,6 +130,29 @@ def test_mixed_circuit_20q(qf, fused):
     assert np.max(np.abs(got - ref)) <= 1e-12
 
 
+def test_fused_plan_reused(qf):
+    """Running the same gate objects again reuses their fused plan; other
+    gates get their own; every run matches the oracle."""
+    from quantumflow_amd import fusion
+    from quantumflow_amd.workloads import random_mixed_circuit
+    n = 14
+    qubits = tuple(range(n))
+    desc = random_mixed_circuit(n, 10, seed=5)
+    gates = build(qf, desc)
+    p1 = fusion.plan_for(gates, qubits)
+    assert fusion.plan_for(list(gates), qubits) is p1
+    circ = qf.Circuit(gates)
+    a1 = circ.run(qf.zero_state(n), fused=True).vec.asarray()
+    a2 = circ.run(qf.zero_state(n), fused=True).vec.asarray()
+    assert np.array_equal(a1, a2)
+    assert np.max(np.abs(a1 - O.run_circuit(desc, qubits=qubits))) <= 1e-12
+    desc2 = random_mixed_circuit(n, 10, seed=6)
+    gates2 = build(qf, desc2)
+    assert fusion.plan_for(gates2, qubits) is not p1
+    a3 = qf.Circuit(gates2).run(qf.zero_state(n), fused=True).vec.asarray()
+    assert np.max(np.abs(a3 - O.run_circuit(desc2, qubits=qubits))) <= 1e-12
+
+
 @pytest.fixture(scope="module")
 def mixed20():
     """The 28-qubit benchmark generator at 20 qubits (depth 40, seed 2024)

@@ -160,3 +160,79 @@ def test_program_evolve_density(qf):
     ref = O.density_from_state(O.run_circuit([('X', (), (0,)), ('H', (), (1,))],
                                              qubits=(0, 1)))
     assert np.max(np.abs(m - ref.reshape(4, 4))) <= 1e-12
+
+
+# ---------------------------------------------------------------------------
+# Quil text through quantumflow_amd.forest (tests/test_quil.py,
+# tests/test_forest.py), on the device state
+
+def _quil(name):
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden', 'quil', name)
+    with open(path) as f:
+        return f.read()
+
+
+def test_run_quil_files(qf):
+    """tests/test_quil.py:36-44: the runnable fixtures run, with the memory
+    and state their instructions fix."""
+    P = qf.forest.quil_to_program
+    ro = qf.Register()
+    assert P(_quil('empty.quil')).run().qubit_nb == 0
+    ket = P(_quil('hello_world.quil')).run()
+    assert ket.memory[ro[0]] == 1
+    ket = P(_quil('measure.quil')).run()
+    assert ket.memory[ro[0]] == 1
+    ket = P(_quil('classical_logic.quil')).run()
+    m = ket.memory
+    assert [m[ro[i]] for i in range(5)] == [0, 0, 0, 1, 0]
+    ket = P(_quil('control_flow.quil')).run()
+    assert ket.memory[ro[0]] == 0
+    ket = P(_quil('bell.quil')).run()
+    assert qf.states_close(ket, qf.ghz_state(2))
+    # the reference QVM's wavefunction for this program
+    # (tests/test_programs.py:190-211)
+    wf_true = np.array([0.00167784 + 1.00210180e-05j, 0.50000000 - 4.99997185e-01j,
+                        0.50000000 - 4.99997185e-01j, 0.00167784 + 1.00210180e-05j])
+    ket = P(_quil('qaoa.quil')).run()
+    assert qf.states_close(ket, qf.State(wf_true.reshape(2, 2)))
+
+
+def test_qvm(qf):
+    """The QVM status machine around the device state
+    (forest/__init__.py:369-444)."""
+    qvm = qf.forest.QuantumFlowQVM()
+    qvm.load(_quil('bell.quil')).run().wait()
+    assert qvm.status == 'done'
+    assert qf.states_close(qvm.wavefunction(), qf.ghz_state(2))
+    qvm.load('X 0\nMEASURE 0 ro[0]\nMEASURE 1 ro[1]').run().wait()
+    assert qvm.read_from_memory_region(region_name='ro') == [1, 0]
+    with pytest.raises(NotImplementedError):
+        qvm.read_from_memory_region(region_name='ro', offsets=[0])
+    prog = qf.Program([qf.Call('X', [], [2]), qf.Measure(2, ('c', 0))])
+    qvm.load(prog).run().wait()
+    assert qvm.wavefunction().memory[('c', 0)] == 1
+
+
+def test_quil_round_trip_wide_state(qf):
+    """A 14-qubit program of standard-gate calls, printed as Quil and parsed
+    back, runs to the same state bit for bit, and to the oracle's state."""
+    rng = np.random.default_rng(21)
+    n = 14
+    desc = [('H', (), (q,)) for q in range(n)]
+    for _ in range(60):
+        r = rng.uniform()
+        if r < 0.5:
+            name = ('RX', 'RY', 'RZ', 'PHASE')[int(rng.integers(4))]
+            desc.append((name, (float(rng.uniform(-7, 7)),), (int(rng.integers(n)),)))
+        else:
+            a, b = (int(x) for x in rng.choice(n, 2, replace=False))
+            desc.append((('CNOT', 'CZ', 'SWAP', 'ISWAP')[int(rng.integers(4))], (), (a, b)))
+    prog = qf.Program([qf.Call(name, list(p), list(qs)) for name, p, qs in desc])
+    parsed = qf.forest.quil_to_program(prog.quil())
+    k0 = prog.run(qf.zero_state(n))
+    k1 = parsed.run(qf.zero_state(n))
+    a0, a1 = k0.vec.asarray(), k1.vec.asarray()
+    assert np.array_equal(a0, a1)
+    ref = O.run_circuit(desc, qubits=tuple(range(n)))
+    assert np.max(np.abs(a1 - ref)) <= 1e-12
