@@ -178,7 +178,12 @@ __device__ __forceinline__ void op_sparse(v2d* lds, int T, const qfx_tile_op& op
 //           gates thus need none of their qubits in registers.
 //   SUB_CX  X on register bit t = code & 3 for the slots whose bits under
 //           the control mask (code >> 2) & 15 equal (code >> 6) & 15: a pure
-//           register exchange (CNOT, CCNOT; controlled SWAP as three)
+//           register exchange (CNOT, CCNOT; controlled SWAP as three).
+//           `mask` may add up to 3 controls that are not register bits:
+//           m = mask & 3 selectors coded as for SUB_DIAGV, selector j
+//           required to equal bit 23 + j of `mask`; where they do not, the
+//           sub-op leaves the lane's amplitudes as they are.  Controlled-X
+//           gates thus need only their target in registers.
 // Uncontrolled exchanges (SWAP, X, the permutation of ISWAP and Y) never
 // reach the kernel: the host relabels which logical slot each register
 // holds, and a leading run of SWAP/X/CNOT gates is folded into the load.
@@ -328,7 +333,24 @@ __device__ __forceinline__ void apply_sub(v2d (&v)[kRegs], const int4 sub, const
       break;
     }
     case SUB_CX: {
-      const unsigned cm = ((unsigned)code >> 2) & 15u, cv = ((unsigned)code >> 6) & 15u;
+      unsigned cm = ((unsigned)code >> 2) & 15u, cv = ((unsigned)code >> 6) & 15u;
+      const unsigned w = (unsigned)sub.w;
+      if (w) {
+        // selector controls: unmet, the test asks the target bit to be both
+        // 0 (pair's low slot) and 1, so no pair swaps
+        bool ok = true;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          if (j < (int)(w & 3u)) {
+            const unsigned p = (w >> (2 + 7 * j)) & 127u;
+            const unsigned b = p < 64u ? (unsigned)((gbase >> p) & 1u) : (egrp >> (p - 64u)) & 1u;
+            ok = ok && b == ((w >> (23 + j)) & 1u);
+          }
+        }
+        const unsigned tb = 1u << (code & 3);
+        cm = ok ? cm : (cm | tb);
+        cv = ok ? cv : (cv | tb);
+      }
       switch (code & 3) {
         case 0: sub_cx<0>(v, cm, cv); break;
         case 1: sub_cx<1>(v, cm, cv); break;
@@ -1292,6 +1314,20 @@ int qfx_tile_check_ops(int T, int block, int flags, const qfx_tile_op* ops, int 
           const int cm = (cd >> 2) & 15, cv = (cd >> 6) & 15;
           if (((cm >> (cd & 3)) & 1) || (cv & ~cm))
             return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: bad CX control bits");
+          // selector controls: 0..3 selectors as for SUB_DIAGV, their values
+          // in bits 23.., nothing above
+          const unsigned w = (unsigned)sub[3], m = w & 3u;
+          bool good = w == 0 || (m != 0 && (w >> (23 + m)) == 0 &&
+                                 ((w >> (2 + 7 * m)) & ((1u << (21 - 7 * m)) - 1u)) == 0);
+          for (unsigned j = 0; good && w && j < m; ++j) {
+            const unsigned p = (w >> (2 + 7 * j)) & 127u;
+            if (p >= 64u) {
+              const int t = (int)(p - 64u);
+              good = t < T;
+              for (int a = 0; good && a < kRegBits; ++a) good = o.t[a] != t;
+            }
+          }
+          if (!good) return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: bad CX selector controls");
         }
         if (need[ty] && (sub[2] < 0 || sub[2] + need[ty] > ncoef))
           return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: sub-op coefficients out of range");

@@ -133,24 +133,25 @@ def gate_types(kind, bits):
     return {b: 'G' for b in bits}
 
 
-def _absorb(bits_list, pending, S, ok=None, free=None, types=None):
+def _absorb(bits_list, pending, S, ok=None, free=None, types=None, need=None):
     """Items (indices into bits_list, in order) a set of bits S takes: an item
-    joins if it is allowed, its bits lie in S (or it is ``free``: a diagonal
-    gate that needs none of its bits in the tile) and it commutes with every
-    item left behind before it: it shares no bit with them or, with
-    ``types`` (gate_types per item), every shared bit carries the same 'Z'
-    or 'X' in both."""
+    joins if it is allowed, the bits it needs lie in S (all of its bits, or
+    ``need[i]``: a controlled X needs only its target; a ``free`` diagonal
+    gate needs none) and it commutes with every item left behind before it:
+    it shares no bit with them or, with ``types`` (gate_types per item),
+    every shared bit carries the same 'Z' or 'X' in both."""
     blocked = {}
     taken = []
     for i in pending:
         b = bits_list[i]
+        nb = b if need is None else need[i]
         if types is None:
             clear = not any(x in blocked for x in b)
         else:
             ti = types[i]
             clear = all(x not in blocked or (ti[x] != 'G' and blocked[x] <= {ti[x]}) for x in b)
         if (clear and (ok is None or ok[i])
-                and (S.issuperset(b) or (free is not None and free[i]))):
+                and (S.issuperset(nb) or (free is not None and free[i]))):
             taken.append(i)
             continue
         for x in b:
@@ -158,12 +159,12 @@ def _absorb(bits_list, pending, S, ok=None, free=None, types=None):
     return taken
 
 
-def _grow(bits_list, pending, S, width, ok=None, cap=40, free=None, types=None):
+def _grow(bits_list, pending, S, width, ok=None, cap=40, free=None, types=None, need=None):
     """Grow the bit set S to at most ``width`` bits, one candidate at a time:
-    the candidates are the bit sets of the first ``cap`` distinct items not
-    yet taken (in circuit order), and the one that takes the most further
-    items per added bit wins.  Returns (S, items taken)."""
-    cur = _absorb(bits_list, pending, S, ok, free, types)
+    the candidates are the needed bit sets of the first ``cap`` distinct
+    items not yet taken (in circuit order), and the one that takes the most
+    further items per added bit wins.  Returns (S, items taken)."""
+    cur = _absorb(bits_list, pending, S, ok, free, types, need)
     while len(S) < width:
         taken = set(cur)
         best = None
@@ -171,11 +172,11 @@ def _grow(bits_list, pending, S, width, ok=None, cap=40, free=None, types=None):
         for i in pending:
             if i in taken or (ok is not None and not ok[i]) or (free is not None and free[i]):
                 continue
-            new = frozenset(bits_list[i]) - S
+            new = frozenset(bits_list[i] if need is None else need[i]) - S
             if not new or new in seen or len(S) + len(new) > width:
                 continue
             seen.add(new)
-            got = _absorb(bits_list, pending, S | new, ok, free, types)
+            got = _absorb(bits_list, pending, S | new, ok, free, types, need)
             gain = (len(got) - len(cur)) / len(new)
             if best is None or gain > best[0]:
                 best = (gain, new, got)
@@ -188,7 +189,8 @@ def _grow(bits_list, pending, S, width, ok=None, cap=40, free=None, types=None):
     return S, cur
 
 
-def plan_passes(bits_list, n, T, L, tileable=None, strategy='gain', free=None, types=None):
+def plan_passes(bits_list, n, T, L, tileable=None, strategy='gain', free=None, types=None,
+                need=None):
     """Group gates (given by their index-bit tuples, in circuit order) into
     passes.  Returns a list of (sorted tile bits, [gate indices]); a gate
     that is not tileable forms a pass of its own with tile bits None.
@@ -196,7 +198,9 @@ def plan_passes(bits_list, n, T, L, tileable=None, strategy='gain', free=None, t
     bits (register phases select their phases by the tile's other index
     bits): they join the first pass that reaches them and add no tile bits.
     ``types`` (gate_types per gate) lets a gate overtake the commuting gates
-    a pass leaves behind even where they share bits.
+    a pass leaves behind even where they share bits.  ``need[i]`` (default:
+    all of gate i's bits) are the bits gate i needs in the tile: a
+    controlled X whose controls may lie outside it needs only its target.
 
     strategy 'gain' grows each pass's tile set by ``_grow`` (the bits that
     take the most gates per added bit); 'greedy' takes gates first come,
@@ -212,7 +216,8 @@ def plan_passes(bits_list, n, T, L, tileable=None, strategy='gain', free=None, t
             pending = pending[1:]
             continue
         if strategy == 'gain':
-            S, _ = _grow(bits_list, pending, set(low), T, tileable, free=free, types=types)
+            S, _ = _grow(bits_list, pending, set(low), T, tileable, free=free, types=types,
+                         need=need)
         else:
             S = set(low)
             blocked = set()
@@ -231,7 +236,7 @@ def plan_passes(bits_list, n, T, L, tileable=None, strategy='gain', free=None, t
             if b not in S:
                 S.add(b)
             b += 1
-        taken = _absorb(bits_list, pending, S, tileable, free, types)
+        taken = _absorb(bits_list, pending, S, tileable, free, types, need)
         passes.append((sorted(S), taken))
         done = set(taken)
         pending = [g for g in pending if g not in done]
@@ -276,6 +281,38 @@ class SelDiagonal:
                 tables[v, x] = kind.diag[r]
         self.tables = tables
         self.diag = None
+
+
+class SelCX:
+    """A controlled X of a pass some of whose controls lie outside the tile:
+    X on the target, controlled by its in-tile controls (a monomial on the
+    in-tile bits, target first) and by the index bits ``sel`` outside the
+    tile, which fix per tile whether it acts.  All controls are on |1>.
+    Duck-types a monomial GateKind."""
+    kind = 'monomial'
+    phase = matrix = diag = None
+
+    def __init__(self, target, in_ctrls, sel):
+        self.in_bits = [target] + list(in_ctrls)
+        self.sel = list(sel)
+        k = self.k = len(self.in_bits)
+        # output row r takes input row src[r]: flip the target (gate qubit 0,
+        # the most significant) where every in-tile control is 1
+        ctl = (1 << (k - 1)) - 1
+        self.src = np.array([r ^ (1 << (k - 1)) if (r & ctl) == ctl else r
+                             for r in range(2 ** k)])
+
+
+def cx_parts(op_or_kind, bits):
+    """(target bit, control bits) of a controlled X without phases acting on
+    ``bits`` (gate-qubit order), else None."""
+    kd = op_or_kind
+    if kd.kind != bk.GateKind.MONOMIAL or kd.phase is not None:
+        return None
+    pat = exchange_pattern(kd)
+    if pat is None or pat[0] != 'cx':
+        return None
+    return bits[pat[1]], [bits[c] for c in pat[2]]
 
 
 class Op:
@@ -448,12 +485,22 @@ def plan_phases(ops, width=REG_BITS, free_diagonals=False):
     and standalone LDS ops, with the commuting-overtake rule of
     plan_passes and its bit-set growth.  With ``free_diagonals`` a diagonal
     op joins the first phase that reaches it whatever that phase's register
-    bits (its other bits select its phases per lane group).  Returns
-    [('phase', bits, ops) | ('lds', op)]."""
+    bits (its other bits select its phases per lane group), and a
+    controlled X needs only its target in registers (its other controls
+    become selectors of the same kind).  Returns [('phase', bits, ops) |
+    ('lds', op)]."""
     bits_list = [tuple(op.bits) for op in ops]
     ok = [phaseable(op) for op in ops]
-    free = [is_diagonal_op(op) and len(op.bits) + len(getattr(op.kind, 'sel', ())) <= MAX_SEL_BITS
-            for op in ops] if free_diagonals else None
+    free = need = None
+    if free_diagonals:
+        free = [is_diagonal_op(op)
+                and len(op.bits) + len(getattr(op.kind, 'sel', ())) <= MAX_SEL_BITS
+                for op in ops]
+        need = []
+        for op, b in zip(ops, bits_list):
+            cx = cx_parts(op.kind, b) if op.matrix is None else None
+            nsel = len(getattr(op.kind, 'sel', ())) + (len(cx[1]) if cx else 0)
+            need.append((cx[0],) if cx and nsel <= MAX_SEL_BITS else b)
     pending = list(range(len(ops)))
     items = []
     while pending:
@@ -462,7 +509,7 @@ def plan_phases(ops, width=REG_BITS, free_diagonals=False):
             items.append(('lds', ops[first]))
             pending = pending[1:]
             continue
-        R, taken = _grow(bits_list, pending, set(), width, ok, free=free)
+        R, taken = _grow(bits_list, pending, set(), width, ok, free=free, need=need)
         items.append(('phase', R, [ops[i] for i in taken]))
         done = set(taken)
         pending = [i for i in pending if i not in done]
@@ -617,6 +664,21 @@ def _encode_phase(R, members, pos_index, T, coefs, subs):
         out.append([SUB_DIAGV if sel else SUB_DIAG, sel, tab])
         pending[0] = None
 
+    def push_cx_sel(t, ctrls, sels):
+        """X on logical bit t controlled by logical bits ctrls and by the
+        selector codes sels (lane-group tile bits 64 + t, index bits outside
+        the tile), all on |1>.  Whether it acts varies across lanes and
+        tiles, so a pending diagonal is emitted first."""
+        flush()
+        cm = cv = 0
+        for b in ctrls:
+            cm |= 1 << q[b]
+            cv |= (1 ^ ((flip[0] >> b) & 1)) << q[b]
+        w = len(sels)
+        for j, c in enumerate(sels):
+            w |= (c << (2 + 7 * j)) | (1 << (23 + j))
+        out.append([SUB_CX, q[t] | (cm << 2) | (cv << 6), w])
+
     def push_cx(t, ctrls):
         if not ctrls:                       # X: relabel only
             flip[0] ^= 1 << t
@@ -637,7 +699,7 @@ def _encode_phase(R, members, pos_index, T, coefs, subs):
     k0 = 0
     for op in members:
         kd = op.kind
-        if isinstance(kd, SelDiagonal) or not in_regs(op):
+        if isinstance(kd, (SelDiagonal, SelCX)) or not in_regs(op):
             break
         p = [reg_of[pos_index[b]] for b in op.bits]
         if op.matrix is None and kd.kind == bk.GateKind.DIAGONAL:
@@ -672,6 +734,16 @@ def _encode_phase(R, members, pos_index, T, coefs, subs):
         if isinstance(kd, SelDiagonal) or (is_diagonal_op(op) and not in_regs(op)):
             sel, tab = _diag_tables(op, reg_of, pos_index)
             push_diag(None, sel, tab)
+            continue
+        cx = cx_parts(kd, op.bits) if op.matrix is None else None
+        if cx is not None and (isinstance(kd, SelCX) or not in_regs(op)):
+            # controls outside the registers: lane-group bits and, for a
+            # SelCX, index bits outside the tile
+            t, ctl = cx
+            rc = [reg_of[pos_index[c]] for c in ctl if pos_index[c] in reg_of]
+            sels = [64 + pos_index[c] for c in ctl if pos_index[c] not in reg_of]
+            sels += list(getattr(kd, 'sel', ()))
+            push_cx_sel(reg_of[pos_index[t]], rc, sels)
             continue
         p = [reg_of[pos_index[b]] for b in op.bits]        # logical bit per gate qubit
         if op.matrix is None and kd.kind == bk.GateKind.DIAGONAL:
@@ -727,7 +799,7 @@ def _encode_phase(R, members, pos_index, T, coefs, subs):
                 subs.extend([SUB_DIAGV, cd, len(coefs), 0])
                 coefs.extend(tab.reshape(-1))
         elif typ == SUB_CX:
-            subs.extend([typ, code, 0, 0])
+            subs.extend([typ, code, 0, data or 0])
         else:
             subs.extend([typ, code, len(coefs), 0])
             coefs.extend(np.asarray(data, dtype=np.complex128).reshape(-1))
@@ -892,7 +964,8 @@ class HostPlan:
     index their pass's coefficient table."""
 
     def __init__(self, gates, qubits, tile_bits=MAX_TILE_BITS, low_bits=3, block=256,
-                 fuse_width=4, mode='reg', persistent=True, reg_io=True, reg_in=True):
+                 fuse_width=4, mode='reg', persistent=True, reg_io=True, reg_in=True,
+                 free_controls=True):
         qubits = tuple(qubits)
         n = len(qubits)
         self.n = n
@@ -912,20 +985,29 @@ class HostPlan:
         self.persistent = persistent
         self.reg_io = reg_io
         self.reg_in = reg_in
+        self.free_controls = free_controls
         self.gates = list(gates)
         bits_list = [tuple(n - 1 - qubits.index(q) for q in g.qubits) for g in gates]
         tileable = [len(b) <= 5 and len(b) <= T for b in bits_list]
         # register phases apply diagonal gates whatever the tile bits: the
         # phases of the qubits outside the tile are selected per tile (only
         # where no pass can turn wide, i.e. every gate acts on <= 3 qubits)
-        free = None
+        # and controlled X gates need only their target in the tile: controls
+        # outside it are read from the tile's index per tile
+        free = need = None
         if self.mode == 'reg' and all(len(b) <= 3 for b in bits_list):
             free = [g.kind.kind == bk.GateKind.DIAGONAL for g in gates]
+            if free_controls:
+                need = []
+                for g, b in zip(gates, bits_list):
+                    cx = cx_parts(g.kind, b)
+                    need.append((cx[0],) if cx else b)
         # commutation-aware overtaking can pack the same gates into fewer
         # passes, or into as many heavier ones: keep it only when it saves a pass
         types = [gate_types(g.kind, b) for g, b in zip(gates, bits_list)]
-        planned = plan_passes(bits_list, n, T, low_bits, tileable, free=free)
-        typed = plan_passes(bits_list, n, T, low_bits, tileable, free=free, types=types)
+        planned = plan_passes(bits_list, n, T, low_bits, tileable, free=free, need=need)
+        typed = plan_passes(bits_list, n, T, low_bits, tileable, free=free, types=types,
+                            need=need)
         if len(typed) < len(planned):
             planned = typed
         self.passes = []           # (kind, payload)
@@ -946,9 +1028,13 @@ class HostPlan:
                 bits = bits_list[i]
                 if tset.issuperset(bits):
                     ops.append(Op(bits, gates[i].kind))
-                else:
+                elif gates[i].kind.kind == bk.GateKind.DIAGONAL:
                     sd = SelDiagonal(gates[i].kind, bits, tset)
                     ops.append(Op(sd.in_bits, sd))
+                else:                       # controlled X, controls outside the tile
+                    t, ctl = cx_parts(gates[i].kind, bits)
+                    sc = SelCX(t, [c for c in ctl if c in tset], [c for c in ctl if c not in tset])
+                    ops.append(Op(sc.in_bits, sc))
             wide = any(len(op.bits) > 3 for op in ops)
             start = len(ops_host)
             c_lo = len(coefs)

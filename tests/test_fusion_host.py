@@ -147,6 +147,10 @@ def _phase_unitary(o, coefs, srcs, sel_val):
             u = np.diag(coefs[c0 + 16 * v:c0 + 16 * v + 16]) @ u
             continue
         if typ == fusion.SUB_CX:
+            # selector controls (mask): selector j must equal bit 23 + j
+            if any(sel_val[(mask >> (2 + 7 * j)) & 127] != (mask >> (23 + j)) & 1
+                   for j in range(mask & 3)):
+                continue
             perm = np.zeros((16, 16))
             t, cm, cv = code & 3, (code >> 2) & 15, (code >> 6) & 15
             for r in range(16):
@@ -198,6 +202,9 @@ def _emulate_ops(ket, n, pos, ops, coefs, srcs):
                 typ, code = srcs[o.src + 4 * q], srcs[o.src + 4 * q + 1]
                 if typ == fusion.SUB_DIAGV:
                     sel |= {(code >> (2 + 7 * j)) & 127 for j in range(code & 3)}
+                if typ == fusion.SUB_CX:
+                    w = srcs[o.src + 4 * q + 3]
+                    sel |= {(w >> (2 + 7 * j)) & 127 for j in range(w & 3)}
             sel = sorted(sel)
 
             def index_bit(c):               # selector code -> state-index bit
@@ -623,6 +630,82 @@ def test_selector_diagonal_validator_rules():
     assert check(1 | ((64 + 5) << 2), 32) == 0
     assert check(1 | ((64 + 2) << 2), 32) == _native.QFX_EINVAL         # a register bit
     assert check(1 | ((64 + 8) << 2), 32) == _native.QFX_EINVAL         # beyond T = 8
+
+
+@pytest.mark.parametrize("n,T,seed", [(12, 8, 0), (13, 9, 1), (12, 7, 2), (14, 8, 3)])
+def test_selector_controls_replay(n, T, seed):
+    """Controlled X gates need only their target in the tile and in the
+    phase's registers: their other controls select per lane group (tile
+    bits outside the registers) or per tile (index bits outside the tile)
+    whether they act (SUB_CX selector controls).  Replaying the plan gives
+    the circuit's state, and the circuit takes fewer passes than with every
+    control in the tile."""
+    rng = np.random.default_rng(seed)
+    desc = []
+    for _ in range(6):
+        for q in rng.permutation(n)[:n // 2]:
+            nm = ('RX', 'RY', 'H', 'T')[rng.integers(4)]
+            desc.append((nm, (float(rng.uniform(0, 6)),) if nm in ('RX', 'RY') else (), (int(q),)))
+        for _ in range(n // 2):
+            nm = ('CNOT', 'CCNOT', 'CZ', 'CNOT')[rng.integers(4)]
+            k = 3 if nm == 'CCNOT' else 2
+            desc.append((nm, (), tuple(int(x) for x in rng.choice(n, k, replace=False))))
+    gates = build_gates(desc)
+    ref = O.run_circuit(desc, qubits=tuple(range(n)))
+    plan = fusion.HostPlan(gates, tuple(range(n)), tile_bits=T, low_bits=3, block=256,
+                           fuse_width=1, mode='reg')
+    lane = glob = 0
+    ket = O.zero_state(n)
+    for kind, payload in plan.passes:
+        pos, start, nops, block, flags, c_lo, ncoef, spos = payload
+        ops = plan.ops_host[start:start + nops]
+        for o in ops:
+            if o.kind != fusion.OP_RPHASE:
+                continue
+            for q in range(o.coef):
+                typ, w = plan.srcs[o.src + 4 * q], plan.srcs[o.src + 4 * q + 3]
+                if typ == fusion.SUB_CX and w:
+                    codes = [(w >> (2 + 7 * j)) & 127 for j in range(w & 3)]
+                    lane += any(c >= 64 for c in codes)
+                    glob += any(c < 64 for c in codes)
+        ket = _emulate_ops(ket, n, pos, ops, plan.coefs[c_lo:c_lo + ncoef], plan.srcs)
+        ket = apply_store(ket, n, pos, spos)
+    assert lane > 0 and glob > 0
+    assert np.max(np.abs(ket - ref)) < 1e-12
+    plain = fusion.HostPlan(gates, tuple(range(n)), tile_bits=T, low_bits=3, block=256,
+                            fuse_width=1, mode='reg', free_controls=False)
+    assert plan.n_passes < plain.n_passes
+
+
+def test_selector_control_validator_rules():
+    """SUB_CX selector controls (the record's fourth word): none, or 1..3
+    selectors coded as for SUB_DIAGV with their required values in bits
+    23 + j, nothing else."""
+    lib = _native.load()
+    o = _native.QfxTileOp()
+    o.kind = fusion.OP_RPHASE
+    o.k = 4
+    for j in range(4):
+        o.t[j] = o.sorted[j] = j
+    o.coef, o.src = 1, 0
+    o.active = o.phase = sum(1 << (5 * j) for j in range(4))   # identity maps
+    fusion.prepare_ops([o])
+    arr = (_native.QfxTileOp * 1)(o)
+
+    def check(w, code=1 | (1 << 2) | (1 << 6)):     # X on bit 1 controlled by bit 0
+        srcs = _native.int_array([fusion.SUB_CX, code, 0, w])
+        return lib.qfx_tile_check_ops(8, 256, 0, arr, 1, 0, srcs, 4)
+    assert check(0) == 0
+    assert check(1 | (20 << 2) | (1 << 23)) == 0                      # index bit 20 on |1>
+    assert check(1 | (20 << 2)) == 0                                  # ... on |0>
+    assert check(2 | (20 << 2) | ((64 + 5) << 9) | (3 << 23)) == 0    # and lane bit 5
+    assert check(3 | (10 << 2) | (11 << 9) | (12 << 16) | (7 << 23)) == 0
+    assert check(1 << 23) == _native.QFX_EINVAL                       # values, no selector
+    assert check(1 | (20 << 2) | (1 << 24)) == _native.QFX_EINVAL     # value beyond m
+    assert check(1 | (20 << 2) | (5 << 9)) == _native.QFX_EINVAL      # stray selector bits
+    assert check(1 | ((64 + 2) << 2)) == _native.QFX_EINVAL           # a register bit
+    assert check(1 | ((64 + 8) << 2)) == _native.QFX_EINVAL           # beyond T = 8
+    assert check(0, code=1 | (2 << 2)) == _native.QFX_EINVAL          # control on the target
 
 
 def test_gate_types_commutation_brute_force():
