@@ -175,7 +175,10 @@ __device__ __forceinline__ void op_sparse(v2d* lds, int T, const qfx_tile_op& op
 //           bit j of v is selector p = (code >> (2 + 7 j)) & 127: state-index
 //           bit p (< 64, outside the tile) or, for p >= 64, tile bit p - 64
 //           of the lane group (a non-register bit of the phase).  Diagonal
-//           gates thus need none of their qubits in registers.
+//           gates thus need none of their qubits in registers.  With
+//           `mask` = 16 | d, the tables are compact: they depend only on the
+//           register bits in d, table v holds 2^popcount(d) phases, and slot
+//           r reads the entry indexed by r's bits under d, in order.
 //   SUB_CX  X on register bit t = code & 3 for the slots whose bits under
 //           the control mask (code >> 2) & 15 equal (code >> 6) & 15: a pure
 //           register exchange (CNOT, CCNOT; controlled SWAP as three).
@@ -263,6 +266,32 @@ __device__ __forceinline__ void sub_diag(v2d (&v)[kRegs], const v2d* __restrict_
   }
 }
 
+// A diagonal given by a compact table over the register bits in `dep`:
+// slot r multiplies by ph[index of r's bits under dep], the index a sum of
+// per-bit strides (uniform scalars), so no register is indexed at run time.
+__device__ __forceinline__ void sub_diag_compact(v2d (&v)[kRegs], const v2d* __restrict__ ph,
+                                                 unsigned dep) {
+  unsigned stride[kRegBits];
+  unsigned k = 0;
+#pragma unroll
+  for (int j = 0; j < kRegBits; ++j) {
+    stride[j] = ((dep >> j) & 1u) ? (1u << k) : 0u;
+    k += (dep >> j) & 1u;
+  }
+#pragma unroll
+  for (int r0 = 0; r0 < kRegs; r0 += 4) {
+#pragma unroll
+    for (int r = r0; r < r0 + 4; ++r) {
+      unsigned idx = 0;
+#pragma unroll
+      for (int j = 0; j < kRegBits; ++j)
+        if (r & (1 << j)) idx += stride[j];
+      v[r] = cmul(ph[idx], v[r]);
+    }
+    __builtin_amdgcn_sched_barrier(0);   // four phases in registers at a time
+  }
+}
+
 // Controlled exchange with runtime control bits: the pair structure (target
 // bit) is compile time, the control test a per-slot select, so no register
 // is ever indexed at run time.  Slot r's pair swaps when its control bits
@@ -329,7 +358,13 @@ __device__ __forceinline__ void apply_sub(v2d (&v)[kRegs], const int4 sub, const
           sel |= b << j;
         }
       }
-      sub_diag(v, m + 16 * sel);
+      const unsigned w = (unsigned)sub.w;
+      if (w) {
+        const unsigned dep = w & 15u;
+        sub_diag_compact(v, m + (sel << __builtin_popcount(dep)), dep);
+      } else {
+        sub_diag(v, m + 16 * sel);
+      }
       break;
     }
     case SUB_CX: {
@@ -1303,8 +1338,12 @@ int qfx_tile_check_ops(int T, int block, int flags, const qfx_tile_op* ops, int 
               for (int a = 0; good && a < kRegBits; ++a) good = o.t[a] != t;
             }
           }
+          // tables: 16 phases each, or compact (mask 16 | d): 2^popcount(d)
+          const unsigned w = (unsigned)sub[3];
+          if (w != 0 && (w >> 4) != 1u) good = false;
+          const int per = w ? (1 << __builtin_popcount(w & 15u)) : kRegs;
           if (!good) return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: bad selector diagonal");
-          if (sub[2] < 0 || sub[2] + (kRegs << m) > ncoef)
+          if (sub[2] < 0 || sub[2] + (per << m) > ncoef)
             return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: sub-op coefficients out of range");
           continue;
         }

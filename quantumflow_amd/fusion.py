@@ -796,8 +796,19 @@ def _encode_phase(R, members, pos_index, T, coefs, subs):
                 cd = len(sel)
                 for j, b in enumerate(sel):
                     cd |= b << (2 + 7 * j)
-                subs.extend([SUB_DIAGV, cd, len(coefs), 0])
-                coefs.extend(tab.reshape(-1))
+                # register bits the tables depend on; fewer than 4: store
+                # compact tables, entry = the slot's bits under dep, in order
+                dep = [j for j in range(REG_BITS)
+                       if not np.array_equal(tab, tab[:, regs16 ^ (1 << j)])]
+                if len(dep) < REG_BITS:
+                    slots = [sum(((i >> a) & 1) << j for a, j in enumerate(dep))
+                             for i in range(2 ** len(dep))]
+                    subs.extend([SUB_DIAGV, cd, len(coefs),
+                                 16 | sum(1 << j for j in dep)])
+                    coefs.extend(tab[:, slots].reshape(-1))
+                else:
+                    subs.extend([SUB_DIAGV, cd, len(coefs), 0])
+                    coefs.extend(tab.reshape(-1))
         elif typ == SUB_CX:
             subs.extend([typ, code, 0, data or 0])
         else:
@@ -955,6 +966,62 @@ def _rebase_coef(o, srcs, off):
         o.coef -= off
 
 
+_SWAP_KIND = None
+
+
+def virtualize_swaps(kinds, bits_list, n):
+    """Remove uncontrolled SWAP-type gates (SWAP, ISWAP, PSWAP) from a gate
+    list by renaming: after a swap of index bits x and y the data of x sits
+    where y's was, so later gates act on the swapped bits and the gate moves
+    nothing.  Its phases (ISWAP, PSWAP) stay as a diagonal on the renamed
+    bits.  Plain SWAPs appended at the end put every bit's data back in
+    place, one transposition per bit still displaced, cycle by cycle.
+    Returns (kinds, bits, number of swaps removed)."""
+    global _SWAP_KIND
+    if _SWAP_KIND is None:
+        _SWAP_KIND = bk.classify(np.array([[1, 0, 0, 0], [0, 0, 1, 0], [0, 1, 0, 0], [0, 0, 0, 1]],
+                                          dtype=np.complex128))
+    where = list(range(n))              # index bit -> where its data sits
+    out_k, out_b = [], []
+    removed = 0
+    for kd, bits in zip(kinds, bits_list):
+        pat = exchange_pattern(kd) if kd.kind == bk.GateKind.MONOMIAL else None
+        if pat is not None and pat[0] == 'swap' and not pat[3]:
+            x, y = bits[pat[1]], bits[pat[2]]
+            where[x], where[y] = where[y], where[x]
+            removed += 1
+            if kd.phase is not None:        # phases on output rows, at the new bits
+                out_k.append(bk.classify(np.diag(kd.phase)))
+                out_b.append(tuple(where[b] for b in bits))
+            continue
+        out_k.append(kd)
+        out_b.append(tuple(where[b] for b in bits))
+    # restore: walk each cycle of the displacement; swapping the data at
+    # where[b] into b fixes b and passes the displaced data along the cycle,
+    # so every swap of the cycle shares the position its walk starts from.
+    # Starting where the cycle's highest position holds its data keeps that
+    # shared position off the low bits (swaps there are register exchanges,
+    # elsewhere store permutations).
+    holder = {p: b for b, p in enumerate(where)}     # position -> bit whose data is there
+    for b0 in range(n):
+        if where[b0] == b0:
+            continue
+        cyc, b = [b0], holder[b0]
+        while b != b0:
+            cyc.append(b)
+            b = holder[b]
+        b = max(cyc, key=lambda c: where[c])
+        while where[b] != b:
+            p = where[b]
+            out_k.append(_SWAP_KIND)
+            out_b.append((p, b))
+            other = holder[b]               # the bit whose data sat at b moves to p
+            where[other], holder[p] = p, other
+            where[b], holder[b] = b, b
+            b = other
+    return out_k, out_b, removed
+
+
 class HostPlan:
     """A gate list compiled into tiled passes on the host: the pass list,
     the op records, coefficient and source tables, validated by the
@@ -965,7 +1032,7 @@ class HostPlan:
 
     def __init__(self, gates, qubits, tile_bits=MAX_TILE_BITS, low_bits=3, block=256,
                  fuse_width=4, mode='reg', persistent=True, reg_io=True, reg_in=True,
-                 free_controls=True):
+                 free_controls=True, virtual_swaps=True):
         qubits = tuple(qubits)
         n = len(qubits)
         self.n = n
@@ -988,6 +1055,13 @@ class HostPlan:
         self.free_controls = free_controls
         self.gates = list(gates)
         bits_list = [tuple(n - 1 - qubits.index(q) for q in g.qubits) for g in gates]
+        kinds = [g.kind for g in gates]
+        # uncontrolled SWAP-type gates move no data where they stand: later
+        # gates are renamed onto the swapped bits, and swaps appended at the
+        # end restore the order (passes then absorb them as store permutations)
+        self.n_virtual = 0
+        if self.mode == 'reg' and virtual_swaps:
+            kinds, bits_list, self.n_virtual = virtualize_swaps(kinds, bits_list, n)
         tileable = [len(b) <= 5 and len(b) <= T for b in bits_list]
         # register phases apply diagonal gates whatever the tile bits: the
         # phases of the qubits outside the tile are selected per tile (only
@@ -996,15 +1070,15 @@ class HostPlan:
         # outside it are read from the tile's index per tile
         free = need = None
         if self.mode == 'reg' and all(len(b) <= 3 for b in bits_list):
-            free = [g.kind.kind == bk.GateKind.DIAGONAL for g in gates]
+            free = [kd.kind == bk.GateKind.DIAGONAL for kd in kinds]
             if free_controls:
                 need = []
-                for g, b in zip(gates, bits_list):
-                    cx = cx_parts(g.kind, b)
+                for kd, b in zip(kinds, bits_list):
+                    cx = cx_parts(kd, b)
                     need.append((cx[0],) if cx else b)
         # commutation-aware overtaking can pack the same gates into fewer
         # passes, or into as many heavier ones: keep it only when it saves a pass
-        types = [gate_types(g.kind, b) for g, b in zip(gates, bits_list)]
+        types = [gate_types(kd, b) for kd, b in zip(kinds, bits_list)]
         planned = plan_passes(bits_list, n, T, low_bits, tileable, free=free, need=need)
         typed = plan_passes(bits_list, n, T, low_bits, tileable, free=free, types=types,
                             need=need)
@@ -1017,8 +1091,7 @@ class HostPlan:
         orders, holders = [], []   # per tile pass: tile bits, qubit each holds at the end
         for pos, members in planned:
             if pos is None:
-                g = gates[members[0]]
-                self.passes.append(('gate', (list(bits_list[members[0]]), g.kind)))
+                self.passes.append(('gate', (list(bits_list[members[0]]), kinds[members[0]])))
                 continue
             tset = set(pos)
             pos = sorted(pos)
@@ -1027,12 +1100,12 @@ class HostPlan:
             for i in members:
                 bits = bits_list[i]
                 if tset.issuperset(bits):
-                    ops.append(Op(bits, gates[i].kind))
-                elif gates[i].kind.kind == bk.GateKind.DIAGONAL:
-                    sd = SelDiagonal(gates[i].kind, bits, tset)
+                    ops.append(Op(bits, kinds[i]))
+                elif kinds[i].kind == bk.GateKind.DIAGONAL:
+                    sd = SelDiagonal(kinds[i], bits, tset)
                     ops.append(Op(sd.in_bits, sd))
                 else:                       # controlled X, controls outside the tile
-                    t, ctl = cx_parts(gates[i].kind, bits)
+                    t, ctl = cx_parts(kinds[i], bits)
                     sc = SelCX(t, [c for c in ctl if c in tset], [c for c in ctl if c not in tset])
                     ops.append(Op(sc.in_bits, sc))
             wide = any(len(op.bits) > 3 for op in ops)
@@ -1053,25 +1126,17 @@ class HostPlan:
                 block, flags = self.block, TILE_MFMA
             elif self.mode == 'reg' and not wide:
                 ops, store_pos = relabel_swaps(ops, pos, low_bits)
-                # diagonals free of the register bits first; a phase whose
-                # selector tables exceed the staging limit replans the pass
-                # with diagonals on register bits
-                marks = (len(ops_host), len(coefs), len(srcs), self.n_phases)
-                for free_diag in ((True, False) if free is not None else (False,)):
-                    del ops_host[marks[0]:], coefs[marks[1]:], srcs[marks[2]:]
-                    self.n_phases = marks[3]
-                    fits = True
-                    for item in plan_phases(ops, free_diagonals=free_diag):
-                        c0 = len(coefs)
-                        if item[0] == 'phase':
-                            ops_host.append(_encode_phase(item[1], item[2], pos_index, T,
-                                                          coefs, srcs))
+                # diagonals and controls free of the register bits; a phase
+                # whose tables exceed the staging limit runs as consecutive
+                # phases on the same bits, each holding part of its gates
+                for item in plan_phases(ops, free_diagonals=free is not None):
+                    if item[0] == 'phase':
+                        for o in self._encode_fitting(item[1], item[2], pos_index, T,
+                                                      coefs, srcs):
+                            ops_host.append(o)
                             self.n_phases += 1
-                        else:
-                            ops_host.append(_encode_op(item[1], pos_index, coefs, srcs))
-                        fits = fits and len(coefs) - c0 <= MAX_STAGED
-                    if fits:
-                        break
+                    else:
+                        ops_host.append(_encode_op(item[1], pos_index, coefs, srcs))
                 block, flags = self.block, 0
                 # a last register phase stores straight to HBM when its lanes
                 # run over the low (contiguous) tile bits
@@ -1129,6 +1194,21 @@ class HostPlan:
             arr = (_native.QfxTileOp * max(1, nops))(*ops_host[start:start + nops])
             _native.check(lib.qfx_tile_check_ops(T, block, flags, arr, nops, ncoef,
                                                  src_arr, len(srcs)), 'qfx_tile_check_ops')
+
+    @staticmethod
+    def _encode_fitting(R, members, pos_index, T, coefs, srcs):
+        """Register phase(s) for ``members`` on bits R, each staging at most
+        MAX_STAGED coefficients: a phase over the limit is cut into two
+        consecutive phases of its first and second half (each leaves the
+        tile in logical order, so they compose)."""
+        c0, s0 = len(coefs), len(srcs)
+        o = _encode_phase(R, members, pos_index, T, coefs, srcs)
+        if len(coefs) - c0 <= MAX_STAGED or len(members) == 1:
+            return [o]
+        del coefs[c0:], srcs[s0:]
+        h = len(members) // 2
+        return (HostPlan._encode_fitting(R, members[:h], pos_index, T, coefs, srcs)
+                + HostPlan._encode_fitting(R, members[h:], pos_index, T, coefs, srcs))
 
     def _add_tile_passes(self, k, ops_host, start, coefs, c_lo, srcs, block, flags):
         """Record tile pass k (ops_host[start:]) with coefficient indices

@@ -144,7 +144,12 @@ def _phase_unitary(o, coefs, srcs, sel_val):
         if typ == fusion.SUB_DIAGV:
             m = code & 3
             v = sum(sel_val[(code >> (2 + 7 * j)) & 127] << j for j in range(m))
-            u = np.diag(coefs[c0 + 16 * v:c0 + 16 * v + 16]) @ u
+            if mask:                        # compact tables over register bits d
+                d = [j for j in range(4) if (mask >> j) & 1]
+                idx = [sum(((r >> j) & 1) << a for a, j in enumerate(d)) for r in range(16)]
+                u = np.diag(coefs[c0 + (v << len(d)) + np.asarray(idx)]) @ u
+            else:
+                u = np.diag(coefs[c0 + 16 * v:c0 + 16 * v + 16]) @ u
             continue
         if typ == fusion.SUB_CX:
             # selector controls (mask): selector j must equal bit 23 + j
@@ -586,17 +591,20 @@ def test_selector_diagonals_replay(n, T, seed):
     ref = O.run_circuit(desc, qubits=tuple(range(n)))
     plan = fusion.HostPlan(gates, tuple(range(n)), tile_bits=T, low_bits=3, block=256,
                            fuse_width=1, mode='reg')
-    nsel = 0
+    nsel = ncompact = 0
     ket = O.zero_state(n)
     for kind, payload in plan.passes:
         pos, start, nops, block, flags, c_lo, ncoef, spos = payload
         ops = plan.ops_host[start:start + nops]
         for o in ops:
             if o.kind == fusion.OP_RPHASE:
-                nsel += sum(plan.srcs[o.src + 4 * q] == fusion.SUB_DIAGV for q in range(o.coef))
+                for q in range(o.coef):
+                    if plan.srcs[o.src + 4 * q] == fusion.SUB_DIAGV:
+                        nsel += 1
+                        ncompact += plan.srcs[o.src + 4 * q + 3] != 0
         ket = _emulate_ops(ket, n, pos, ops, plan.coefs[c_lo:c_lo + ncoef], plan.srcs)
         ket = apply_store(ket, n, pos, spos)
-    assert nsel > 0
+    assert nsel > 0 and ncompact > 0
     assert np.max(np.abs(ket - ref)) < 1e-12
     # without selector diagonals the same circuit needs more passes
     bits_list = [tuple(n - 1 - q for q in g.qubits) for g in gates]
@@ -626,6 +634,14 @@ def test_selector_diagonal_validator_rules():
     assert check(0, 16) == _native.QFX_EINVAL
     assert check(1 | (5 << 2) | (1 << 9), 32) == _native.QFX_EINVAL      # stray bits
     assert check(3 | (10 << 2) | (11 << 9) | (12 << 16), 128) == 0
+    # compact tables (fourth word 16 | d): 2^popcount(d) phases per table
+    def check_c(code, w, ncoef):
+        srcs = _native.int_array([fusion.SUB_DIAGV, code, 0, w])
+        return lib.qfx_tile_check_ops(8, 256, 0, arr, 1, ncoef, srcs, 4)
+    assert check_c(2 | (20 << 2) | (33 << 9), 16 | 0b0101, 16) == 0
+    assert check_c(2 | (20 << 2) | (33 << 9), 16 | 0b0101, 15) == _native.QFX_EINVAL
+    assert check_c(1 | (20 << 2), 16, 2) == 0                        # no register bit
+    assert check_c(1 | (20 << 2), 32 | 1, 64) == _native.QFX_EINVAL  # stray flag bits
     # lane-group selectors: tile bits outside the registers (here t = 0..3)
     assert check(1 | ((64 + 5) << 2), 32) == 0
     assert check(1 | ((64 + 2) << 2), 32) == _native.QFX_EINVAL         # a register bit
