@@ -13,6 +13,8 @@ place, to the working state.  Prints one JSON line:
   roofline   dominant kernel: algorithmic bytes per launch (32 * 2^28 B: one
              read + one write of the state) / its average duration, timed
              with HIP events on the launching stream inside the timed region
+  tile_pass_fp64  the same launches against the FP64 peak: the plan's
+             algorithmic floating-point operations per launch / duration
   fused_mfma_blocks  the circuit fused into dense 3/4/5-qubit blocks on the
              matrix cores (FP64 MFMA)
   cpu_baseline  the numpy oracle (np.einsum per gate, 1 thread) on a
@@ -34,6 +36,7 @@ N_QUBITS = 28
 DEPTH = 40
 SEED = 2024
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
 METRIC = "28q c128 gates/s as % HBM roofline; QAOA value+grad/s vs plain, same GPU"
 
 
@@ -580,6 +583,15 @@ def main():
             "traffic": measured_traffic(plan, n),
             "kernel": kname, "avg_ms": round(avg_ms, 4), "launches_timed": launches,
             "bytes_per_launch": bytes_per_pass}
+    fp64 = None
+    if plan is not None and kname == "tile_pass":
+        # the same launches against the FP64 peak: the plan's algorithmic
+        # floating-point work (sub-op counts, fusion.HostPlan.flops)
+        fl = plan.flops()
+        per_launch = sum(fl) / len(fl)
+        fp64 = {"flops_per_launch": round(per_launch), "achieved_TFLOPs":
+                round(per_launch / (avg_ms * 1e-3) / 1e12, 2), "peak_TFLOPs": FP64_PEAK_TFLOPS,
+                "frac": round(per_launch / (avg_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4)}
 
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "gates/s", "n_gpus": 1,
@@ -595,6 +607,7 @@ def main():
             100.0 * value / gates_per_step * (plan.n_passes if plan is not None else gates_per_step)
             * bytes_per_pass / (HBM_PEAK_GBS * 1e9), 2),
         "roofline": roof,
+        "tile_pass_fp64": fp64,
     }
     if plan is not None:
         out["config"].update(plan.stats())

@@ -969,19 +969,22 @@ def _rebase_coef(o, srcs, off):
 _SWAP_KIND = None
 
 
-def virtualize_swaps(kinds, bits_list, n):
-    """Remove uncontrolled SWAP-type gates (SWAP, ISWAP, PSWAP) from a gate
-    list by renaming: after a swap of index bits x and y the data of x sits
-    where y's was, so later gates act on the swapped bits and the gate moves
-    nothing.  Its phases (ISWAP, PSWAP) stay as a diagonal on the renamed
-    bits.  Plain SWAPs appended at the end put every bit's data back in
-    place, one transposition per bit still displaced, cycle by cycle.
-    Returns (kinds, bits, number of swaps removed)."""
+def _swap_kind():
     global _SWAP_KIND
     if _SWAP_KIND is None:
         _SWAP_KIND = bk.classify(np.array([[1, 0, 0, 0], [0, 0, 1, 0], [0, 1, 0, 0], [0, 0, 0, 1]],
                                           dtype=np.complex128))
-    where = list(range(n))              # index bit -> where its data sits
+    return _SWAP_KIND
+
+
+def rename_swaps(kinds, bits_list, n):
+    """Remove uncontrolled SWAP-type gates (SWAP, ISWAP, PSWAP) from a gate
+    list by renaming: after a swap of index bits x and y the data of x sits
+    where y's was, so later gates act on the swapped bits and the gate moves
+    nothing.  Its phases (ISWAP, PSWAP) stay as a diagonal on the renamed
+    bits.  Returns (kinds, bits, where, number removed): where[b] is the
+    index bit holding b's data after the last gate."""
+    where = list(range(n))
     out_k, out_b = [], []
     removed = 0
     for kd, bits in zip(kinds, bits_list):
@@ -996,30 +999,156 @@ def virtualize_swaps(kinds, bits_list, n):
             continue
         out_k.append(kd)
         out_b.append(tuple(where[b] for b in bits))
-    # restore: walk each cycle of the displacement; swapping the data at
-    # where[b] into b fixes b and passes the displaced data along the cycle,
-    # so every swap of the cycle shares the position its walk starts from.
-    # Starting where the cycle's highest position holds its data keeps that
-    # shared position off the low bits (swaps there are register exchanges,
-    # elsewhere store permutations).
+    return out_k, out_b, where, removed
+
+
+def restoring_swaps(where, pivot='low'):
+    """Plain swaps (as index-bit pairs, in order) that move the data of every
+    bit b from where[b] back to b.  Each walks one cycle of the
+    displacement: swapping the data at where[b] into b fixes b and passes
+    the displaced data along the cycle, so every swap of the cycle shares
+    the position its walk starts from, one swap per bit of the cycle but
+    one.  pivot 'low' starts where the cycle's lowest position holds its
+    data, making that shared position a low bit when the cycle has one (the
+    low bits are in every tile, so each swap needs one more tile bit only);
+    'high' starts at its highest position, keeping the swaps off the low
+    bits (pass-level store permutations instead of register relabelling)."""
+    where = list(where)
     holder = {p: b for b, p in enumerate(where)}     # position -> bit whose data is there
-    for b0 in range(n):
+    out = []
+    for b0 in range(len(where)):
         if where[b0] == b0:
             continue
         cyc, b = [b0], holder[b0]
         while b != b0:
             cyc.append(b)
             b = holder[b]
-        b = max(cyc, key=lambda c: where[c])
+        b = (min if pivot == 'low' else max)(cyc, key=lambda c: where[c])
         while where[b] != b:
             p = where[b]
-            out_k.append(_SWAP_KIND)
-            out_b.append((p, b))
+            out.append((p, b))
             other = holder[b]               # the bit whose data sat at b moves to p
             where[other], holder[p] = p, other
             where[b], holder[b] = b, b
             b = other
+    return out
+
+
+def virtualize_swaps(kinds, bits_list, n, pivot='low'):
+    """rename_swaps, then the restoring swaps appended: the same circuit.
+    Returns (kinds, bits, number of swaps removed)."""
+    out_k, out_b, where, removed = rename_swaps(kinds, bits_list, n)
+    for pq in restoring_swaps(where, pivot):
+        out_k.append(_swap_kind())
+        out_b.append(pq)
     return out_k, out_b, removed
+
+
+def plan_passes_restoring(kinds, bits_list, where, n, T, L, prepare, pivot='low'):
+    """plan_passes (commutation-aware) for a gate list renamed by
+    rename_swaps, whose data ends displaced (where[b]: the index bit holding
+    b's data after the last gate).  After each pass is chosen, its store
+    permutation also moves data home where that costs nothing: for tile
+    positions p above the low L bits whose data belongs (at the end) to a
+    bit q that is also such a tile position, the pass stores p to q (plain
+    swaps appended to the pass, which relabel_swaps absorbs), and every
+    later gate is renamed accordingly.  The swaps that restore what is
+    still displaced trail the pending gates, recomputed after each pass,
+    and join passes like any gate.  ``prepare(kinds, bits)`` gives
+    (tileable, free, need, types) per gate.  Returns (passes, kinds, bits):
+    the kinds and bits lists extended by the swaps the passes hold."""
+    kinds, bits_list = list(kinds), list(bits_list)
+    tileable, free, need, types = (list(x) if x is not None else None
+                                   for x in prepare(kinds, bits_list))
+    low = set(range(min(L, n)))
+    final = list(where)                     # bit -> index bit holding its data at the end
+
+    def add_gate(kd, bits):
+        t, f, nd, ty = prepare([kd], [bits])
+        kinds.append(kd)
+        bits_list.append(tuple(bits))
+        tileable.append(t[0])
+        if free is not None:
+            free.append(f[0])
+        if need is not None:
+            need.append(nd[0] if nd is not None else tuple(bits))
+        types.append(ty[0])
+        return len(kinds) - 1
+
+    def rename(i, m):
+        bits = tuple(m.get(b, b) for b in bits_list[i])
+        bits_list[i] = bits
+        t, f, nd, ty = prepare([kinds[i]], [bits])
+        if need is not None:
+            need[i] = nd[0] if nd is not None else bits
+        types[i] = ty[0]
+
+    def tail_swaps():
+        return [add_gate(_swap_kind(), pq) for pq in restoring_swaps(final, pivot)]
+
+    tail = set(tail_swaps())
+    pending = list(range(len(kinds) - len(tail))) + sorted(tail)
+    passes = []
+    while pending:
+        first = pending[0]
+        if not tileable[first]:
+            passes.append((None, [first]))
+            pending = pending[1:]
+            continue
+        S, _ = _grow(bits_list, pending, set(low), T, tileable, free=free, types=types,
+                     need=need)
+        b = 0
+        while len(S) < T:
+            if b not in S:
+                S.add(b)
+            b += 1
+        taken = _absorb(bits_list, pending, S, tileable, free, types, need)
+        done = set(taken)
+        # restoring swaps taken by this pass have moved their data
+        for i in taken:
+            if i in tail:
+                p, q = bits_list[i]
+                final = [q if f == p else p if f == q else f for f in final]
+        # store permutation: move data home among the tile's positions above
+        # its contiguous low run (those keep the store coalesced)
+        run = 0
+        while run in S:
+            run += 1
+        upper = {q for q in S if q >= max(run, L)}
+        holder = {f: bb for bb, f in enumerate(final)}
+        sigma = {}                          # data at position p goes to sigma[p]
+        for pos in sorted(upper):
+            dest = holder[pos]              # the bit whose data ends at pos
+            if dest != pos and dest in upper:
+                sigma[pos] = dest
+        extra = []
+        if sigma:
+            # complete the partial map to a permutation of the upper positions
+            free_dst = sorted(upper - set(sigma.values()))
+            for pos in sorted(upper - set(sigma)):
+                d = free_dst.pop(0)
+                if d != pos:
+                    sigma[pos] = d
+            # each cycle c0 -> c1 -> ... as swaps (c0, c1), (c0, c2), ...
+            seen = set()
+            for c0 in sorted(sigma):
+                if c0 in seen:
+                    continue
+                c = sigma[c0]
+                seen.add(c0)
+                while c != c0:
+                    extra.append(add_gate(_swap_kind(), (c0, c)))
+                    seen.add(c)
+                    c = sigma[c]
+            final = [sigma.get(f, f) for f in final]
+        passes.append((sorted(S), taken + extra))
+        pending = [i for i in pending if i not in done and i not in tail]
+        for i in pending:
+            if sigma:
+                rename(i, sigma)
+        tail = set(tail_swaps())
+        pending += sorted(tail)
+    return passes, kinds, bits_list
 
 
 class HostPlan:
@@ -1054,36 +1183,55 @@ class HostPlan:
         self.reg_in = reg_in
         self.free_controls = free_controls
         self.gates = list(gates)
-        bits_list = [tuple(n - 1 - qubits.index(q) for q in g.qubits) for g in gates]
-        kinds = [g.kind for g in gates]
-        # uncontrolled SWAP-type gates move no data where they stand: later
-        # gates are renamed onto the swapped bits, and swaps appended at the
-        # end restore the order (passes then absorb them as store permutations)
+        bits0 = [tuple(n - 1 - qubits.index(q) for q in g.qubits) for g in gates]
+        kinds0 = [g.kind for g in gates]
+
+        def prepare(kinds, bits_list):
+            """tileable, free, need and commutation types of a gate list"""
+            tileable = [len(b) <= 5 and len(b) <= T for b in bits_list]
+            # register phases apply diagonal gates whatever the tile bits:
+            # the phases of the qubits outside the tile are selected per tile
+            # (only where no pass can turn wide, i.e. every gate acts on <= 3
+            # qubits), and controlled X gates need only their target in the
+            # tile: controls outside it are read from the tile's index per tile
+            free = need = None
+            if self.mode == 'reg' and all(len(b) <= 3 for b in bits_list):
+                free = [kd.kind == bk.GateKind.DIAGONAL for kd in kinds]
+                if free_controls:
+                    need = []
+                    for kd, b in zip(kinds, bits_list):
+                        cx = cx_parts(kd, b)
+                        need.append((cx[0],) if cx else b)
+            types = [gate_types(kd, b) for kd, b in zip(kinds, bits_list)]
+            return tileable, free, need, types
+
         self.n_virtual = 0
         if self.mode == 'reg' and virtual_swaps:
-            kinds, bits_list, self.n_virtual = virtualize_swaps(kinds, bits_list, n)
-        tileable = [len(b) <= 5 and len(b) <= T for b in bits_list]
-        # register phases apply diagonal gates whatever the tile bits: the
-        # phases of the qubits outside the tile are selected per tile (only
-        # where no pass can turn wide, i.e. every gate acts on <= 3 qubits)
-        # and controlled X gates need only their target in the tile: controls
-        # outside it are read from the tile's index per tile
-        free = need = None
-        if self.mode == 'reg' and all(len(b) <= 3 for b in bits_list):
-            free = [kd.kind == bk.GateKind.DIAGONAL for kd in kinds]
-            if free_controls:
-                need = []
-                for kd, b in zip(kinds, bits_list):
-                    cx = cx_parts(kd, b)
-                    need.append((cx[0],) if cx else b)
-        # commutation-aware overtaking can pack the same gates into fewer
-        # passes, or into as many heavier ones: keep it only when it saves a pass
-        types = [gate_types(kd, b) for kd, b in zip(kinds, bits_list)]
-        planned = plan_passes(bits_list, n, T, low_bits, tileable, free=free, need=need)
-        typed = plan_passes(bits_list, n, T, low_bits, tileable, free=free, types=types,
-                            need=need)
-        if len(typed) < len(planned):
-            planned = typed
+            # uncontrolled SWAP-type gates move no data where they stand:
+            # later gates are renamed onto the swapped bits, and swaps
+            # appended at the end restore the order.  Both walks of the
+            # restoring cycles are planned (commutation-aware); the fewer
+            # passes win, ties going to the high walk (fewer phases)
+            rk, rb, where, self.n_virtual = rename_swaps(kinds0, bits0, n)
+            best = None
+            for pivot in ('high', 'low'):
+                planned, kinds, bits_list = plan_passes_restoring(
+                    rk, rb, where, n, T, low_bits, prepare, pivot=pivot)
+                if best is None or len(planned) < len(best[0]):
+                    best = (planned, kinds, bits_list)
+            planned, kinds, bits_list = best
+            tileable, free, need, types = prepare(kinds, bits_list)
+        else:
+            kinds, bits_list = kinds0, bits0
+            tileable, free, need, types = prepare(kinds, bits_list)
+            # commutation-aware overtaking can pack the same gates into fewer
+            # passes, or into as many heavier ones: keep it only when it saves
+            # a pass
+            planned = plan_passes(bits_list, n, T, low_bits, tileable, free=free, need=need)
+            typed = plan_passes(bits_list, n, T, low_bits, tileable, free=free, types=types,
+                                need=need)
+            if len(typed) < len(planned):
+                planned = typed
         self.passes = []           # (kind, payload)
         ops_host, coefs, srcs = [], [], []
         self.n_phases = 0
@@ -1246,6 +1394,35 @@ class HostPlan:
             self.passes.append(('tile', (k, start + a, b - a, block, fl, bounds[a],
                                          bounds[b] - bounds[a], b == len(ops))))
             a = b
+
+    # real FP64 operations per amplitude: a complex multiply is 6, a complex
+    # multiply-add 8; a 2x2 real matrix on a complex pair is 6 per output
+    _SUB_FLOPS = {SUB_D1: 14, SUB_D1R: 6, SUB_D1X: 6, SUB_DIAG: 6, SUB_DIAGV: 6, SUB_CX: 0}
+
+    def flops(self):
+        """Algorithmic FP64 operations of the plan's tile passes, per pass
+        launch in order: 2^n times the per-amplitude count of every op (a
+        register phase: its sub-ops; a dense k-qubit op: 2^k complex
+        multiply-adds less one add, 8 * 2^k - 2; a sparse op: one complex
+        multiply; a matrix-core block: 8 * 2^k)."""
+        out = []
+        for kind, payload in self.passes:
+            if kind != 'tile':
+                continue
+            _, start, nops, *_ = payload
+            per_amp = 0
+            for o in self.ops_host[start:start + nops]:
+                if o.kind == OP_RPHASE:
+                    per_amp += sum(self._SUB_FLOPS[self.srcs[o.src + 4 * q]]
+                                   for q in range(o.coef))
+                elif o.kind == OP_DENSE:
+                    per_amp += 8 * 2 ** o.k - 2
+                elif o.kind == OP_SPARSE:
+                    per_amp += 6
+                else:
+                    per_amp += 8 * 2 ** o.k
+            out.append(per_amp * 2 ** self.n)
+        return out
 
     def stats(self):
         ng = len(self.gates)

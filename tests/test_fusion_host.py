@@ -499,10 +499,13 @@ def test_pass_split_at_staging_limit(monkeypatch):
     assert np.max(np.abs(ket - ref)) < 1e-12
 
 
-def test_pass_level_swap_relabel():
+@pytest.mark.parametrize("virtual", [False, True])
+def test_pass_level_swap_relabel(virtual):
     """SWAP/ISWAP gates in a pass become a relabelling of tile bits plus the
     ISWAP phases; the pass stores through a permutation; replaying with the
-    store permutation gives the circuit's state exactly."""
+    store permutation gives the circuit's state exactly.  With virtual swaps
+    the SWAP-type gates are renamed away before planning and restored by
+    swaps at the end; the replay is exact as well."""
     n, T = 10, 8
     rng = np.random.default_rng(12)
     names = [('SWAP', 2), ('ISWAP', 2), ('RX', 1), ('CNOT', 2), ('CZ', 2), ('H', 1)]
@@ -516,8 +519,11 @@ def test_pass_level_swap_relabel():
     psi = (psi / np.linalg.norm(psi)).reshape([2] * n)
     ref = O.run_circuit(desc, qubits=tuple(range(n)), ket=psi)
     plan = fusion.HostPlan(gates, tuple(range(n)), tile_bits=T, low_bits=3, block=256,
-                           fuse_width=1, mode='reg')
-    assert any(p[1][7] is not None for p in plan.passes)
+                           fuse_width=1, mode='reg', virtual_swaps=virtual)
+    if virtual:
+        assert plan.n_virtual == sum(d[0] in ('SWAP', 'ISWAP') for d in desc) > 0
+    else:
+        assert any(p[1][7] is not None for p in plan.passes)
     ket = psi
     for kind, payload in plan.passes:
         pos, start, nops, block, flags, c_lo, ncoef, spos = payload
@@ -785,3 +791,56 @@ def test_pass_of_only_absorbed_swaps_keeps_its_store_permutation():
                            plan.coefs[c_lo:c_lo + ncoef], plan.srcs)
         ket = apply_store(ket, n, pos, spos)
     assert np.array_equal(ket, ref)
+
+
+def test_plan_flops_closed_form():
+    """HostPlan.flops: 2^n times the per-amplitude FP64 operations of each
+    sub-op: a real or RX-type 2x2 matrix 6, a general 2x2 matrix 14 (two
+    complex multiplies and an add per output), a phase 6 (one complex
+    multiply), an exchange 0."""
+    import quantumflow_amd as qf
+    n = 8
+    cases = [([qf.H(0)], 6), ([qf.RX(0.3, 1)], 6), ([qf.RZ(0.3, 2)], 6), ([qf.CNOT(0, 5)], 0),
+             ([qf.TX(0.3, 1)], 14), ([qf.H(0), qf.RZ(0.2, 3), qf.CNOT(1, 2), qf.RY(0.4, 4)], 18)]
+    for gl, per_amp in cases:
+        plan = fusion.HostPlan(gl, tuple(range(n)), tile_bits=8, low_bits=3, block=16,
+                               fuse_width=1, mode='reg')
+        assert plan.flops() == [per_amp * 2 ** n], [g.name for g in gl]
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_virtualize_swaps_is_the_same_circuit(seed):
+    """virtualize_swaps: the renamed gate list with its restoring swaps,
+    applied gate by gate by the oracle, gives the original circuit's state;
+    no SWAP-type gate is left before the restoring tail of plain swaps, at
+    most n - 1 of them (one per displaced bit, less one per cycle)."""
+    n = 9
+    rng = np.random.default_rng(seed)
+    names = [('SWAP', 2), ('ISWAP', 2), ('RX', 1), ('CNOT', 2), ('CZ', 2), ('H', 1),
+             ('CCNOT', 3), ('PSWAP', 2)]
+    desc = []
+    for _ in range(60):
+        nm, k = names[int(rng.integers(len(names)))]
+        params = (float(rng.uniform(0, 6)),) if nm in ('RX', 'PSWAP') else ()
+        desc.append((nm, params, tuple(int(q) for q in rng.permutation(n)[:k])))
+    gates = build_gates(desc)
+    psi = rng.normal(size=2 ** n) + 1j * rng.normal(size=2 ** n)
+    psi = (psi / np.linalg.norm(psi)).reshape([2] * n)
+    ref = O.run_circuit(desc, qubits=tuple(range(n)), ket=psi)
+    bits = [tuple(n - 1 - q for q in g.qubits) for g in gates]
+    kinds, nbits, removed = fusion.virtualize_swaps([g.kind for g in gates], bits, n)
+    assert removed == sum(d[0] in ('SWAP', 'ISWAP', 'PSWAP') for d in desc)
+    ket = psi
+    for kd, b in zip(kinds, nbits):
+        m = kd.matrix.reshape([2] * (2 * len(b)))
+        ket = O.tensormul(m, ket, [n - 1 - x for x in b])
+    assert np.max(np.abs(ket - ref)) < 1e-12
+    # the tail: plain swaps only; before it, no SWAP-type gate remains
+    tail = 0
+    while tail < len(kinds) and fusion.exchange_pattern(kinds[-1 - tail]) is not None \
+            and fusion.exchange_pattern(kinds[-1 - tail])[0] == 'swap':
+        tail += 1
+    for kd in kinds[:len(kinds) - tail]:
+        pat = fusion.exchange_pattern(kd) if kd.kind == 'monomial' else None
+        assert pat is None or pat[0] != 'swap' or pat[3]
+    assert tail <= n - 1
