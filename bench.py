@@ -332,6 +332,70 @@ def qaoa_batches_only(steps, pass_times=False):
     return out
 
 
+def subop_costs(n=N_QUBITS, reps=5):
+    """Time per register-phase sub-op kind: one-pass plans on an n-qubit
+    state whose gates all act on the tile's bits (index bits 3..11 as
+    targets; controls and phase qubits on bits 20..27, outside the tile),
+    16 and 48 gates of one kind, HIP events around `reps` runs.  The
+    difference between the two sizes over the difference in sub-ops is the
+    cost of one more sub-op of that kind."""
+    import numpy as np
+    import torch
+    import quantumflow_amd as qf
+    from quantumflow_amd import backend as bk
+    from quantumflow_amd import fusion
+    from collections import Counter
+    rng = np.random.default_rng(0)
+
+    def q(b):                       # index bit -> qubit
+        return n - 1 - b
+    names = {fusion.SUB_D1: "D1", fusion.SUB_D1R: "D1R", fusion.SUB_D1X: "D1X",
+             fusion.SUB_DIAG: "DIAG", fusion.SUB_DIAGV: "DIAGV", fusion.SUB_CX: "CX"}
+
+    def gates(kind, k):
+        out = []
+        for i in range(k):
+            t = q(3 + i % 9)
+            if kind == "rx":
+                out.append(qf.RX(float(rng.uniform(0, 6)), t))
+            elif kind == "ry":
+                out.append(qf.RY(float(rng.uniform(0, 6)), t))
+            elif kind == "cx_in_registers":
+                a, b = rng.choice(9, 2, replace=False) + 3
+                out.append(qf.CNOT(q(int(a)), q(int(b))))
+            elif kind == "cx_control_outside_tile":
+                out.append(qf.CNOT(q(20 + i % 8), t))
+            elif kind == "cz_outside_tile":
+                out.append(qf.CZ(q(20 + i % 8), t))
+        return out
+    work = bk.zero_state_tensor(n).reshape(-1).clone()
+    out = {}
+    for kind in ("rx", "ry", "cx_in_registers", "cx_control_outside_tile", "cz_outside_tile"):
+        row = {}
+        for k in (16, 48):
+            plan = fusion.TilePlan(gates(kind, k), tuple(range(n)), tile_bits=12, low_bits=3,
+                                   block=256, fuse_width=1, mode="reg")
+            subs = Counter(names[plan.srcs[o.src + 4 * j]] for o in plan.ops_host
+                           if o.kind == fusion.OP_RPHASE for j in range(o.coef))
+            plan.run_inplace(work)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                plan.run_inplace(work)
+            e1.record()
+            torch.cuda.synchronize()
+            row["gates_%d" % k] = {"ms_per_pass": round(e0.elapsed_time(e1) / reps / plan.n_passes, 4),
+                                   "passes": plan.n_passes, "phases": plan.n_phases,
+                                   "sub_ops": dict(subs)}
+        a, b = row["gates_16"], row["gates_48"]
+        extra = sum(b["sub_ops"].values()) - sum(a["sub_ops"].values())
+        if extra > 0 and a["passes"] == b["passes"] == 1:
+            row["us_per_extra_sub_op"] = round((b["ms_per_pass"] - a["ms_per_pass"]) * 1e3 / extra, 1)
+        out[kind] = row
+    return out
+
+
 def config1_section(reps=20):
     """The 12-qubit, 20-layer rotation circuit (360 gates) from |0...0>:
     the state is 64 KiB, so this is launch- and latency-bound: microseconds
@@ -510,13 +574,16 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one plain device copy of the state (4 GiB read + 4 GiB "
                          "written at 28 qubits) to scale HBM byte counters against")
-    ap.add_argument("--only", choices=["qaoa"],
-                    help="run only the batched QAOA launches (for profilers) and print "
-                         "their timings")
+    ap.add_argument("--only", choices=["qaoa", "subops"],
+                    help="qaoa: run only the batched QAOA launches (for profilers) and print "
+                         "their timings; subops: time one-pass plans per sub-op kind")
     ap.add_argument("--pass-times", action="store_true",
                     help="with --only qaoa: also report each global pass's span and busy "
                          "times from the kernel's wall-clock stamps")
     args = ap.parse_args()
+    if args.only == "subops":
+        print(json.dumps({"subop_costs": subop_costs()}), flush=True)
+        return
     if args.only == "qaoa":
         res = {"qaoa_batches": qaoa_batches_only(max(1, args.steps), args.pass_times)}
         if args.copy_calibration:
