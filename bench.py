@@ -314,8 +314,12 @@ def qaoa_batches_only(steps, pass_times=False):
             batch.replay()
         torch.cuda.synchronize()
         batch.check()
-        out[name] = {"instances": len(graphs), "ms_per_batch":
-                     round((time.perf_counter() - t0) / steps * 1e3, 3)}
+        ms = (time.perf_counter() - t0) / steps * 1e3
+        sb, cb = batch.algorithmic_bytes()
+        out[name] = {"instances": len(graphs), "ms_per_batch": round(ms, 3),
+                     "algorithmic_state_GB": round(sb / 1e9, 3),
+                     "algorithmic_cut_GB": round(cb / 1e9, 3),
+                     "algorithmic_TBs": round((sb + cb) / (ms * 1e-3) / 1e12, 3)}
         if pass_times:
             pt = batch.pass_times()
             span = sum(q["span_us"] for q in pt["passes"])

@@ -328,7 +328,16 @@ typedef struct {
   int32_t angle_off;  /* beta[0..p-1] then gamma[0..p-1] in angles */
   int32_t slot0;      /* first partial slot of this instance */
   int32_t nslots;     /* partial slots of this instance */
-  int32_t pad;
+  int32_t mirror;     /* 1: the instance runs on half of its state.  MaxCut
+                         states are symmetric under flipping every bit, so an
+                         amplitude whose top index bit (n-1) is set equals that
+                         of the complementary index: psi and lambda hold the
+                         2^(n-1) amplitudes with bit n-1 clear, every pass has
+                         bit n-1 as its highest tile bit, its items cover only
+                         the tiles whose highest non-tile bit is clear, and an
+                         element with bit n-1 set is loaded from and stored to
+                         the complementary index.  Values and gradients count
+                         each item twice.  0: the whole state. */
 } qfx_qaoa_inst;
 
 typedef struct {

@@ -80,7 +80,17 @@ struct Ctx {
   const double* gamma;
   const uint16_t* cnt;  // cut counts of this instance (or null)
   const double* cval;   // weighted cut values (or null)
+  uint64_t mmask;       // mirror instances: all n index bits; else 0
+  int mshift;           // mirror instances: n - 1
 };
+
+// Where the amplitude of full-state index z is kept: a mirror instance keeps
+// only the indices with bit n-1 clear, and one with it set is the
+// complementary index (MaxCut states are symmetric under flipping every
+// bit); other instances keep z itself.
+__device__ __forceinline__ uint64_t stored(const Ctx& c, uint64_t z) {
+  return z ^ (c.mmask & (0ull - ((z >> c.mshift) & 1ull)));
+}
 
 __device__ __forceinline__ double cut_of(const Ctx& c, uint64_t z) {
   return c.cnt ? (double)c.cnt[z] : c.cval[z];
@@ -115,7 +125,7 @@ __device__ __forceinline__ void for_elems(const Ctx& c, F&& f) {
   uint64_t zh = 0;
 #pragma unroll 4
   for (int k = 0; k < nk; ++k) {
-    f(threadIdx.x + (unsigned)k * BS, zl | zh);
+    f(threadIdx.x + (unsigned)k * BS, stored(c, zl | zh));
     zh = ((zh | ~hm) + h0) & hm;
   }
 }
@@ -175,7 +185,7 @@ __device__ __forceinline__ void io_tile_rx(const Ctx& c, v2d* L, v2d* g, unsigne
       const unsigned e = threadIdx.x + (unsigned)(k0 + i) * BS;
       v[i] = v2d{0.0, 0.0};
       if (LOAD) {
-        if (k0 + i < nk) v[i] = g[zl | zh];
+        if (k0 + i < nk) v[i] = g[stored(c, zl | zh)];
         zh = ((zh | ~hm) + h0) & hm;
       } else if (k0 + i < nk) {
         v[i] = L[swz(e)];
@@ -188,7 +198,7 @@ __device__ __forceinline__ void io_tile_rx(const Ctx& c, v2d* L, v2d* g, unsigne
       if (LOAD) {
         if (k0 + i < nk) L[swz(e)] = v[i];
       } else {
-        if (k0 + i < nk) g[zl | zh] = v[i];
+        if (k0 + i < nk) g[stored(c, zl | zh)] = v[i];
         zh = ((zh | ~hm) + h0) & hm;
       }
     }
@@ -306,7 +316,7 @@ __device__ __forceinline__ double rx_chunk(v2d* L, v2d* LL, int T, const int* bi
     if (COST) {
 #pragma unroll
       for (int r = 0; r < (1 << R); ++r) {
-        v[r] = cmul(cost_phase(cx, tab, zb | zoff[r]), v[r]);
+        v[r] = cmul(cost_phase(cx, tab, (uint32_t)stored(cx, zb | zoff[r])), v[r]);
       }
     }
     if (ADJ) {
@@ -406,6 +416,8 @@ __device__ void run_item(const qfx_qaoa_inst& in, const qfx_qaoa_pass& ps, uint3
   c.gamma = angles + in.angle_off + in.p;
   c.cnt = in.weighted ? nullptr : cnt_pool + in.cut_off;
   c.cval = in.weighted ? val_pool + in.cut_off : nullptr;
+  c.mmask = in.mirror ? ((uint64_t)1 << in.n) - 1 : 0;
+  c.mshift = in.mirror ? in.n - 1 : 0;
   uint64_t base = tile;
 #pragma unroll
   for (int j = 0; j < kMaxT; ++j) {
@@ -648,7 +660,8 @@ __global__ __launch_bounds__(kReduceBlock) void qaoa_reduce_kernel(
   s = wave_sum(s);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
   __syncthreads();
-  if (threadIdx.x == 0) out[i] = (red[0] + red[1]) + (red[2] + red[3]);
+  // a mirror instance's items each stand for their tile and its complement
+  if (threadIdx.x == 0) out[i] = ((red[0] + red[1]) + (red[2] + red[3])) * (in.mirror ? 2.0 : 1.0);
 }
 
 __global__ void zero_sync_kernel(unsigned* sync, int* err) {
@@ -753,9 +766,12 @@ int qfx_qaoa_check(const qfx_qaoa_inst* insts, int ninst, const qfx_qaoa_pass* p
       if (in.nedges < 0 || in.nedges >= kMaxEdgesTab)
         return qfx::set_error(QFX_ERANGE, "qfx_qaoa_check: too many edges for the phase table");
     }
-    if (in.psi_off < 0 || in.psi_off + len > psi_pool_len)
+    if (in.mirror != 0 && in.mirror != 1)
+      return qfx::set_error(QFX_EINVAL, "qfx_qaoa_check: mirror must be 0 or 1");
+    const int64_t slen = in.mirror ? len / 2 : len;   // amplitudes kept
+    if (in.psi_off < 0 || in.psi_off + slen > psi_pool_len)
       return qfx::set_error(QFX_EINVAL, "qfx_qaoa_check: state out of range");
-    if (mode == QFX_QAOA_BACKWARD && (in.lam_off < 0 || in.lam_off + len > lam_pool_len))
+    if (mode == QFX_QAOA_BACKWARD && (in.lam_off < 0 || in.lam_off + slen > lam_pool_len))
       return qfx::set_error(QFX_EINVAL, "qfx_qaoa_check: lambda out of range");
     if (in.slot0 < 0 || in.nslots < 0 || in.slot0 + in.nslots > npartial)
       return qfx::set_error(QFX_EINVAL, "qfx_qaoa_check: partial slots out of range");
@@ -809,6 +825,11 @@ int qfx_qaoa_check(const qfx_qaoa_inst* insts, int ninst, const qfx_qaoa_pass* p
     const qfx_qaoa_pass& ps = passes[item.pass];
     if (ps.T > in.n || item.tile < 0 || ((int64_t)item.tile >> (in.n - ps.T)) != 0)
       return qfx::set_error(QFX_EINVAL, "qfx_qaoa_check: tile out of range");
+    // mirror: bit n-1 is the pass's highest tile bit, the tile's highest
+    // non-tile bit is clear (so its complement tile is not an item)
+    if (in.mirror && (ps.T < 1 || ps.pos[ps.T - 1] != in.n - 1 || in.n - ps.T < 1 ||
+                      ((int64_t)item.tile >> (in.n - ps.T - 1)) != 0))
+      return qfx::set_error(QFX_EINVAL, "qfx_qaoa_check: mirror pass or tile out of range");
     for (int j = 0; j < ps.T; ++j)
       if (ps.pos[j] >= in.n) return qfx::set_error(QFX_EINVAL, "qfx_qaoa_check: tile bit >= n");
     for (int si = 0; si < ps.nstage; ++si)

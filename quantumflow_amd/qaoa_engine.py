@@ -50,7 +50,7 @@ class QaoaInst(ctypes.Structure):
                 ('nedges', ctypes.c_int32), ('wsum', ctypes.c_double),
                 ('cut_off', ctypes.c_int64), ('psi_off', ctypes.c_int64),
                 ('lam_off', ctypes.c_int64), ('angle_off', ctypes.c_int32),
-                ('slot0', ctypes.c_int32), ('nslots', ctypes.c_int32), ('pad', ctypes.c_int32)]
+                ('slot0', ctypes.c_int32), ('nslots', ctypes.c_int32), ('mirror', ctypes.c_int32)]
 
 
 class QaoaPass(ctypes.Structure):
@@ -84,6 +84,7 @@ def tile_sets(n, T, low=None):
         return [tuple(range(n))]
     if low is None:
         low = LOW_BITS if 2 * T - n >= LOW_BITS else (2 if 2 * T - n == 2 else LOW_BITS)
+    low = min(low, T - 1)               # every later set must take a new bit
     sets = [tuple(range(T))]
     lo = list(range(low))
     b = T
@@ -97,6 +98,21 @@ def tile_sets(n, T, low=None):
         sets.append(tuple(sorted(st)))
         b += T - low
     return sets
+
+
+def tile_sets_mirror(n, T, low=None):
+    """Tile bit sets for a mirror instance: like tile_sets over the bits
+    0..n-2 with T-1 bits each, every set then completed by bit n-1 as its
+    highest tile bit."""
+    top = n - 1
+    return [tuple(sorted(set(s_) | {top})) for s_ in tile_sets(n - 1, T - 1, low)]
+
+
+def use_mirror(n, T):
+    """Run an n-qubit instance on half its state (complement symmetry): when
+    it needs more than one pass and leaves at least one non-tile bit besides
+    the mirrored one (n >= T + 2)."""
+    return n >= T + 2 and T >= 2
 
 
 def fuse_stages(stages):
@@ -142,7 +158,7 @@ def io_stages(tset, stages, flags, block):
     return stages
 
 
-def build_program(n, p, mode, T, fuse=True, block=512):
+def build_program(n, p, mode, T, fuse=True, block=512, mirror=False):
     """Passes of one instance: list of (tile bits, [(stage, layer, global bit
     set)], flags).  With `fuse`, forward-mode programs merge each cost stage
     into the mixer stage of its layer (fuse_stages) and apply RX bits held
@@ -162,7 +178,7 @@ def build_program(n, p, mode, T, fuse=True, block=512):
         for l in reversed(range(p)):
             seq.append((ST_RX_ADJ, l, allbits))
             seq.append((ST_COST_ADJ, l, None))
-    sets = tile_sets(n, T)
+    sets = tile_sets_mirror(n, T) if mirror else tile_sets(n, T)
     passes = []
     cur = [sets[0], []]
     for typ, layer, bits in seq:
@@ -201,6 +217,26 @@ def build_program(n, p, mode, T, fuse=True, block=512):
     return out
 
 
+def program_bytes(n, p, mode, T, block=512, mirror=False, weighted=False, fuse=True):
+    """Algorithmic HBM bytes of one instance's program: (state, cut table).
+    State: 16 B per kept amplitude (2^n, or 2^(n-1) for a mirror instance)
+    for every load and store of psi and lambda a pass makes (a one-pass
+    instance keeps its state in LDS).  Cut table: 2 B (unit weights) or 8 B
+    per kept amplitude for every stage that reads the cost of each index
+    (cost layers, fused cost-mixer stages, <C>, lambda = C psi, adjoint cost
+    layers)."""
+    prog = build_program(n, p, mode, T, fuse, block, mirror=mirror)
+    kept = 2 ** (n - 1) if mirror else 2 ** n
+    reads = (ST_COST, ST_COST_RX, ST_EXPECT, ST_MAKE_LAMBDA, ST_COST_ADJ)
+    state = cut = 0
+    for _, stages, flags in prog:
+        if len(prog) > 1:
+            state += 16 * kept * sum(bool(flags & f) for f in
+                                     (PF_LOAD_PSI, PF_LOAD_LAM, PF_STORE_PSI, PF_STORE_LAM))
+        cut += (8 if weighted else 2) * kept * sum(st[0] in reads for st in stages)
+    return state, cut
+
+
 def graph_arrays(graph):
     n = len(graph)
     if sorted(graph.nodes()) != list(range(n)):
@@ -219,7 +255,8 @@ class QAOABatch:
     optional); p: layers; mode: MODE_FORWARD (values) or MODE_BACKWARD
     (values and gradients)."""
 
-    def __init__(self, graphs, p, mode=MODE_BACKWARD, block=None, tile_bits=None, fuse=True):
+    def __init__(self, graphs, p, mode=MODE_BACKWARD, block=None, tile_bits=None, fuse=True,
+                 mirror=True):
         if not 1 <= p <= MAX_LAYERS:
             raise ValueError('p must be 1..%d' % MAX_LAYERS)
         if block is None or tile_bits is None:
@@ -230,6 +267,8 @@ class QAOABatch:
         self.graphs = list(graphs)
         self.p = p
         self.mode = mode
+        self._mirror = bool(mirror)
+        self._fuse = fuse
         self.block = block
         self.width = 1 if mode == MODE_FORWARD else 1 + 2 * p
         T_max = 13 if mode == MODE_FORWARD else 12
@@ -252,9 +291,12 @@ class QAOABatch:
         for i, g in enumerate(self.graphs):
             n, edges, w, weighted = graph_arrays(g)
             self.n.append(n)
-            prog = build_program(n, p, mode, T, fuse, block)
+            mir = bool(mirror) and use_mirror(n, T)
+            prog = build_program(n, p, mode, T, fuse, block, mirror=mir)
             inst = insts[i]
             inst.n, inst.p, inst.weighted, inst.nedges = n, p, int(weighted), len(edges)
+            inst.mirror = int(mir)
+            kept = 2 ** (n - 1) if mir else 2 ** n          # amplitudes in the pools
             inst.wsum = float(np.sum(w))
             if weighted:
                 inst.cut_off = val_len
@@ -265,10 +307,10 @@ class QAOABatch:
             cut_jobs.append((n, edges, w, weighted, inst.cut_off))
             if len(prog) > 1:
                 inst.psi_off = psi_len
-                psi_len += 2 ** n
+                psi_len += kept
                 if mode == MODE_BACKWARD:
                     inst.lam_off = lam_len
-                    lam_len += 2 ** n
+                    lam_len += kept
             else:
                 # a one-pass instance never touches the pools; its offsets
                 # point at the start, which the pools are sized to cover
@@ -297,7 +339,8 @@ class QAOABatch:
                 pass_list.append(ps)
                 while len(items_by_k) <= q:
                     items_by_k.append([])
-                for t in range(2 ** (n - ps.T)):
+                # a mirror instance's items: tiles whose highest non-tile bit is clear
+                for t in range(2 ** (n - ps.T - (1 if mir else 0))):
                     items_by_k[q].append((i, pidx, t, slot if accum else -1, 2 ** n))
                     if accum:
                         slot += 1
@@ -374,6 +417,18 @@ class QAOABatch:
         a.max_items_per_k = max(kstart[k + 1] - kstart[k] for k in range(self.nk))
         a.tile_bits = T
         self._graph = None
+
+    def algorithmic_bytes(self):
+        """(state bytes, cut-table bytes) one launch moves, summed over the
+        instances (program_bytes)."""
+        st = ct = 0
+        for g, n in zip(self.graphs, self.n):
+            weighted = graph_arrays(g)[3]
+            a, b = program_bytes(n, self.p, self.mode, self.T, self.block,
+                                 self._mirror and use_mirror(n, self.T), weighted, self._fuse)
+            st += a
+            ct += b
+        return st, ct
 
     @staticmethod
     def _vp(t, byte_off=0):

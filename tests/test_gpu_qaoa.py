@@ -299,3 +299,35 @@ def test_program_runs_with_another_block_size(E):
         v, _ = b.evaluate(beta, gamma)
         b.check()
         assert torch.max(torch.abs(v - ref)).item() <= 1e-12
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_mirror_half_state_matches_full_state(E, mode):
+    """Instances run on half their state (complement symmetry of MaxCut:
+    psi(z) = psi(~z), items only over tiles whose highest non-tile bit is
+    clear, results counted twice) give the values and gradients of the
+    whole-state program, weighted graphs included; the mirror pools hold
+    half the amplitudes."""
+    rng = np.random.default_rng(12 + mode)
+    graphs = [er(n, 0.45, 200 + n) for n in (15, 16, 18, 21)]
+    g = er(17, 0.5, 77)
+    for u, v in g.edges():
+        g[u][v]['weight'] = float(rng.uniform(0.1, 2.0))
+    graphs.append(g)
+    p = 3
+    beta = rng.uniform(0, np.pi / 2, size=(len(graphs), p))
+    gamma = rng.uniform(0, np.pi, size=(len(graphs), p))
+    half = E.QAOABatch(graphs, p, mode=mode)
+    full = E.QAOABatch(graphs, p, mode=mode, mirror=False)
+    assert half.psi_pool.numel() < full.psi_pool.numel()
+    v1, g1 = half.evaluate(beta, gamma)
+    v2, g2 = full.evaluate(beta, gamma)
+    half.check()
+    full.check()
+    assert np.max(np.abs(v1.cpu().numpy() - v2.cpu().numpy())) <= 1e-11
+    if mode == 1:
+        assert np.max(np.abs(g1.cpu().numpy() - g2.cpu().numpy())) <= 1e-11
+    # and the oracle, for the largest unweighted instance
+    i = 3
+    ref = O.qaoa_expectation_fast(len(graphs[i]), list(graphs[i].edges()), beta[i], gamma[i])
+    assert abs(v1.cpu().numpy()[i] - ref) <= 1e-10

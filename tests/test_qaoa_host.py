@@ -16,11 +16,11 @@ from oracle import qf_oracle as O
 from quantumflow_amd import qaoa_engine as E
 
 
-def emulate(n, edges, weights, beta, gamma, mode, T, fuse=True, block=512):
+def emulate(n, edges, weights, beta, gamma, mode, T, fuse=True, block=512, mirror=False):
     cuts = O.graph_cuts(n, edges, weights).reshape(-1)
     W = float(len(edges) if weights is None else np.sum(weights))
     p = len(beta)
-    prog = E.build_program(n, p, mode, T, fuse, block)
+    prog = E.build_program(n, p, mode, T, fuse, block, mirror=mirror)
     idx = np.arange(2 ** n)
     psi = lam = None
     value = 0.0
@@ -95,6 +95,11 @@ def test_program_emulation_matches_oracle(n, T, p, fuse):
     assert np.max(np.abs(gg - sg)) < 1e-12
     v2, _, _ = emulate(n, edges, None, beta, gamma, E.MODE_FORWARD, T + 1, fuse)
     assert abs(v2 - ref) < 1e-12
+    if E.use_mirror(n, T):
+        # the half-state schedules (every set holding bit n-1) are complete too
+        vm, gbm, ggm = emulate(n, edges, None, beta, gamma, E.MODE_BACKWARD, T, fuse, mirror=True)
+        assert abs(vm - ref) < 1e-12
+        assert np.max(np.abs(gbm - sb)) < 1e-12 and np.max(np.abs(ggm - sg)) < 1e-12
     # small workgroups put register-held RX bits inside these small tiles
     for block in (2, 4, 8):
         v3, _, _ = emulate(n, edges, None, beta, gamma, E.MODE_FORWARD, T, fuse, block)
@@ -216,3 +221,88 @@ def test_autograd_forward_checks_status():
     v.sum().backward()
     assert ok.checks == 1
     assert torch.allclose(beta.grad, -2 * beta.detach())
+
+
+def test_program_bytes_closed_form():
+    """program_bytes: a forward program of p + 1 passes loads and stores the
+    state between passes, 2 p state transfers of 16 B per amplitude; a
+    mirror instance with the same pass count moves half; a one-pass instance
+    only reads its cut counts (2 B per amplitude per cost-reading stage)."""
+    from quantumflow_amd import qaoa_engine as E
+    n, p, T = 22, 4, 13
+    prog = E.build_program(n, p, E.MODE_FORWARD, T)
+    assert len(prog) == p + 1
+    st, ct = E.program_bytes(n, p, E.MODE_FORWARD, T)
+    assert st == 16 * 2 ** n * 2 * p
+    reads = sum(s[0] in (E.ST_COST, E.ST_COST_RX, E.ST_EXPECT) for _, stages, _ in prog
+                for s in stages)
+    assert reads == p + 1 and ct == 2 * 2 ** n * (p + 1)
+    mprog = E.build_program(n, p, E.MODE_FORWARD, T, mirror=True)
+    assert len(mprog) == len(prog)
+    assert all(tset[-1] == n - 1 and len(tset) == T for tset, _, _ in mprog)
+    mst, mct = E.program_bytes(n, p, E.MODE_FORWARD, T, mirror=True)
+    assert (mst, mct) == (st // 2, ct // 2)
+    st1, ct1 = E.program_bytes(10, p, E.MODE_FORWARD, T)
+    assert st1 == 0 and ct1 == 2 * 2 ** 10 * (p + 1)
+    assert E.use_mirror(T + 2, T) and not E.use_mirror(T + 1, T)
+
+
+def _emulate_mirror_tiles(n, edges, beta, gamma, T):
+    """A forward mirror program run tile by tile as the kernel runs it: the
+    state kept is the half with index bit n-1 clear; each item of a pass
+    gathers its tile (the pass's tile bits over the item's non-tile bits,
+    whose highest is clear), an element with bit n-1 set from the
+    complementary index; stages act on the tile; the tile is scattered back
+    the same way.  <C> sums every item twice."""
+    cuts = O.graph_cuts(n, edges).reshape(-1)
+    W = float(len(edges))
+    p = len(beta)
+    prog = E.build_program(n, p, E.MODE_FORWARD, T, fuse=False, mirror=True)
+    full = (1 << n) - 1
+    kept = np.zeros(2 ** (n - 1), dtype=complex)
+
+    def stored(z):
+        return z ^ full if (z >> (n - 1)) & 1 else z
+
+    value = 0.0
+    for tset, stages, flags in prog:
+        assert tset[-1] == n - 1 and len(tset) == T
+        others = [b for b in range(n) if b not in tset]
+        for tile in range(2 ** (len(others) - 1)):
+            base = sum(((tile >> j) & 1) << b for j, b in enumerate(others))
+            zs = [base | sum(((e >> j) & 1) << b for j, b in enumerate(tset)) for e in range(2 ** T)]
+            at = np.array([stored(z) for z in zs])
+            v = kept[at].copy()
+            c = cuts[at]
+            for typ, layer, bits in stages:
+                if typ == E.ST_INIT:
+                    v[:] = 2.0 ** (-n / 2)
+                elif typ == E.ST_COST:
+                    v = v * np.exp(1j * gamma[layer] * (W / 2 - c))
+                elif typ == E.ST_RX:
+                    cs, sn = np.cos(beta[layer] / 2), np.sin(beta[layer] / 2)
+                    for b in sorted(bits):
+                        j = tset.index(b)
+                        e = np.arange(2 ** T)
+                        lo, hi = e[(e >> j) & 1 == 0], e[(e >> j) & 1 == 1]
+                        x, y = v[lo].copy(), v[hi].copy()
+                        v[lo], v[hi] = cs * x - 1j * sn * y, cs * y - 1j * sn * x
+                elif typ == E.ST_EXPECT:
+                    value += 2 * np.sum(np.abs(v) ** 2 * c)
+            kept[at] = v
+    return value
+
+
+@pytest.mark.parametrize("n,T,p", [(8, 6, 2), (9, 6, 2), (10, 7, 3)])
+def test_mirror_program_on_tiles_matches_oracle(n, T, p):
+    """The half-state (mirror) layout, run tile by tile on the host as the
+    kernel runs it, gives the oracle's <C>; its stage schedule (full-state
+    emulation) covers every RX bit once per layer."""
+    rng = np.random.default_rng(n * 7 + T)
+    g = nx.gnp_random_graph(n, 0.5, seed=int(rng.integers(1000)))
+    edges = list(g.edges())
+    beta = rng.uniform(0, np.pi / 2, size=p)
+    gamma = rng.uniform(0, np.pi, size=p)
+    ref = O.qaoa_expectation(n, edges, beta, gamma)
+    assert E.use_mirror(n, T)
+    assert abs(_emulate_mirror_tiles(n, edges, beta, gamma, T) - ref) < 1e-12
