@@ -78,12 +78,13 @@ def tile_sets(n, T, low=None):
     `low` bits plus successive chunks of T - low higher bits, each padded to
     T bits with the lowest bits it does not hold (bigger tiles, fewer work
     items, and more bits on which pending RX stages can run).  By default
-    `low` is 3 (128-byte contiguous runs), lowered to 2 when that is what
-    lets two sets cover all bits."""
+    `low` is 3 (128-byte contiguous runs), lowered to 2 or 1 when that is
+    what lets two sets cover all bits (two passes per layer instead of four
+    outweigh the shorter runs of the second set)."""
     if n <= T:
         return [tuple(range(n))]
     if low is None:
-        low = LOW_BITS if 2 * T - n >= LOW_BITS else (2 if 2 * T - n == 2 else LOW_BITS)
+        low = LOW_BITS if 2 * T - n >= LOW_BITS else (2 * T - n if 2 * T - n >= 1 else LOW_BITS)
     low = min(low, T - 1)               # every later set must take a new bit
     sets = [tuple(range(T))]
     lo = list(range(low))
