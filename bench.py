@@ -379,7 +379,7 @@ def subop_costs(n=N_QUBITS, reps=5):
         for k in (16, 48):
             plan = fusion.TilePlan(gates(kind, k), tuple(range(n)), tile_bits=12, low_bits=3,
                                    block=256, fuse_width=1, mode="reg")
-            subs = Counter(names[plan.srcs[o.src + 4 * j]] for o in plan.ops_host
+            subs = Counter(names[plan.srcs[o.src + 4 * j] & fusion.REC_TYPE] for o in plan.ops_host
                            if o.kind == fusion.OP_RPHASE for j in range(o.coef))
             plan.run_inplace(work)
             torch.cuda.synchronize()

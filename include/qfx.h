@@ -218,7 +218,8 @@ typedef struct {
 /* A register phase keeps, per lane, the 16 amplitudes spanned by its four
  * register bits in registers and applies its sub-ops there.  Sub-op records
  * are 4 ints {type, position code, coef index, mask}, starting at a multiple
- * of 4 ints in srcs:
+ * of 4 ints in srcs (srcs holds at least one record at `src`, also for a
+ * phase without sub-ops: the first record is read ahead):
  *   1  2x2 matrix (4 coefs) on register bit `code`;
  *   7  real 2x2 matrix (real parts of the 4 coefs are read);
  *   8  2x2 matrix with real diagonal and imaginary off-diagonal (real parts
@@ -227,13 +228,30 @@ typedef struct {
  *      bit);
  *   5  X on register bit code & 3 where the register bits of the mask
  *      (code >> 2) & 15 equal the values (code >> 6) & 15 (X, CNOT, CCNOT
- *      and their negated-control forms as register exchanges);
+ *      and their negated-control forms as register exchanges); `mask` may
+ *      add selector controls: m = mask & 3 selectors coded as type 9's,
+ *      selector j required to equal bit 23 + j of `mask`;
  *   9  2^m tables of 16 phases (table v at coef index 16 v), m = code & 3
- *      in 1..3, applied as type 4 with table v, where bit j of v is selector
+ *      in 0..3, applied as type 4 with table v, where bit j of v is selector
  *      p = (code >> (2 + 7 j)) & 127: the tile's state-index bit p (p < 64,
  *      a bit outside the tile) or the lane group's tile bit p - 64 (a tile
  *      bit outside the phase's registers).  Diagonal gates need none of
- *      their qubits in registers or in the tile.
+ *      their qubits in registers or in the tile.  `mask` = 16 | d: compact
+ *      tables of 2^popcount(d) phases, slot r reading the entry of its bits
+ *      under d; bits 8..11 of `mask`: bits f of the lane's flip mask (below)
+ *      the table reads, slot r taking the entry of slot r ^ (flips & f)
+ *      (m = 0 only with such bits).
+ * Lane flips: any record may carry a controlled X gate whose controls are
+ * all selectors, applied before it without moving amplitudes: a 20-bit
+ * word in bits 8.. of the type (bit 0 set; bits 1-2 the register bit; bits 3-9 a selector coded as type
+ * 9's and bit 10 its required value; bit 11 a second selector in bits
+ * 12-18, value in bit 19).  A lane whose selectors hold records the X in its
+ * flip mask f: register r then holds what register r ^ f holds without
+ * flips, and the phase stores it there.  Records after a flip that read a
+ * flipped bit carry a flag: types 1/7/8 code | 4 with 8 coefs (m, then
+ * X m X, read on lanes where the bit is flipped), type 5 code | 1 << 10
+ * (control values compared through f); diagonals are type 9 records that
+ * name the flip-mask bits they read.
  * A register phase's `phase` and `active` words are its load and store
  * maps: register slot r holds logical phase index c ^ (XOR of column j over
  * the set bits j of r), column j in bits 4j..4j+3 and c in bits 16..19.

@@ -187,6 +187,7 @@ __device__ __forceinline__ void op_sparse(v2d* lds, int T, const qfx_tile_op& op
 //           required to equal bit 23 + j of `mask`; where they do not, the
 //           sub-op leaves the lane's amplitudes as they are.  Controlled-X
 //           gates thus need only their target in registers.
+// Any record may also carry lane flips (see kD1Flip below).
 // Uncontrolled exchanges (SWAP, X, the permutation of ISWAP and Y) never
 // reach the kernel: the host relabels which logical slot each register
 // holds, and a leading run of SWAP/X/CNOT gates is folded into the load.
@@ -203,6 +204,25 @@ enum SubKind : int {
 constexpr int kRegBits = 4;
 constexpr int kRegs = 1 << kRegBits;
 
+// Lane flips.  A controlled X whose controls are all selectors (index bits
+// outside the tile, lane-group tile bits) moves no data: the lane records
+// it in a flip mask fm over its register bits, so that register r holds
+// what register r ^ fm holds on a lane without flips, and the phase's store
+// writes register r where register r ^ fm belongs (one XOR of its address
+// base).  A flip rides on a later record, as a 20-bit word in bits 8.. of
+// its type word, applied before the record.  Word: bit 0 set,
+// bits 1-2 register bit, bits 3-9 selector code (as SUB_DIAGV's) and bit 10
+// its required value, bit 11 a second selector in bits 12-18 with its value
+// in bit 19.  The host folds flips into the selector tables of later
+// diagonals; a 2x2 matrix on a flipped bit carries kD1Flip and 8
+// coefficients (m, then X m X) and reads the second half on flipped lanes;
+// a CX whose controls are flipped carries kCxFlip; a diagonal names in its
+// SUB_DIAGV mask bits 8.. the flip-mask bits it reads, and its slot r takes
+// the entry of slot r ^ (fm & those bits).
+constexpr int kD1Flip = 4;                 // 2x2 codes
+constexpr int kCxFlip = 1 << 10;           // SUB_CX code: controls compared through fm
+constexpr int kRecType = 255;
+
 // 1-qubit sub-ops walk the 8 register pairs (r, r | 1 << A); a scheduling
 // fence after the first four keeps half of the results in flight at a time
 // (all eight would push the kernel past 256 VGPRs, i.e. below two waves per
@@ -211,8 +231,8 @@ template <int A>
 constexpr int pair_lo(int p) { return ((p >> A) << (A + 1)) | (p & ((1 << A) - 1)); }
 
 template <int A>
-__device__ __forceinline__ void sub_d1(v2d (&v)[kRegs], const v2d* __restrict__ m) {
-  const v2d m0 = m[0], m1 = m[1], m2 = m[2], m3 = m[3];
+__device__ __forceinline__ void sub_d1(v2d (&v)[kRegs], const v2d m0, const v2d m1, const v2d m2,
+                                       const v2d m3) {
 #pragma unroll
   for (int p = 0; p < kRegs / 2; ++p) {
     const int r = pair_lo<A>(p);
@@ -229,8 +249,8 @@ __device__ __forceinline__ void sub_d1(v2d (&v)[kRegs], const v2d* __restrict__ 
 }
 
 template <int A>
-__device__ __forceinline__ void sub_d1r(v2d (&v)[kRegs], const v2d* __restrict__ m) {
-  const double m0 = m[0].x, m1 = m[1].x, m2 = m[2].x, m3 = m[3].x;
+__device__ __forceinline__ void sub_d1r(v2d (&v)[kRegs], const double m0, const double m1,
+                                        const double m2, const double m3) {
 #pragma unroll
   for (int p = 0; p < kRegs / 2; ++p) {
     const int r = pair_lo<A>(p);
@@ -241,10 +261,10 @@ __device__ __forceinline__ void sub_d1r(v2d (&v)[kRegs], const v2d* __restrict__
   }
 }
 
+// [[c0, i s0], [i s1, c1]]
 template <int A>
-__device__ __forceinline__ void sub_d1x(v2d (&v)[kRegs], const v2d* __restrict__ m) {
-  // [[c0, i s0], [i s1, c1]]
-  const double c0 = m[0].x, s0 = m[1].y, s1 = m[2].y, c1 = m[3].x;
+__device__ __forceinline__ void sub_d1x(v2d (&v)[kRegs], const double c0, const double s0,
+                                        const double s1, const double c1) {
 #pragma unroll
   for (int p = 0; p < kRegs / 2; ++p) {
     const int r = pair_lo<A>(p);
@@ -292,6 +312,73 @@ __device__ __forceinline__ void sub_diag_compact(v2d (&v)[kRegs], const v2d* __r
   }
 }
 
+// Diagonals on a lane with flips f: slot r takes the phase of slot r ^ f.
+__device__ __forceinline__ void sub_diag_flip(v2d (&v)[kRegs], const v2d* __restrict__ ph,
+                                              unsigned f) {
+#pragma unroll
+  for (int r0 = 0; r0 < kRegs; r0 += 4) {
+#pragma unroll
+    for (int r = r0; r < r0 + 4; ++r) v[r] = cmul(ph[(unsigned)r ^ f], v[r]);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// The compact index is XOR-linear in the slot (distinct bits per register
+// bit), so slot r ^ f has index idx(r) ^ idx(f).
+__device__ __forceinline__ void sub_diag_compact_flip(v2d (&v)[kRegs], const v2d* __restrict__ ph,
+                                                      unsigned dep, unsigned f) {
+  unsigned stride[kRegBits];
+  unsigned k = 0, fi = 0;
+#pragma unroll
+  for (int j = 0; j < kRegBits; ++j) {
+    stride[j] = ((dep >> j) & 1u) ? (1u << k) : 0u;
+    k += (dep >> j) & 1u;
+    fi |= ((f >> j) & 1u) ? stride[j] : 0u;
+  }
+#pragma unroll
+  for (int r0 = 0; r0 < kRegs; r0 += 4) {
+#pragma unroll
+    for (int r = r0; r < r0 + 4; ++r) {
+      unsigned idx = 0;
+#pragma unroll
+      for (int j = 0; j < kRegBits; ++j)
+        if (r & (1 << j)) idx += stride[j];
+      v[r] = cmul(ph[idx ^ fi], v[r]);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// Marks a branch that reads flips: the compiler may not turn its few
+// instructions into selects that every record without the flag would run.
+__device__ __forceinline__ void flagged_branch() { __asm__ volatile(""); }
+
+// Whether the selector controls of word w (as in SUB_CX) hold.
+__device__ __forceinline__ bool selectors_hold(unsigned w, uint64_t gbase, unsigned egrp) {
+  bool ok = true;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    if (j < (int)(w & 3u)) {
+      const unsigned p = (w >> (2 + 7 * j)) & 127u;
+      const unsigned b = p < 64u ? (unsigned)((gbase >> p) & 1u) : (egrp >> (p - 64u)) & 1u;
+      ok = ok && b == ((w >> (23 + j)) & 1u);
+    }
+  }
+  return ok;
+}
+
+// Register-bit mask of a lane flip word (0 where its selectors do not hold).
+__device__ __forceinline__ unsigned flip_of(unsigned f, uint64_t gbase, unsigned egrp) {
+  const unsigned p0 = (f >> 3) & 127u, p1 = (f >> 12) & 127u;
+  const unsigned b0 = p0 < 64u ? (unsigned)((gbase >> p0) & 1u) : (egrp >> (p0 - 64u)) & 1u;
+  bool ok = b0 == ((f >> 10) & 1u);
+  if (f & 2048u) {
+    const unsigned b1 = p1 < 64u ? (unsigned)((gbase >> p1) & 1u) : (egrp >> (p1 - 64u)) & 1u;
+    ok = ok && b1 == ((f >> 19) & 1u);
+  }
+  return ok ? 1u << ((f >> 1) & 3u) : 0u;
+}
+
 // Controlled exchange with runtime control bits: the pair structure (target
 // bit) is compile time, the control test a per-slot select, so no register
 // is ever indexed at run time.  Slot r's pair swaps when its control bits
@@ -313,37 +400,44 @@ __device__ __forceinline__ void sub_cx(v2d (&v)[kRegs], unsigned cm, unsigned cv
 // let the dispatch drop its do-nothing path, which would otherwise copy all
 // 16 register amplitudes on every sub-op.
 __device__ __forceinline__ void apply_sub(v2d (&v)[kRegs], const int4 sub, const v2d* coef,
-                                          uint64_t gbase, unsigned egrp) {
+                                          uint64_t gbase, unsigned egrp, unsigned& fm) {
   const int type = sub.x, code = sub.y;
   const v2d* m = coef + sub.z;
   switch (type) {
-    case SUB_D1:
-      switch (code) {
-        case 0: sub_d1<0>(v, m); break;
-        case 1: sub_d1<1>(v, m); break;
-        case 2: sub_d1<2>(v, m); break;
-        case 3: sub_d1<3>(v, m); break;
+    case SUB_D1: {
+      // flagged: 8 coefficients, X m X in the second four for flipped lanes
+      const v2d* mm = m + ((code & kD1Flip) ? (((fm >> (code & 3)) & 1u) << 2) : 0u);
+      switch (code & 3) {
+        case 0: sub_d1<0>(v, mm[0], mm[1], mm[2], mm[3]); break;
+        case 1: sub_d1<1>(v, mm[0], mm[1], mm[2], mm[3]); break;
+        case 2: sub_d1<2>(v, mm[0], mm[1], mm[2], mm[3]); break;
+        case 3: sub_d1<3>(v, mm[0], mm[1], mm[2], mm[3]); break;
         default: __builtin_unreachable();
       }
       break;
-    case SUB_D1R:
-      switch (code) {
-        case 0: sub_d1r<0>(v, m); break;
-        case 1: sub_d1r<1>(v, m); break;
-        case 2: sub_d1r<2>(v, m); break;
-        case 3: sub_d1r<3>(v, m); break;
+    }
+    case SUB_D1R: {
+      const v2d* mm = m + ((code & kD1Flip) ? (((fm >> (code & 3)) & 1u) << 2) : 0u);
+      switch (code & 3) {
+        case 0: sub_d1r<0>(v, mm[0].x, mm[1].x, mm[2].x, mm[3].x); break;
+        case 1: sub_d1r<1>(v, mm[0].x, mm[1].x, mm[2].x, mm[3].x); break;
+        case 2: sub_d1r<2>(v, mm[0].x, mm[1].x, mm[2].x, mm[3].x); break;
+        case 3: sub_d1r<3>(v, mm[0].x, mm[1].x, mm[2].x, mm[3].x); break;
         default: __builtin_unreachable();
       }
       break;
-    case SUB_D1X:
-      switch (code) {
-        case 0: sub_d1x<0>(v, m); break;
-        case 1: sub_d1x<1>(v, m); break;
-        case 2: sub_d1x<2>(v, m); break;
-        case 3: sub_d1x<3>(v, m); break;
+    }
+    case SUB_D1X: {
+      const v2d* mm = m + ((code & kD1Flip) ? (((fm >> (code & 3)) & 1u) << 2) : 0u);
+      switch (code & 3) {
+        case 0: sub_d1x<0>(v, mm[0].x, mm[1].y, mm[2].y, mm[3].x); break;
+        case 1: sub_d1x<1>(v, mm[0].x, mm[1].y, mm[2].y, mm[3].x); break;
+        case 2: sub_d1x<2>(v, mm[0].x, mm[1].y, mm[2].y, mm[3].x); break;
+        case 3: sub_d1x<3>(v, mm[0].x, mm[1].y, mm[2].y, mm[3].x); break;
         default: __builtin_unreachable();
       }
       break;
+    }
     case SUB_DIAG: sub_diag(v, m); break;
     case SUB_DIAGV: {
       // gbase: the tile's state index with its tile bits zero (uniform);
@@ -359,29 +453,30 @@ __device__ __forceinline__ void apply_sub(v2d (&v)[kRegs], const int4 sub, const
         }
       }
       const unsigned w = (unsigned)sub.w;
-      if (w) {
+      // flip-mask bits the tables read (w >> 8): slot r takes the entry of
+      // slot r ^ (fm & those bits)
+      const unsigned fx = (w >> 8) & 15u;
+      if (w & 16u) {
         const unsigned dep = w & 15u;
-        sub_diag_compact(v, m + (sel << __builtin_popcount(dep)), dep);
+        if (fx) sub_diag_compact_flip(v, m + (sel << __builtin_popcount(dep)), dep, fm & fx);
+        else sub_diag_compact(v, m + (sel << __builtin_popcount(dep)), dep);
       } else {
-        sub_diag(v, m + 16 * sel);
+        if (fx) sub_diag_flip(v, m + 16 * sel, fm & fx);
+        else sub_diag(v, m + 16 * sel);
       }
       break;
     }
     case SUB_CX: {
       unsigned cm = ((unsigned)code >> 2) & 15u, cv = ((unsigned)code >> 6) & 15u;
+      if (code & kCxFlip) {                 // control values as this lane holds them
+        flagged_branch();
+        cv ^= fm & cm;
+      }
       const unsigned w = (unsigned)sub.w;
       if (w) {
         // selector controls: unmet, the test asks the target bit to be both
         // 0 (pair's low slot) and 1, so no pair swaps
-        bool ok = true;
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-          if (j < (int)(w & 3u)) {
-            const unsigned p = (w >> (2 + 7 * j)) & 127u;
-            const unsigned b = p < 64u ? (unsigned)((gbase >> p) & 1u) : (egrp >> (p - 64u)) & 1u;
-            ok = ok && b == ((w >> (23 + j)) & 1u);
-          }
-        }
+        const bool ok = selectors_hold(w, gbase, egrp);
         const unsigned tb = 1u << (code & 3);
         cm = ok ? cm : (cm | tb);
         cv = ok ? cv : (cv | tb);
@@ -458,28 +553,48 @@ __device__ __forceinline__ unsigned phase_lb(const qfx_tile_op& op, unsigned g) 
   return g;
 }
 
-__device__ __forceinline__ void run_subs(v2d (&v)[kRegs], const qfx_tile_op& op,
-                                         const v2d* coef, const int* __restrict__ subs,
-                                         uint64_t gbase, unsigned egrp) {
+// Returns the lane's flip mask, which the phase's store applies.
+__device__ __forceinline__ unsigned run_subs(v2d (&v)[kRegs], const qfx_tile_op& op,
+                                             const v2d* coef, const int* __restrict__ subs,
+                                             uint64_t gbase, unsigned egrp) {
   // the next record's scalar load is issued with the current one's (scalar
   // loads complete out of order, so one wait covers both)
   const int4* rec = reinterpret_cast<const int4*>(subs + op.src);
   const int nsub = op.coef;
+  unsigned fm = 0;
   int4 next = rec[0];
   // Two records per trip: every case of apply_sub writes its results to
   // fresh registers, and with one record per trip the loop latch copied
   // all 16 amplitudes back (32 64-bit moves per sub-op).  With two, the
   // second copy's results land in the loop's own registers.
+  // A record's lane flips are applied outside the dispatch: a case that
+  // left the amplitudes alone would bring the copies back.
+  auto flips = [&](int4& r) {
+    const unsigned f = (unsigned)r.x >> 8;
+    if (f) fm ^= flip_of(f, gbase, egrp);
+    r.x &= kRecType;
+  };
   for (int i = 0; i < nsub; i += 2) {
-    const int4 cur = next;
+    int4 cur = next;
     next = rec[i + 1 < nsub ? i + 1 : i];
-    apply_sub(v, cur, coef, gbase, egrp);
+    flips(cur);
+    apply_sub(v, cur, coef, gbase, egrp, fm);
     if (i + 1 < nsub) {
-      const int4 cur2 = next;
+      int4 cur2 = next;
       next = rec[i + 2 < nsub ? i + 2 : i + 1];
-      apply_sub(v, cur2, coef, gbase, egrp);
+      flips(cur2);
+      apply_sub(v, cur2, coef, gbase, egrp, fm);
     }
   }
+  return fm;
+}
+
+// XOR of the basis values of the set bits of a flip mask: the address
+// offset between register r and register r ^ fm.
+template <typename U>
+__device__ __forceinline__ U flip_off(unsigned fm, const U (&b)[4]) {
+  return ((fm & 1u) ? b[0] : U(0)) ^ ((fm & 2u) ? b[1] : U(0)) ^ ((fm & 4u) ? b[2] : U(0)) ^
+         ((fm & 8u) ? b[3] : U(0));
 }
 
 // The tile kernels declare no static LDS, so their dynamic LDS array starts
@@ -501,9 +616,9 @@ __device__ __forceinline__ void phase_lds_load(v2d (&v)[kRegs], const v2d* /*lds
 }
 
 __device__ __forceinline__ void phase_lds_store(const v2d (&v)[kRegs], v2d* lds, unsigned slb,
-                                                const PhaseLds& p) {
+                                                const PhaseLds& p, unsigned fm) {
   unsigned a[kRegs];
-  xor_tree(a, slb ^ p.sc, p.s);
+  xor_tree(a, slb ^ p.sc ^ flip_off(fm, p.s), p.s);
 #pragma unroll
   for (int r = 0; r < kRegs; ++r) *lds_byte(a[r]) = v[r];
 }
@@ -519,8 +634,8 @@ __device__ __forceinline__ void op_rphase(v2d* lds, int T, const qfx_tile_op& op
     const unsigned slb = sw(egrp) << 4;
     v2d v[kRegs];
     phase_lds_load(v, lds, slb, p);
-    run_subs(v, op, coef, subs, gbase, egrp);
-    phase_lds_store(v, lds, slb, p);
+    const unsigned fm = run_subs(v, op, coef, subs, gbase, egrp);
+    phase_lds_store(v, lds, slb, p, fm);
   }
 }
 
@@ -874,9 +989,10 @@ __device__ __forceinline__ PhaseGlobal phase_global(const qfx_tile_op& op, unsig
   return g;
 }
 
-__device__ __forceinline__ void phase_gstore(const v2d (&v)[kRegs], v2d* p, const PhaseGlobal& g) {
+__device__ __forceinline__ void phase_gstore(const v2d (&v)[kRegs], v2d* p, const PhaseGlobal& g,
+                                             unsigned fm) {
   uint64_t a[kRegs];
-  xor_tree(a, g.c, g.b);
+  xor_tree(a, g.c ^ flip_off(fm, g.b), g.b);
 #pragma unroll
   for (int r = 0; r < kRegs; ++r) st_stream(p + a[r], v[r]);
 }
@@ -952,8 +1068,8 @@ __global__ __launch_bounds__(BS) void tile_reg_kernel(v2d* state, int T, TilePos
       v2d v[kRegs];
       const unsigned egrp = phase_lb(last, (unsigned)tid);
       phase_lds_load(v, lds, sw(egrp) << 4, lp);
-      run_subs(v, last, cf, srcs, base, egrp);
-      phase_gstore(v, state + base + l_lane, lg);
+      const unsigned fm = run_subs(v, last, cf, srcs, base, egrp);
+      phase_gstore(v, state + base + l_lane, lg, fm);
     }
     base = nbase;
     __syncthreads();
@@ -1030,12 +1146,12 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void ti
 #pragma unroll
       for (int r = 0; r < kRegs; ++r) v[r] = pf[r];
       phase_gload(pf, state + nbase + f_lane, fg);
-      run_subs(v, first, cf, srcs, base, f_egrp);
+      const unsigned fm = run_subs(v, first, cf, srcs, base, f_egrp);
       // No LDS is involved, but with a permuted store a wave writes
       // amplitudes other waves of the workgroup load: every wave must hold
       // this tile in registers (v) before any stores it.
       if (pos.permuted) __syncthreads();
-      phase_gstore(v, state + base + s_lane, sg);
+      phase_gstore(v, state + base + s_lane, sg, fm);
       base = nbase;
     }
     return;
@@ -1055,8 +1171,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void ti
 #pragma unroll
     for (int r = 0; r < kRegs; ++r) v[r] = pf[r];
     phase_gload(pf, state + nbase + f_lane, fg);
-    run_subs(v, first, cf, srcs, base, f_egrp);
-    phase_lds_store(v, lds, sw(f_egrp) << 4, phase_lds(first));
+    const unsigned ffm = run_subs(v, first, cf, srcs, base, f_egrp);
+    phase_lds_store(v, lds, sw(f_egrp) << 4, phase_lds(first), ffm);
     __syncthreads();
     for (int i = 1; i < nmid_end; ++i) {
       apply_op<BS, false, false>(lds, T, ops[i], cf, coef, srcs, base);
@@ -1067,9 +1183,9 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(2))) void ti
       const qfx_tile_op& last = ops[nops - 1];
       const unsigned egrp = phase_lb(last, (unsigned)tid);
       phase_lds_load(v, lds, sw(egrp) << 4, phase_lds(last));
-      run_subs(v, last, cf, srcs, base, egrp);
+      const unsigned lfm = run_subs(v, last, cf, srcs, base, egrp);
       phase_gstore(v, state + base + tile_state_off(egrp, T, pos.d),
-                   phase_global(last, last.active, T, pos.d));
+                   phase_global(last, last.active, T, pos.d), lfm);
     } else {
       v2d* q0 = state + base + soff;
       v2d* q1 = q0 + sb1;
@@ -1316,19 +1432,52 @@ int qfx_tile_check_ops(int T, int block, int flags, const qfx_tile_op* ops, int 
                                   "qfx_tile_check_ops: register-phase LDS words do not match "
                                   "its bits and maps (run qfx_tile_prepare_ops)");
       }
-      if (o.coef < 0 || o.src < 0 || o.src % 4 != 0 || o.src + 4 * o.coef > nsrcs)
+      // the sub-op loop reads the record at src even when the list is empty
+      if (o.coef < 0 || o.src < 0 || o.src % 4 != 0 ||
+          o.src + 4 * (o.coef > 0 ? o.coef : 1) > nsrcs)
         return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: sub-ops out of range");
       for (int q = 0; q < o.coef; ++q) {
         const int* sub = srcs + o.src + 4 * q;
-        // codes per type: D1/D1R/D1X 4, DIAG 1, CX 4 targets x 16 control
-        // masks x 16 control values; control masks may not touch the target
+        // codes per type: D1/D1R/D1X 4 register bits (+ kD1Flip: 8
+        // coefficients), DIAG 0, CX 4 targets x 16 control masks x 16
+        // control values (+ kCxFlip); control masks may not touch the target
         // DIAGV: 1..3 selector bits, each an index bit < 64, 16 << m coefs
-        static const int ncode[9] = {0, 4, 0, 0, 1, 1024, 0, 4, 4};
+        static const int ncode[9] = {0, 8, 0, 0, 1, 2048, 0, 8, 8};
         static const int need[9] = {0, 4, 0, 0, kRegs, 0, 0, 4, 4};
-        const int ty = sub[0], cd = sub[1];
+        // a selector code: an index bit < 64 outside the tile (the caller's
+        // tile positions are not known here), or 64 + a tile bit outside the
+        // registers
+        auto selector_ok = [&](unsigned p) {
+          if (p < 64u) return true;
+          const int t = (int)(p - 64u);
+          bool good = t < T;
+          for (int a = 0; good && a < kRegBits; ++a) good = o.t[a] != t;
+          return good;
+        };
+        // a lane flip riding on the record (type word bits 8..): a 20-bit
+        // word, bit 0 set, nothing above
+        auto flip_ok = [&](unsigned f) {
+          if (f == 0) return true;
+          if (!(f & 1u) || (f >> 20) != 0 || !selector_ok((f >> 3) & 127u)) return false;
+          if (f & 2048u) return selector_ok((f >> 12) & 127u);
+          return ((f >> 12) & 255u) == 0;
+        };
+        if (sub[0] < 0 || sub[2] < 0 || !flip_ok((unsigned)sub[0] >> 8))
+          return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: bad lane flip");
+        const int ty = sub[0] & kRecType, cd = sub[1], cz = sub[2];
+        // a selector-control word (SUB_CX): 1..3 selectors as for SUB_DIAGV,
+        // their values in bits 23.., nothing above
+        auto controls_ok = [&](unsigned w) {
+          const unsigned m = w & 3u;
+          bool good = m != 0 && (w >> (23 + m)) == 0 &&
+                      ((w >> (2 + 7 * m)) & ((1u << (21 - 7 * m)) - 1u)) == 0;
+          for (unsigned j = 0; good && j < m; ++j) good = selector_ok((w >> (2 + 7 * j)) & 127u);
+          return good;
+        };
         if (ty == SUB_DIAGV) {
-          const unsigned c = (unsigned)cd, m = c & 3u;
-          bool good = m != 0 && (c >> (2 + 7 * m)) == 0;
+          const unsigned c = (unsigned)cd, m = c & 3u, w = (unsigned)sub[3];
+          // m selectors (none only with flip-mask bits, w >> 8)
+          bool good = (m != 0 || ((w >> 8) & 15u)) && (c >> (2 + 7 * m)) == 0;
           for (unsigned j = 0; good && j < m; ++j) {
             const unsigned p = (c >> (2 + 7 * j)) & 127u;
             // a lane-group selector must be a tile bit outside the registers
@@ -1339,11 +1488,10 @@ int qfx_tile_check_ops(int T, int block, int flags, const qfx_tile_op* ops, int 
             }
           }
           // tables: 16 phases each, or compact (mask 16 | d): 2^popcount(d)
-          const unsigned w = (unsigned)sub[3];
-          if (w != 0 && (w >> 4) != 1u) good = false;
-          const int per = w ? (1 << __builtin_popcount(w & 15u)) : kRegs;
+          if ((w & ~0xF1Fu) != 0 || (!(w & 16u) && (w & 15u))) good = false;
+          const int per = (w & 16u) ? (1 << __builtin_popcount(w & 15u)) : kRegs;
           if (!good) return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: bad selector diagonal");
-          if (sub[2] < 0 || sub[2] + (per << m) > ncoef)
+          if (cz + (per << m) > ncoef)
             return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: sub-op coefficients out of range");
           continue;
         }
@@ -1353,22 +1501,13 @@ int qfx_tile_check_ops(int T, int block, int flags, const qfx_tile_op* ops, int 
           const int cm = (cd >> 2) & 15, cv = (cd >> 6) & 15;
           if (((cm >> (cd & 3)) & 1) || (cv & ~cm))
             return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: bad CX control bits");
-          // selector controls: 0..3 selectors as for SUB_DIAGV, their values
-          // in bits 23.., nothing above
-          const unsigned w = (unsigned)sub[3], m = w & 3u;
-          bool good = w == 0 || (m != 0 && (w >> (23 + m)) == 0 &&
-                                 ((w >> (2 + 7 * m)) & ((1u << (21 - 7 * m)) - 1u)) == 0);
-          for (unsigned j = 0; good && w && j < m; ++j) {
-            const unsigned p = (w >> (2 + 7 * j)) & 127u;
-            if (p >= 64u) {
-              const int t = (int)(p - 64u);
-              good = t < T;
-              for (int a = 0; good && a < kRegBits; ++a) good = o.t[a] != t;
-            }
-          }
-          if (!good) return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: bad CX selector controls");
+          // selector controls: none, or a selector-control word
+          const unsigned w = (unsigned)sub[3];
+          if (w != 0 && !controls_ok(w))
+            return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: bad CX selector controls");
         }
-        if (need[ty] && (sub[2] < 0 || sub[2] + need[ty] > ncoef))
+        const int nc = need[ty] << ((ty != SUB_CX && (cd & kD1Flip)) ? 1 : 0);
+        if (nc && cz + nc > ncoef)
           return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: sub-op coefficients out of range");
       }
       continue;

@@ -40,6 +40,13 @@ from .states import State
 
 OP_DENSE, OP_SPARSE, OP_RPHASE, OP_MFMA = 0, 1, 2, 3
 SUB_D1, SUB_DIAG, SUB_CX, SUB_D1R, SUB_D1X, SUB_DIAGV = 1, 4, 5, 7, 8, 9
+# record flags of sub-ops that read a bit a lane flip may have hit (2x2
+# matrices: 8 coefficients, m then X m X; CX: controls through the flips)
+D1_FLIP, CX_FLIP = 4, 1 << 10
+# a record's type sits in the low bits of its word, a lane flip riding on
+# the record in the bits above
+REC_TYPE = 255
+MAX_FLIP_SEL = 2            # selectors of one lane flip
 MAX_SEL_BITS = 3          # index bits outside the tile a register diagonal may read
 REG_BITS = 4
 MAX_TILE_BITS = 13
@@ -581,7 +588,7 @@ def d1_variant(m):
     return SUB_D1
 
 
-def _encode_phase(R, members, pos_index, T, coefs, subs):
+def _encode_phase(R, members, pos_index, T, coefs, subs, flips=True):
     """qfx_tile_op of a register phase over tile bits R plus its sub-ops.
 
     Logical bit A of the phase is tile bit regs[A]; register i starts out
@@ -593,7 +600,14 @@ def _encode_phase(R, members, pos_index, T, coefs, subs):
     the slot of logical phi(i), and a leading run of SWAP/X/CNOT gates and
     diagonals is folded into the load map.  Both maps are affine on the
     4-bit logical index and are stored as matrix | offset << 16 (column j
-    = image of bit j): ``active`` the store map, ``phase`` the load map."""
+    = image of bit j): ``active`` the store map, ``phase`` the load map.
+
+    With ``flips``, a controlled X whose controls are all selectors (no
+    register bit) becomes a lane flip riding on a later record: it moves no
+    amplitudes.  It may ride past records that do not read its bit (it
+    commutes with them), one flip per record.  Later diagonals name the
+    flip-mask bits they read; later 2x2 matrices on its bit and CX gates
+    controlled by it carry a flag."""
     tl = sorted(pos_index[b] for b in R)
     # diagonals whose bits all lie in R may fold into the load map; the others
     # (bits outside R: lane-group or outside-tile selectors) may not
@@ -777,43 +791,119 @@ def _encode_phase(R, members, pos_index, T, coefs, subs):
             continue
         out.append([SUB_D1, j, m])
     flush()
+
+    def deps(tab):                          # register bits a table depends on
+        return sum(1 << j for j in range(REG_BITS)
+                   if not np.array_equal(tab, tab[:, regs16 ^ (1 << j)]))
+
+    def flip_of(code, data):
+        """(register bit, [(selector code, value)]) of a CX entry that can
+        be a lane flip, else None."""
+        if not flips or not data or (code >> 2) & 15:
+            return None
+        m = data & 3
+        if m > MAX_FLIP_SEL:
+            return None
+        return (code & 3, [((data >> (2 + 7 * j)) & 127, (data >> (23 + j)) & 1)
+                           for j in range(m)])
+
+    def emits(k):                           # does entry k give a record
+        typ, code, data = out[k]
+        if typ in (SUB_DIAG, SUB_DIAGV):
+            return not np.all(data == 1.0)
+        return typ != SUB_CX or flip_of(code, data) is None
+
+    def reads(k, t):                        # does entry k's record read register bit t
+        typ, code, data = out[k]
+        if typ in (SUB_DIAG, SUB_DIAGV):
+            return (deps(data) >> t) & 1
+        if typ == SUB_CX:                   # its target commutes with a flip
+            return (code >> (2 + t)) & 1
+        return code == t
+
+    # Which exchanges become lane flips, and the record each rides on: the
+    # first free record after it, passing only records that do not read its
+    # bit.  The others stay exchange records.
+    carrier, virtual = {}, set()            # carrier: record -> flip
+    for k, (typ, code, data) in enumerate(out):
+        f = flip_of(code, data) if typ == SUB_CX else None
+        if f is None:
+            continue
+        j = None
+        for i in range(k + 1, len(out)):
+            if not emits(i):
+                continue
+            if i not in carrier:
+                j = i
+                break
+            if reads(i, f[0]):
+                break
+        if j is not None:
+            carrier[j] = f
+            virtual.add(k)
+
+    def flip_word(f):
+        t, sels = f
+        w = 1 | (t << 1) | (sels[0][0] << 3) | (sels[0][1] << 10)
+        if len(sels) > 1:
+            w |= (1 << 11) | (sels[1][0] << 12) | (sels[1][1] << 19)
+        return w
+
     nsub = 0
-    for typ, code, data in out:
+    active = []                             # flips applied so far
+    for k, (typ, code, data) in enumerate(out):
+        if k in virtual:
+            continue
+        if k in carrier:                    # applied before this record
+            active.append(carrier[k])
+        flipped = 0
+        for t, _ in active:
+            flipped |= 1 << t
         if typ == SUB_D1:
             typ = d1_variant(data)
         if typ in (SUB_DIAG, SUB_DIAGV):
+            # the flip-mask bits a table reads: slot r takes the entry of
+            # slot r ^ (flips & fmask) on each lane
             sel, tab = code, data
             if np.all(tab == 1.0):
                 continue
-            if not sel:
+            fmask = flipped & deps(tab)
+            dep = [j for j in range(REG_BITS) if (deps(tab) >> j) & 1]
+            if not sel and not fmask:
                 mask = 0
                 for r in range(2 ** REG_BITS):
                     if tab[0, r] != 1.0:
                         mask |= 1 << r
-                subs.extend([SUB_DIAG, 0, len(coefs), mask])
+                rec = [SUB_DIAG, 0, len(coefs), mask]
                 coefs.extend(tab[0])
             else:
                 cd = len(sel)
                 for j, b in enumerate(sel):
                     cd |= b << (2 + 7 * j)
-                # register bits the tables depend on; fewer than 4: store
-                # compact tables, entry = the slot's bits under dep, in order
-                dep = [j for j in range(REG_BITS)
-                       if not np.array_equal(tab, tab[:, regs16 ^ (1 << j)])]
+                # fewer than 4 bits: compact tables, entry = the slot's bits
+                # under dep, in order
                 if len(dep) < REG_BITS:
                     slots = [sum(((i >> a) & 1) << j for a, j in enumerate(dep))
                              for i in range(2 ** len(dep))]
-                    subs.extend([SUB_DIAGV, cd, len(coefs),
-                                 16 | sum(1 << j for j in dep)])
+                    rec = [SUB_DIAGV, cd, len(coefs),
+                           16 | sum(1 << j for j in dep) | fmask << 8]
                     coefs.extend(tab[:, slots].reshape(-1))
                 else:
-                    subs.extend([SUB_DIAGV, cd, len(coefs), 0])
+                    rec = [SUB_DIAGV, cd, len(coefs), fmask << 8]
                     coefs.extend(tab.reshape(-1))
         elif typ == SUB_CX:
-            subs.extend([typ, code, 0, data or 0])
+            cm = (code >> 2) & 15
+            rec = [typ, code | (CX_FLIP if flipped & cm else 0), 0, data or 0]
         else:
-            subs.extend([typ, code, len(coefs), 0])
-            coefs.extend(np.asarray(data, dtype=np.complex128).reshape(-1))
+            m = np.asarray(data, dtype=np.complex128)
+            rec = [typ, code, len(coefs), 0]
+            coefs.extend(m.reshape(-1))
+            if (flipped >> code) & 1:       # X m X for lanes holding the pair reversed
+                rec[1] |= D1_FLIP
+                coefs.extend(m[::-1, ::-1].reshape(-1))
+        if k in carrier:
+            rec[0] |= flip_word(carrier[k]) << 8
+        subs.extend(rec)
         nsub += 1
     o.coef = nsub
     o.active = _affine_code([phi(i) for i in range(2 ** REG_BITS)])   # store map
@@ -952,7 +1042,7 @@ def _op_coef_start(o, srcs):
     """First coefficient index an encoded op reads (None if it reads none)."""
     if o.kind == OP_RPHASE:
         idx = [srcs[o.src + 4 * q + 2] for q in range(o.coef)
-               if srcs[o.src + 4 * q] in _COEF_SUBS]
+               if srcs[o.src + 4 * q] & REC_TYPE in _COEF_SUBS]
         return min(idx) if idx else None
     return o.coef
 
@@ -960,7 +1050,7 @@ def _op_coef_start(o, srcs):
 def _rebase_coef(o, srcs, off):
     if o.kind == OP_RPHASE:
         for q in range(o.coef):
-            if srcs[o.src + 4 * q] in _COEF_SUBS:
+            if srcs[o.src + 4 * q] & REC_TYPE in _COEF_SUBS:
                 srcs[o.src + 4 * q + 2] -= off
     else:
         o.coef -= off
@@ -1161,7 +1251,7 @@ class HostPlan:
 
     def __init__(self, gates, qubits, tile_bits=MAX_TILE_BITS, low_bits=3, block=256,
                  fuse_width=4, mode='reg', persistent=True, reg_io=True, reg_in=True,
-                 free_controls=True, virtual_swaps=True):
+                 free_controls=True, virtual_swaps=True, lane_flips=True):
         qubits = tuple(qubits)
         n = len(qubits)
         self.n = n
@@ -1182,6 +1272,7 @@ class HostPlan:
         self.reg_io = reg_io
         self.reg_in = reg_in
         self.free_controls = free_controls
+        self.lane_flips = lane_flips
         self.gates = list(gates)
         bits0 = [tuple(n - 1 - qubits.index(q) for q in g.qubits) for g in gates]
         kinds0 = [g.kind for g in gates]
@@ -1280,7 +1371,7 @@ class HostPlan:
                 for item in plan_phases(ops, free_diagonals=free is not None):
                     if item[0] == 'phase':
                         for o in self._encode_fitting(item[1], item[2], pos_index, T,
-                                                      coefs, srcs):
+                                                      coefs, srcs, lane_flips):
                             ops_host.append(o)
                             self.n_phases += 1
                     else:
@@ -1330,6 +1421,8 @@ class HostPlan:
             self.passes[i] = ('tile', (orders[k], start, nops, block, flags, c_lo, ncoef, dest))
         self.n_passes = len(self.passes)
         self.n_ops = len(ops_host)
+        srcs.extend([0, 0, 0, 0])       # the sub-op loop reads one record ahead,
+        # also at the src of a phase without sub-ops
         self.ops_host, self.coefs, self.srcs = ops_host, coefs, srcs
         prepare_ops(ops_host)
         # validate every pass with its own block and flags
@@ -1344,19 +1437,19 @@ class HostPlan:
                                                  src_arr, len(srcs)), 'qfx_tile_check_ops')
 
     @staticmethod
-    def _encode_fitting(R, members, pos_index, T, coefs, srcs):
+    def _encode_fitting(R, members, pos_index, T, coefs, srcs, flips=True):
         """Register phase(s) for ``members`` on bits R, each staging at most
         MAX_STAGED coefficients: a phase over the limit is cut into two
         consecutive phases of its first and second half (each leaves the
         tile in logical order, so they compose)."""
         c0, s0 = len(coefs), len(srcs)
-        o = _encode_phase(R, members, pos_index, T, coefs, srcs)
+        o = _encode_phase(R, members, pos_index, T, coefs, srcs, flips)
         if len(coefs) - c0 <= MAX_STAGED or len(members) == 1:
             return [o]
         del coefs[c0:], srcs[s0:]
         h = len(members) // 2
-        return (HostPlan._encode_fitting(R, members[:h], pos_index, T, coefs, srcs)
-                + HostPlan._encode_fitting(R, members[h:], pos_index, T, coefs, srcs))
+        return (HostPlan._encode_fitting(R, members[:h], pos_index, T, coefs, srcs, flips)
+                + HostPlan._encode_fitting(R, members[h:], pos_index, T, coefs, srcs, flips))
 
     def _add_tile_passes(self, k, ops_host, start, coefs, c_lo, srcs, block, flags):
         """Record tile pass k (ops_host[start:]) with coefficient indices
@@ -1413,7 +1506,7 @@ class HostPlan:
             per_amp = 0
             for o in self.ops_host[start:start + nops]:
                 if o.kind == OP_RPHASE:
-                    per_amp += sum(self._SUB_FLOPS[self.srcs[o.src + 4 * q]]
+                    per_amp += sum(self._SUB_FLOPS[self.srcs[o.src + 4 * q] & REC_TYPE]
                                    for q in range(o.coef))
                 elif o.kind == OP_DENSE:
                     per_amp += 8 * 2 ** o.k - 2

@@ -93,6 +93,27 @@ def test_encoded_ops_validate():
                                   len(srcs)) == _native.QFX_EINVAL
 
 
+def test_empty_phase_needs_a_record():
+    """A register phase without sub-ops still has its first record read
+    ahead, so the validator wants one record at its src; plans append one."""
+    T = 8
+    pidx = {b: b for b in range(T)}
+    coefs, srcs = [], []
+    o = fusion._encode_phase([0, 1, 2, 3], [], pidx, T, coefs, srcs)
+    assert o.coef == 0 and o.src == len(srcs)
+    fusion.prepare_ops([o])
+    arr = (_native.QfxTileOp * 1)(o)
+    lib = _native.load()
+    assert lib.qfx_tile_check_ops(T, 256, 0, arr, 1, 0, _native.int_array(srcs + [0]),
+                                  len(srcs)) == _native.QFX_EINVAL
+    padded = srcs + [0, 0, 0, 0]
+    assert lib.qfx_tile_check_ops(T, 256, 0, arr, 1, 0, _native.int_array(padded),
+                                  len(padded)) == 0
+    gates = build_gates(random_mixed_circuit(10, 4, seed=3))
+    plan = fusion.HostPlan(gates, tuple(range(10)), tile_bits=8, low_bits=3, block=256)
+    assert plan.srcs[-4:] == [0, 0, 0, 0]
+
+
 def test_gatekind_classification():
     assert bk.classify(O.gate_matrix('CNOT')).kind == 'monomial'
     assert bk.classify(O.gate_matrix('CNOT')).phase is None
@@ -135,8 +156,22 @@ def _phase_unitary(o, coefs, srcs, sel_val):
     value (selector diagonals)."""
     u = np.eye(16, dtype=complex)
     d2 = [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]
+    fm = 0                                  # the lane's flip mask
+
+    def holds(w):                           # selector j must equal bit 23 + j
+        return all(sel_val[(w >> (2 + 7 * j)) & 127] == (w >> (23 + j)) & 1
+                   for j in range(w & 3))
+
+    def flip(f):                            # a lane-flip word riding on a record
+        sels = [((f >> 3) & 127, (f >> 10) & 1)]
+        if f & 2048:
+            sels.append(((f >> 12) & 127, (f >> 19) & 1))
+        return (1 << ((f >> 1) & 3)) if all(sel_val[c] == val for c, val in sels) else 0
     for q in range(o.coef):
         typ, code, c0, mask = srcs[o.src + 4 * q: o.src + 4 * q + 4]
+        if typ >> 8:                        # a lane flip applies before the record
+            fm ^= flip(typ >> 8)
+        typ &= fusion.REC_TYPE
         if typ == fusion.SUB_DIAG:
             ph = np.where([(mask >> r) & 1 for r in range(16)], coefs[c0:c0 + 16], 1.0)
             u = np.diag(ph) @ u
@@ -144,30 +179,34 @@ def _phase_unitary(o, coefs, srcs, sel_val):
         if typ == fusion.SUB_DIAGV:
             m = code & 3
             v = sum(sel_val[(code >> (2 + 7 * j)) & 127] << j for j in range(m))
-            if mask:                        # compact tables over register bits d
+            rs = np.arange(16) ^ (fm & (mask >> 8) & 15)   # slots the lane reads
+            if mask & 16:                   # compact tables over register bits d
                 d = [j for j in range(4) if (mask >> j) & 1]
-                idx = [sum(((r >> j) & 1) << a for a, j in enumerate(d)) for r in range(16)]
+                idx = [sum(((r >> j) & 1) << a for a, j in enumerate(d)) for r in rs]
                 u = np.diag(coefs[c0 + (v << len(d)) + np.asarray(idx)]) @ u
             else:
-                u = np.diag(coefs[c0 + 16 * v:c0 + 16 * v + 16]) @ u
+                u = np.diag(coefs[c0 + 16 * v + rs]) @ u
             continue
         if typ == fusion.SUB_CX:
             # selector controls (mask): selector j must equal bit 23 + j
-            if any(sel_val[(mask >> (2 + 7 * j)) & 127] != (mask >> (23 + j)) & 1
-                   for j in range(mask & 3)):
+            if mask and not holds(mask):
                 continue
             perm = np.zeros((16, 16))
             t, cm, cv = code & 3, (code >> 2) & 15, (code >> 6) & 15
+            if code & fusion.CX_FLIP:
+                cv ^= fm & cm
             for r in range(16):
                 dst = r ^ (1 << t) if (r & cm) == cv else r
                 perm[dst, r] = 1
             u = perm @ u
             continue
         if typ in (fusion.SUB_D1, fusion.SUB_D1R, fusion.SUB_D1X):
-            bits = [code]
+            bits = [code & 3]
         else:
             bits = list(d2[code])
         k = len(bits)
+        if k == 1 and code & fusion.D1_FLIP and (fm >> bits[0]) & 1:
+            c0 += 4                         # X m X: the lane holds the pair reversed
         m = coefs[c0:c0 + 4 ** k].reshape(2 ** k, 2 ** k)
         # the specialised kernels read only part of each coefficient
         if typ == fusion.SUB_D1R:
@@ -190,7 +229,8 @@ def _phase_unitary(o, coefs, srcs, sel_val):
     store = np.zeros((16, 16))
     for r in range(16):
         load[r, affine(o.phase, r)] = 1
-        store[affine(o.active, r), r] = 1
+        # register r holds what register r ^ fm holds without flips
+        store[affine(o.active, r ^ fm), r] = 1
     return store @ u @ load
 
 
@@ -205,6 +245,10 @@ def _emulate_ops(ket, n, pos, ops, coefs, srcs):
             sel = set()                     # selector codes of the phase
             for q in range(o.coef):
                 typ, code = srcs[o.src + 4 * q], srcs[o.src + 4 * q + 1]
+                f = typ >> 8                # a lane flip's selectors
+                if f:
+                    sel |= {(f >> 3) & 127} | ({(f >> 12) & 127} if f & 2048 else set())
+                typ &= fusion.REC_TYPE
                 if typ == fusion.SUB_DIAGV:
                     sel |= {(code >> (2 + 7 * j)) & 127 for j in range(code & 3)}
                 if typ == fusion.SUB_CX:
@@ -265,6 +309,7 @@ def test_register_phase_encoding_replays(T, seed):
                 ops.append(fusion._encode_op(item[1], pidx, coefs, srcs))
         nphase += len(ops)
         fusion.prepare_ops(ops)
+        srcs.extend([0, 0, 0, 0])           # the record read ahead (as plans do)
         arr = (_native.QfxTileOp * len(ops))(*ops)
         assert _native.load().qfx_tile_check_ops(T, 256, 0, arr, len(ops), len(coefs),
                                                  _native.int_array(srcs), len(srcs)) == 0
@@ -340,6 +385,7 @@ def test_register_phases_permutation_heavy(seed):
             else:
                 ops.append(fusion._encode_op(item[1], pidx, coefs, srcs))
         fusion.prepare_ops(ops)
+        srcs.extend([0, 0, 0, 0])           # the record read ahead (as plans do)
         arr = (_native.QfxTileOp * len(ops))(*ops)
         assert _native.load().qfx_tile_check_ops(T, 256, 0, arr, len(ops), len(coefs),
                                                  _native.int_array(srcs), len(srcs)) == 0
@@ -648,6 +694,14 @@ def test_selector_diagonal_validator_rules():
     assert check_c(2 | (20 << 2) | (33 << 9), 16 | 0b0101, 15) == _native.QFX_EINVAL
     assert check_c(1 | (20 << 2), 16, 2) == 0                        # no register bit
     assert check_c(1 | (20 << 2), 32 | 1, 64) == _native.QFX_EINVAL  # stray flag bits
+    # flip-mask bits the tables read (mask bits 8..11): no more coefficients
+    assert check_c(1 | (20 << 2), 16 | 0b0101 | 0b0100 << 8, 8) == 0
+    assert check_c(1 | (20 << 2), 16 | 0b0101 | 0b0100 << 8, 7) == _native.QFX_EINVAL
+    assert check_c(0, 0b0011 << 8, 16) == 0                          # flip bits only
+    assert check_c(0, 0b0011 << 8, 15) == _native.QFX_EINVAL
+    assert check_c(0, 0, 16) == _native.QFX_EINVAL                   # no selector
+    assert check_c(1 | (20 << 2), 0b0101, 64) == _native.QFX_EINVAL  # dep without 16
+    assert check_c(1 | (20 << 2), 1 << 12, 64) == _native.QFX_EINVAL
     # lane-group selectors: tile bits outside the registers (here t = 0..3)
     assert check(1 | ((64 + 5) << 2), 32) == 0
     assert check(1 | ((64 + 2) << 2), 32) == _native.QFX_EINVAL         # a register bit
@@ -686,8 +740,12 @@ def test_selector_controls_replay(n, T, seed):
                 continue
             for q in range(o.coef):
                 typ, w = plan.srcs[o.src + 4 * q], plan.srcs[o.src + 4 * q + 3]
-                if typ == fusion.SUB_CX and w:
-                    codes = [(w >> (2 + 7 * j)) & 127 for j in range(w & 3)]
+                f = typ >> 8                # a lane flip
+                codes = [f >> 3 & 127] if f else []
+                codes += [f >> 12 & 127] if f & 2048 else []
+                if typ & fusion.REC_TYPE == fusion.SUB_CX and w:
+                    codes += [(w >> (2 + 7 * j)) & 127 for j in range(w & 3)]
+                if codes:
                     lane += any(c >= 64 for c in codes)
                     glob += any(c < 64 for c in codes)
         ket = _emulate_ops(ket, n, pos, ops, plan.coefs[c_lo:c_lo + ncoef], plan.srcs)
@@ -697,6 +755,93 @@ def test_selector_controls_replay(n, T, seed):
     plain = fusion.HostPlan(gates, tuple(range(n)), tile_bits=T, low_bits=3, block=256,
                             fuse_width=1, mode='reg', free_controls=False)
     assert plan.n_passes < plain.n_passes
+
+
+def _lane_flip_replay(seed, flips):
+    """Replay of a plan of a 14-qubit random circuit (max deviation from the
+    oracle's state) and its counts of flips and flagged records."""
+    n, T = 14, 9
+    desc = random_mixed_circuit(n, 14, seed=seed)
+    gates = build_gates(desc)
+    ref = O.run_circuit(desc, qubits=tuple(range(n)))
+    plan = fusion.HostPlan(gates, tuple(range(n)), tile_bits=T, low_bits=3, block=256,
+                           fuse_width=1, mode='reg', lane_flips=flips)
+    ket = O.zero_state(n)
+    count = {'flip': 0, 'd1': 0, 'cx': 0, 'past': 0}
+    for kind, payload in plan.passes:
+        pos, start, nops, block, flags, c_lo, ncoef, spos = payload
+        ops = plan.ops_host[start:start + nops]
+        for o in ops:
+            for q in range(o.coef if o.kind == fusion.OP_RPHASE else 0):
+                typ, code = plan.srcs[o.src + 4 * q: o.src + 4 * q + 2]
+                count['flip'] += bool(typ >> 8)
+                # a flip riding past the record before it (that record
+                # carries one too)
+                count['past'] += bool(typ >> 8) and q > 0 and bool(plan.srcs[o.src + 4 * q - 4] >> 8)
+                typ &= fusion.REC_TYPE
+                count['d1'] += (typ in (fusion.SUB_D1, fusion.SUB_D1R, fusion.SUB_D1X)
+                                and bool(code & fusion.D1_FLIP))
+                count['cx'] += typ == fusion.SUB_CX and bool(code & fusion.CX_FLIP)
+        ket = _emulate_ops(ket, n, pos, ops, plan.coefs[c_lo:c_lo + ncoef], plan.srcs)
+        ket = apply_store(ket, n, pos, spos)
+    return np.max(np.abs(ket - ref)), count
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2, 3])
+def test_lane_flips_replay(seed):
+    """A controlled X whose controls are all selectors becomes a lane flip
+    riding on the next record: no amplitudes move, later diagonals take it
+    into their selector tables, 2x2 matrices and CX gates reading its bit
+    carry their flag, and the phase's store puts every register where it
+    belongs.  Replayed, plans with and without flips give the circuit's
+    state."""
+    err, count = _lane_flip_replay(seed, True)
+    assert err < 1e-12 and count['flip'] > 0
+    err, count = _lane_flip_replay(seed, False)
+    assert err < 1e-12 and not any(count.values())
+
+
+def test_lane_flip_flags_all_seen():
+    """Over those circuits the plans hold flips on consecutive records and
+    flags of every kind, so the replays above cover each flagged reading."""
+    kinds = set()
+    for seed in range(4):
+        kinds |= {k for k, v in _lane_flip_replay(seed, True)[1].items() if v}
+    assert kinds == {'flip', 'd1', 'cx', 'past'}
+
+
+def test_lane_flip_validator_rules():
+    """A lane-flip word riding on a record (type bits 8..): bit 0 set,
+    selectors as for SUB_DIAGV, nothing above bit 19; flagged 2x2 records
+    need 8 coefficients."""
+    T = 9
+    pidx = {b: b for b in range(T)}
+    coefs, srcs = [], []
+    o = fusion._encode_phase([0, 1, 2, 3], [], pidx, T, coefs, srcs)
+    lib = _native.load()
+
+    def check(rec, ncoef=16):
+        o.src, o.coef = 0, 1
+        fusion.prepare_ops([o])
+        arr = (_native.QfxTileOp * 1)(o)
+        return lib.qfx_tile_check_ops(T, 256, 0, arr, 1, ncoef, _native.int_array(rec), 4)
+    f = 1 | (2 << 1) | (70 << 3) | (1 << 10)        # bit 2 where tile bit 6 is 1
+    g = f | (1 << 11) | (40 << 12)                   # and index bit 40 is 0
+    d1 = [fusion.SUB_D1R, 3, 0, 0]
+    assert check(d1) == 0
+    assert check([d1[0] | f << 8, 3, 0, 0]) == 0
+    assert check([d1[0] | g << 8, 3, 4, 0]) == 0
+    assert check([d1[0] | (f ^ 1) << 8, 3, 0, 0]) == _native.QFX_EINVAL      # bit 0 clear
+    assert check([d1[0] | (f | 1 << 20) << 8, 3, 0, 0]) == _native.QFX_EINVAL
+    assert check([d1[0] | (f | 1 << 12) << 8, 3, 0, 0]) == _native.QFX_EINVAL  # stray selector
+    bad = 1 | (66 << 3)                              # a register bit as selector
+    assert check([d1[0] | bad << 8, 3, 0, 0]) == _native.QFX_EINVAL
+    assert check([fusion.SUB_D1R, 3 | fusion.D1_FLIP, 8, 0]) == 0
+    assert check([fusion.SUB_D1R, 3 | fusion.D1_FLIP, 9, 0]) == _native.QFX_EINVAL
+    assert check([fusion.SUB_D1R, 8, 0, 0]) == _native.QFX_EINVAL
+    assert check([fusion.SUB_CX, 1 | (1 << 2) | (1 << 6) | fusion.CX_FLIP, 0, 0]) == 0
+    assert check([fusion.SUB_CX, 1 << 11, 0, 0]) == _native.QFX_EINVAL
+    assert check([fusion.SUB_DIAG, 1, 0, 0]) == _native.QFX_EINVAL
 
 
 def test_selector_control_validator_rules():

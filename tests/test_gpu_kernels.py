@@ -222,6 +222,33 @@ def test_tile_plan_permutations_exact(bk):
         assert np.array_equal(got, ref), mode
 
 
+@pytest.mark.parametrize("flips", [True, False])
+def test_lane_flips_on_device(bk, flips):
+    """Plans whose selector-controlled X gates are lane flips (no amplitudes
+    moved; folded tables, flagged reads, flipped stores), and the same plans without
+    flips, reproduce the oracle's state on random circuits of 15 and 16
+    qubits, for every register-phase kernel the plans use."""
+    import quantumflow_amd as qf
+    from quantumflow_amd import fusion
+    from quantumflow_amd.fusion import TilePlan
+    from quantumflow_amd.workloads import random_mixed_circuit, build_gates
+    rng = np.random.default_rng(33)
+    for n, T, block in ((15, 10, 64), (16, 12, 256)):
+        for seed in range(3):
+            print("lane flips=%s n=%d T=%d seed=%d" % (flips, n, T, seed), flush=True)
+            desc = random_mixed_circuit(n, 16, seed=100 + seed)
+            psi = rand_state(n, rng)
+            ref = O.run_circuit(desc, qubits=tuple(range(n)), ket=psi)
+            plan = TilePlan(build_gates(desc), tuple(range(n)), tile_bits=T, low_bits=3,
+                            block=block, fuse_width=1, mode='reg', lane_flips=flips)
+            nflip = sum(bool(plan.srcs[o.src + 4 * q] >> 8)
+                        for o in plan.ops_host if o.kind == fusion.OP_RPHASE
+                        for q in range(o.coef))
+            assert (nflip > 0) == flips
+            got = plan.run(qf.State(psi)).vec.asarray()
+            assert np.max(np.abs(got - ref)) <= TOL
+
+
 def test_tile_pass_of_only_absorbed_swaps(bk):
     """A pass left with no ops (its SWAP gates absorbed as a store
     permutation) still moves the amplitudes, bit for bit."""
