@@ -371,16 +371,28 @@ def subop_costs(n=N_QUBITS, reps=5):
                 out.append(qf.CNOT(q(20 + i % 8), t))
             elif kind == "cz_outside_tile":
                 out.append(qf.CZ(q(20 + i % 8), t))
+            elif kind == "rx_carrying_a_flip":
+                # a CNOT controlled outside the tile, then an RX on another
+                # bit: the CNOT becomes a lane flip riding on the RX record
+                out.append(qf.CNOT(q(20 + i % 8), t))
+                out.append(qf.RX(float(rng.uniform(0, 6)), q(3 + (i + 4) % 9)))
         return out
     work = bk.zero_state_tensor(n).reshape(-1).clone()
     out = {}
-    for kind in ("rx", "ry", "cx_in_registers", "cx_control_outside_tile", "cz_outside_tile"):
+    # (lone CNOTs controlled outside the tile have no record to ride on as
+    # lane flips: they stay exchange records)
+    for kind in ("rx", "ry", "cx_in_registers", "cx_control_outside_tile", "cz_outside_tile",
+                 "rx_carrying_a_flip"):
         row = {}
         for k in (16, 48):
             plan = fusion.TilePlan(gates(kind, k), tuple(range(n)), tile_bits=12, low_bits=3,
                                    block=256, fuse_width=1, mode="reg")
             subs = Counter(names[plan.srcs[o.src + 4 * j] & fusion.REC_TYPE] for o in plan.ops_host
                            if o.kind == fusion.OP_RPHASE for j in range(o.coef))
+            flips = sum(bool(plan.srcs[o.src + 4 * j] >> 8) for o in plan.ops_host
+                        if o.kind == fusion.OP_RPHASE for j in range(o.coef))
+            if flips:
+                subs["lane flips riding"] = flips
             plan.run_inplace(work)
             torch.cuda.synchronize()
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -393,7 +405,8 @@ def subop_costs(n=N_QUBITS, reps=5):
                                    "passes": plan.n_passes, "phases": plan.n_phases,
                                    "sub_ops": dict(subs)}
         a, b = row["gates_16"], row["gates_48"]
-        extra = sum(b["sub_ops"].values()) - sum(a["sub_ops"].values())
+        extra = sum(v for k2, v in b["sub_ops"].items() if k2 in names.values()) - \
+            sum(v for k2, v in a["sub_ops"].items() if k2 in names.values())
         if extra > 0 and a["passes"] == b["passes"] == 1:
             row["us_per_extra_sub_op"] = round((b["ms_per_pass"] - a["ms_per_pass"]) * 1e3 / extra, 1)
         out[kind] = row
