@@ -588,7 +588,7 @@ def d1_variant(m):
     return SUB_D1
 
 
-def _encode_phase(R, members, pos_index, T, coefs, subs, flips=True):
+def _encode_phase(R, members, pos_index, T, coefs, subs, flips=True, store_cx=True):
     """qfx_tile_op of a register phase over tile bits R plus its sub-ops.
 
     Logical bit A of the phase is tile bit regs[A]; register i starts out
@@ -792,6 +792,44 @@ def _encode_phase(R, members, pos_index, T, coefs, subs, flips=True):
         out.append([SUB_D1, j, m])
     flush()
 
+    # A CNOT with its control and target in registers that commutes with
+    # every sub-op after it in the phase moves to the phase's end, where
+    # the affine store map absorbs it (slot r then stores where E(r) does):
+    # no exchange record.  Diagonals after it take it into their tables.
+    def commutes(entry, c, t):
+        typ, code, data = entry
+        if typ in (SUB_DIAG, SUB_DIAGV):
+            return True                     # with its table re-indexed
+        if typ == SUB_CX:
+            return (code & 3) != c and not (code >> (2 + t)) & 1
+        m = data                            # 2x2 on register bit `code`
+        if code == t:                       # X-symmetric: RX-type, identity
+            return m[0, 0] == m[1, 1] and m[0, 1] == m[1, 0]
+        if code == c:
+            return m[0, 1] == 0 and m[1, 0] == 0
+        return True
+
+    deferred = {}                           # position -> register-slot permutation E
+    if store_cx:
+        for k, (typ, code, data) in enumerate(out):
+            if typ != SUB_CX:
+                continue
+            cm = (code >> 2) & 15
+            if data or bin(cm).count('1') != 1:
+                continue
+            c, t = cm.bit_length() - 1, code & 3
+            # everything after it, deferred exchanges included, commutes
+            if all(commutes(e, c, t) for e in out[k + 1:]):
+                cv = (code >> 6) & 15
+                deferred[k] = np.where((regs16 & cm) == cv, regs16 ^ (1 << t), regs16)
+        for k, e in deferred.items():
+            for j in range(k + 1, len(out)):
+                tj, cj, dj = out[j]
+                if tj in (SUB_DIAG, SUB_DIAGV):
+                    out[j] = [tj, cj, dj[:, e]]     # D'(r) = D(E(r))
+        out = [x for k, x in enumerate(out) if k not in deferred]
+        deferred = list(deferred.values())
+
     def deps(tab):                          # register bits a table depends on
         return sum(1 << j for j in range(REG_BITS)
                    if not np.array_equal(tab, tab[:, regs16 ^ (1 << j)]))
@@ -906,7 +944,10 @@ def _encode_phase(R, members, pos_index, T, coefs, subs, flips=True):
         subs.extend(rec)
         nsub += 1
     o.coef = nsub
-    o.active = _affine_code([phi(i) for i in range(2 ** REG_BITS)])   # store map
+    store = list(range(2 ** REG_BITS))
+    for e in deferred:                      # slot r stores where E(r) would
+        store = [int(e[x]) for x in store]
+    o.active = _affine_code([phi(x) for x in store])                  # store map
     o.phase = _affine_code(list(load_inv))                            # load map
     return o
 
@@ -1251,7 +1292,7 @@ class HostPlan:
 
     def __init__(self, gates, qubits, tile_bits=MAX_TILE_BITS, low_bits=3, block=256,
                  fuse_width=4, mode='reg', persistent=True, reg_io=True, reg_in=True,
-                 free_controls=True, virtual_swaps=True, lane_flips=True):
+                 free_controls=True, virtual_swaps=True, lane_flips=True, store_cx=True):
         qubits = tuple(qubits)
         n = len(qubits)
         self.n = n
@@ -1273,6 +1314,7 @@ class HostPlan:
         self.reg_in = reg_in
         self.free_controls = free_controls
         self.lane_flips = lane_flips
+        self.store_cx = store_cx
         self.gates = list(gates)
         bits0 = [tuple(n - 1 - qubits.index(q) for q in g.qubits) for g in gates]
         kinds0 = [g.kind for g in gates]
@@ -1371,7 +1413,7 @@ class HostPlan:
                 for item in plan_phases(ops, free_diagonals=free is not None):
                     if item[0] == 'phase':
                         for o in self._encode_fitting(item[1], item[2], pos_index, T,
-                                                      coefs, srcs, lane_flips):
+                                                      coefs, srcs, lane_flips, store_cx):
                             ops_host.append(o)
                             self.n_phases += 1
                     else:
@@ -1437,19 +1479,21 @@ class HostPlan:
                                                  src_arr, len(srcs)), 'qfx_tile_check_ops')
 
     @staticmethod
-    def _encode_fitting(R, members, pos_index, T, coefs, srcs, flips=True):
+    def _encode_fitting(R, members, pos_index, T, coefs, srcs, flips=True, store_cx=True):
         """Register phase(s) for ``members`` on bits R, each staging at most
         MAX_STAGED coefficients: a phase over the limit is cut into two
         consecutive phases of its first and second half (each leaves the
         tile in logical order, so they compose)."""
         c0, s0 = len(coefs), len(srcs)
-        o = _encode_phase(R, members, pos_index, T, coefs, srcs, flips)
+        o = _encode_phase(R, members, pos_index, T, coefs, srcs, flips, store_cx)
         if len(coefs) - c0 <= MAX_STAGED or len(members) == 1:
             return [o]
         del coefs[c0:], srcs[s0:]
         h = len(members) // 2
-        return (HostPlan._encode_fitting(R, members[:h], pos_index, T, coefs, srcs, flips)
-                + HostPlan._encode_fitting(R, members[h:], pos_index, T, coefs, srcs, flips))
+        return (HostPlan._encode_fitting(R, members[:h], pos_index, T, coefs, srcs, flips,
+                                         store_cx)
+                + HostPlan._encode_fitting(R, members[h:], pos_index, T, coefs, srcs, flips,
+                                           store_cx))
 
     def _add_tile_passes(self, k, ops_host, start, coefs, c_lo, srcs, block, flags):
         """Record tile pass k (ops_host[start:]) with coefficient indices

@@ -810,6 +810,32 @@ def test_lane_flip_flags_all_seen():
     assert kinds == {'flip', 'd1', 'cx', 'past'}
 
 
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_store_map_cnots_replay(seed):
+    """A CNOT with control and target in registers that commutes with every
+    later sub-op of its phase is absorbed by the phase's affine store map
+    (diagonals after it re-indexed): fewer exchange records, same state."""
+    n, T = 12, 9
+    desc = random_mixed_circuit(n, 20, seed=40 + seed)
+    gates = build_gates(desc)
+    ref = O.run_circuit(desc, qubits=tuple(range(n)))
+    ncx = {}
+    for store_cx in (True, False):
+        plan = fusion.HostPlan(gates, tuple(range(n)), tile_bits=T, low_bits=3, block=256,
+                               fuse_width=1, mode='reg', store_cx=store_cx)
+        ket = O.zero_state(n)
+        for kind, payload in plan.passes:
+            pos, start, nops, block, flags, c_lo, ncoef, spos = payload
+            ops = plan.ops_host[start:start + nops]
+            ket = _emulate_ops(ket, n, pos, ops, plan.coefs[c_lo:c_lo + ncoef], plan.srcs)
+            ket = apply_store(ket, n, pos, spos)
+        assert np.max(np.abs(ket - ref)) < 1e-12
+        ncx[store_cx] = sum(plan.srcs[o.src + 4 * q] & fusion.REC_TYPE == fusion.SUB_CX
+                            for o in plan.ops_host if o.kind == fusion.OP_RPHASE
+                            for q in range(o.coef))
+    assert ncx[True] < ncx[False]
+
+
 def test_lane_flip_validator_rules():
     """A lane-flip word riding on a record (type bits 8..): bit 0 set,
     selectors as for SUB_DIAGV, nothing above bit 19; flagged 2x2 records
