@@ -822,12 +822,32 @@ def _encode_phase(R, members, pos_index, T, coefs, subs, flips=True, store_cx=Tr
             if all(commutes(e, c, t) for e in out[k + 1:]):
                 cv = (code >> 6) & 15
                 deferred[k] = np.where((regs16 & cm) == cv, regs16 ^ (1 << t), regs16)
+        # the others that commute with every sub-op before them move to the
+        # phase's start, where the load map absorbs them (register s loads
+        # what register E(s) would: E is an involution)
+        front = {}
+        for k, (typ, code, data) in enumerate(out):
+            if typ != SUB_CX or k in deferred:
+                continue
+            cm = (code >> 2) & 15
+            if data or bin(cm).count('1') != 1:
+                continue
+            c, t = cm.bit_length() - 1, code & 3
+            if all(commutes(e, c, t) for e in out[:k]):
+                cv = (code >> 6) & 15
+                front[k] = np.where((regs16 & cm) == cv, regs16 ^ (1 << t), regs16)
         for k, e in deferred.items():
             for j in range(k + 1, len(out)):
                 tj, cj, dj = out[j]
                 if tj in (SUB_DIAG, SUB_DIAGV):
                     out[j] = [tj, cj, dj[:, e]]     # D'(r) = D(E(r))
-        out = [x for k, x in enumerate(out) if k not in deferred]
+        for k, e in front.items():
+            for j in range(k):
+                tj, cj, dj = out[j]
+                if tj in (SUB_DIAG, SUB_DIAGV):
+                    out[j] = [tj, cj, dj[:, e]]     # D'(r) = D(E(r)), E = E^-1
+            load_inv = load_inv[e]
+        out = [x for k, x in enumerate(out) if k not in deferred and k not in front]
         deferred = list(deferred.values())
 
     def deps(tab):                          # register bits a table depends on

@@ -813,8 +813,9 @@ def test_lane_flip_flags_all_seen():
 @pytest.mark.parametrize("seed", [0, 1, 2])
 def test_store_map_cnots_replay(seed):
     """A CNOT with control and target in registers that commutes with every
-    later sub-op of its phase is absorbed by the phase's affine store map
-    (diagonals after it re-indexed): fewer exchange records, same state."""
+    later sub-op of its phase is absorbed by the phase's affine store map,
+    one that commutes with every earlier sub-op by its load map (diagonals
+    it passes re-indexed): fewer exchange records, same state."""
     n, T = 12, 9
     desc = random_mixed_circuit(n, 20, seed=40 + seed)
     gates = build_gates(desc)
