@@ -881,7 +881,9 @@ def _encode_phase(R, members, pos_index, T, coefs, subs, flips=True, store_cx=Tr
 
     # Which exchanges become lane flips, and the record each rides on: the
     # first free record after it, passing only records that do not read its
-    # bit.  The others stay exchange records.
+    # bit; failing that, the nearest free record before it that does not
+    # read its bit either (the flip commutes with every record it passes).
+    # The others stay exchange records.
     carrier, virtual = {}, set()            # carrier: record -> flip
     for k, (typ, code, data) in enumerate(out):
         f = flip_of(code, data) if typ == SUB_CX else None
@@ -896,6 +898,15 @@ def _encode_phase(R, members, pos_index, T, coefs, subs, flips=True, store_cx=Tr
                 break
             if reads(i, f[0]):
                 break
+        if j is None:
+            for i in range(k - 1, -1, -1):
+                if not emits(i):
+                    continue
+                if reads(i, f[0]):
+                    break
+                if i not in carrier:
+                    j = i
+                    break
         if j is not None:
             carrier[j] = f
             virtual.add(k)
