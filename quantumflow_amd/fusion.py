@@ -792,6 +792,31 @@ def _encode_phase(R, members, pos_index, T, coefs, subs, flips=True, store_cx=Tr
         out.append([SUB_D1, j, m])
     flush()
 
+    # A 2x2 matrix on register bit j moves forward to the next one on j
+    # when nothing between reads or writes bit j, and the two become one
+    # record (their product).
+    def touches(entry, j):
+        typ, code, data = entry
+        if typ in (SUB_DIAG, SUB_DIAGV):
+            return not np.array_equal(data, data[:, regs16 ^ (1 << j)])
+        if typ == SUB_CX:
+            return (code & 3) == j or (code >> (2 + j)) & 1
+        return code == j
+
+    k = 0
+    while k < len(out):
+        if out[k][0] == SUB_D1:
+            j = out[k][1]
+            for k2 in range(k + 1, len(out)):
+                if out[k2][0] == SUB_D1 and out[k2][1] == j:
+                    out[k2] = [SUB_D1, j, out[k2][2] @ out[k][2]]
+                    del out[k]
+                    k -= 1
+                    break
+                if touches(out[k2], j):
+                    break
+        k += 1
+
     # A CNOT with its control and target in registers that commutes with
     # every sub-op after it in the phase moves to the phase's end, where
     # the affine store map absorbs it (slot r then stores where E(r) does):

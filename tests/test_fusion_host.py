@@ -837,6 +837,30 @@ def test_store_map_cnots_replay(seed):
     assert ncx[True] < ncx[False]
 
 
+def test_same_bit_matrices_merge_across_other_bits():
+    """Two 2x2 matrices on one register bit with only records that leave
+    that bit alone between them become one record (their product)."""
+    import quantumflow_amd as qf
+    n = 10
+    gates = [qf.RX(0.3, 9), qf.CZ(8, 7), qf.RY(0.2, 6), qf.RY(0.7, 9), qf.H(8)]
+    desc = [('RX', (0.3,), (9,)), ('CZ', (), (8, 7)), ('RY', (0.2,), (6,)),
+            ('RY', (0.7,), (9,)), ('H', (), (8,))]
+    ref = O.run_circuit(desc, qubits=tuple(range(n)))
+    plan = fusion.HostPlan(gates, tuple(range(n)), tile_bits=8, low_bits=3, block=256,
+                           fuse_width=1, mode='reg')
+    recs = [plan.srcs[o.src + 4 * q] & fusion.REC_TYPE for o in plan.ops_host
+            if o.kind == fusion.OP_RPHASE for q in range(o.coef)]
+    two_by_two = [t for t in recs if t in (fusion.SUB_D1, fusion.SUB_D1R, fusion.SUB_D1X)]
+    assert len(two_by_two) == 3             # RX and RY on qubit 9 merged
+    ket = O.zero_state(n)
+    for kind, payload in plan.passes:
+        pos, start, nops, block, flags, c_lo, ncoef, spos = payload
+        ket = _emulate_ops(ket, n, pos, plan.ops_host[start:start + nops],
+                           plan.coefs[c_lo:c_lo + ncoef], plan.srcs)
+        ket = apply_store(ket, n, pos, spos)
+    assert np.max(np.abs(ket - ref)) < 1e-12
+
+
 def test_lane_flip_validator_rules():
     """A lane-flip word riding on a record (type bits 8..): bit 0 set,
     selectors as for SUB_DIAGV, nothing above bit 19; flagged 2x2 records
