@@ -861,6 +861,31 @@ def test_same_bit_matrices_merge_across_other_bits():
     assert np.max(np.abs(ket - ref)) < 1e-12
 
 
+def test_diagonals_merge_across_other_records():
+    """A diagonal moves forward past records that leave its bits alone (here
+    a CNOT controlled outside the tile, whose selector control made the
+    planner emit the first diagonal early) to the next diagonal, and the
+    two become one record."""
+    import quantumflow_amd as qf
+    n = 10
+    desc = [('CZ', (), (9, 8)), ('CNOT', (), (0, 7)), ('PHASE', (0.4,), (9,)),
+            ('RY', (0.5,), (6,))]
+    gates = [qf.CZ(9, 8), qf.CNOT(0, 7), qf.PHASE(0.4, 9), qf.RY(0.5, 6)]
+    ref = O.run_circuit(desc, qubits=tuple(range(n)))
+    plan = fusion.HostPlan(gates, tuple(range(n)), tile_bits=8, low_bits=3, block=256,
+                           fuse_width=1, mode='reg', store_cx=False)
+    recs = [plan.srcs[o.src + 4 * q] & fusion.REC_TYPE for o in plan.ops_host
+            if o.kind == fusion.OP_RPHASE for q in range(o.coef)]
+    assert sum(t in (fusion.SUB_DIAG, fusion.SUB_DIAGV) for t in recs) == 1
+    ket = O.zero_state(n)
+    for kind, payload in plan.passes:
+        pos, start, nops, block, flags, c_lo, ncoef, spos = payload
+        ket = _emulate_ops(ket, n, pos, plan.ops_host[start:start + nops],
+                           plan.coefs[c_lo:c_lo + ncoef], plan.srcs)
+        ket = apply_store(ket, n, pos, spos)
+    assert np.max(np.abs(ket - ref)) < 1e-12
+
+
 def test_lane_flip_validator_rules():
     """A lane-flip word riding on a record (type bits 8..): bit 0 set,
     selectors as for SUB_DIAGV, nothing above bit 19; flagged 2x2 records
