@@ -803,51 +803,63 @@ def _encode_phase(R, members, pos_index, T, coefs, subs, flips=True, store_cx=Tr
             return (code & 3) == j or (code >> (2 + j)) & 1
         return code == j
 
-    k = 0
-    while k < len(out):
-        if out[k][0] == SUB_D1:
-            j = out[k][1]
-            for k2 in range(k + 1, len(out)):
-                if out[k2][0] == SUB_D1 and out[k2][1] == j:
-                    out[k2] = [SUB_D1, j, out[k2][2] @ out[k][2]]
-                    del out[k]
-                    k -= 1
-                    break
-                if touches(out[k2], j):
-                    break
-        k += 1
+    def merge_matrices():
+        changed = False
+        k = 0
+        while k < len(out):
+            if out[k][0] == SUB_D1:
+                j = out[k][1]
+                for k2 in range(k + 1, len(out)):
+                    if out[k2][0] == SUB_D1 and out[k2][1] == j:
+                        out[k2] = [SUB_D1, j, out[k2][2] @ out[k][2]]
+                        del out[k]
+                        changed = True
+                        k -= 1
+                        break
+                    if touches(out[k2], j):
+                        break
+            k += 1
+        return changed
 
     # A diagonal moves forward to the next diagonal past 2x2 matrices on
     # bits it does not read and exchanges (an exchange E without selector
     # controls re-indexes it, D'(r) = D(E(r)); one with selector controls
     # must leave its bits alone), and the two become one record when their
     # selectors fit one table.
-    k = 0
-    while k < len(out):
-        if out[k][0] in (SUB_DIAG, SUB_DIAGV):
-            sel, tab = out[k][1], out[k][2]
-            for k2 in range(k + 1, len(out)):
-                t2, c2, d2 = out[k2]
-                if t2 in (SUB_DIAG, SUB_DIAGV):
-                    union = list(sel) + [b for b in c2 if b not in sel]
-                    if len(union) <= MAX_SEL_BITS:
-                        merged = expand(tab, list(sel), union) * expand(d2, list(c2), union)
-                        out[k2] = [SUB_DIAGV if union else SUB_DIAG, union, merged]
-                        del out[k]
-                        k -= 1
-                    break
-                if t2 == SUB_CX:
-                    if d2:
-                        if touches([SUB_DIAG, sel, tab], c2 & 3):
-                            break
+    def merge_diagonals():
+        changed = False
+        k = 0
+        while k < len(out):
+            if out[k][0] in (SUB_DIAG, SUB_DIAGV):
+                sel, tab = out[k][1], out[k][2]
+                for k2 in range(k + 1, len(out)):
+                    t2, c2, d2 = out[k2]
+                    if t2 in (SUB_DIAG, SUB_DIAGV):
+                        union = list(sel) + [b for b in c2 if b not in sel]
+                        if len(union) <= MAX_SEL_BITS:
+                            prod = expand(tab, list(sel), union) * expand(d2, list(c2), union)
+                            out[k2] = [SUB_DIAGV if union else SUB_DIAG, union, prod]
+                            del out[k]
+                            changed = True
+                            k -= 1
+                        break
+                    if t2 == SUB_CX:
+                        if d2:
+                            if touches([SUB_DIAG, sel, tab], c2 & 3):
+                                break
+                            continue
+                        cm, cv = (c2 >> 2) & 15, (c2 >> 6) & 15
+                        tab = tab[:, np.where((regs16 & cm) == cv, regs16 ^ (1 << (c2 & 3)),
+                                              regs16)]
                         continue
-                    cm, cv = (c2 >> 2) & 15, (c2 >> 6) & 15
-                    tab = tab[:, np.where((regs16 & cm) == cv, regs16 ^ (1 << (c2 & 3)),
-                                          regs16)]
-                    continue
-                if touches([SUB_DIAG, sel, tab], c2):
-                    break
-        k += 1
+                    if touches([SUB_DIAG, sel, tab], c2):
+                        break
+            k += 1
+        return changed
+
+    # each merge can open the way for the other kind
+    while merge_matrices() | merge_diagonals():
+        pass
 
     # A CNOT with its control and target in registers that commutes with
     # every sub-op after it in the phase moves to the phase's end, where
