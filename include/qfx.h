@@ -220,7 +220,9 @@ typedef struct {
  * are 4 ints {type, position code, coef index, mask}, starting at a multiple
  * of 4 ints in srcs (srcs holds at least one record at `src`, also for a
  * phase without sub-ops: the first record is read ahead):
- *   1  2x2 matrix (4 coefs) on register bit `code`;
+ *   1  2x2 matrix (4 coefs) on register bit `code`; with selectors in
+ *      `mask` (m = mask & 3 codes as type 9's) one matrix per selector
+ *      value v, at coef index 4 v (types 7 and 8 alike);
  *   7  real 2x2 matrix (real parts of the 4 coefs are read);
  *   8  2x2 matrix with real diagonal and imaginary off-diagonal (real parts
  *      of coefs 0 and 3, imaginary parts of coefs 1 and 2 are read);
@@ -248,8 +250,8 @@ typedef struct {
  * 12-18, value in bit 19).  A lane whose selectors hold records the X in its
  * flip mask f: register r then holds what register r ^ f holds without
  * flips, and the phase stores it there.  Records after a flip that read a
- * flipped bit carry a flag: types 1/7/8 code | 4 with 8 coefs (m, then
- * X m X, read on lanes where the bit is flipped), type 5 code | 1 << 10
+ * flipped bit carry a flag: types 1/7/8 code | 4 with 8 coefs per matrix
+ * (m, then X m X, read on lanes where the bit is flipped; matrix v at 8 v), type 5 code | 1 << 10
  * (control values compared through f); diagonals are type 9 records that
  * name the flip-mask bits they read.
  * A register phase's `phase` and `active` words are its load and store

@@ -367,6 +367,28 @@ __device__ __forceinline__ bool selectors_hold(unsigned w, uint64_t gbase, unsig
   return ok;
 }
 
+// Coefficient offset of a 2x2 record on this lane: with selectors in its
+// mask word (coded as SUB_DIAGV's code) the record holds one set of
+// coefficients per selector value v, 4 each, 8 when flagged (m, then
+// X m X, the second read where the bit is flipped).
+__device__ __forceinline__ unsigned d1_offset(const int4 sub, unsigned fm, uint64_t gbase,
+                                              unsigned egrp) {
+  const int code = sub.y;
+  const bool flag = code & kD1Flip;
+  unsigned off = flag ? (((fm >> (code & 3)) & 1u) << 2) : 0u;
+  const unsigned w = (unsigned)sub.w;
+  unsigned sel = 0;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    if (j < (int)(w & 3u)) {
+      const unsigned p = (w >> (2 + 7 * j)) & 127u;
+      const unsigned b = p < 64u ? (unsigned)((gbase >> p) & 1u) : (egrp >> (p - 64u)) & 1u;
+      sel |= b << j;
+    }
+  }
+  return off + (sel << (flag ? 3 : 2));
+}
+
 // Register-bit mask of a lane flip word (0 where its selectors do not hold).
 __device__ __forceinline__ unsigned flip_of(unsigned f, uint64_t gbase, unsigned egrp) {
   const unsigned p0 = (f >> 3) & 127u, p1 = (f >> 12) & 127u;
@@ -406,7 +428,7 @@ __device__ __forceinline__ void apply_sub(v2d (&v)[kRegs], const int4 sub, const
   switch (type) {
     case SUB_D1: {
       // flagged: 8 coefficients, X m X in the second four for flipped lanes
-      const v2d* mm = m + ((code & kD1Flip) ? (((fm >> (code & 3)) & 1u) << 2) : 0u);
+      const v2d* mm = m + d1_offset(sub, fm, gbase, egrp);
       switch (code & 3) {
         case 0: sub_d1<0>(v, mm[0], mm[1], mm[2], mm[3]); break;
         case 1: sub_d1<1>(v, mm[0], mm[1], mm[2], mm[3]); break;
@@ -417,7 +439,7 @@ __device__ __forceinline__ void apply_sub(v2d (&v)[kRegs], const int4 sub, const
       break;
     }
     case SUB_D1R: {
-      const v2d* mm = m + ((code & kD1Flip) ? (((fm >> (code & 3)) & 1u) << 2) : 0u);
+      const v2d* mm = m + d1_offset(sub, fm, gbase, egrp);
       switch (code & 3) {
         case 0: sub_d1r<0>(v, mm[0].x, mm[1].x, mm[2].x, mm[3].x); break;
         case 1: sub_d1r<1>(v, mm[0].x, mm[1].x, mm[2].x, mm[3].x); break;
@@ -428,7 +450,7 @@ __device__ __forceinline__ void apply_sub(v2d (&v)[kRegs], const int4 sub, const
       break;
     }
     case SUB_D1X: {
-      const v2d* mm = m + ((code & kD1Flip) ? (((fm >> (code & 3)) & 1u) << 2) : 0u);
+      const v2d* mm = m + d1_offset(sub, fm, gbase, egrp);
       switch (code & 3) {
         case 0: sub_d1x<0>(v, mm[0].x, mm[1].y, mm[2].y, mm[3].x); break;
         case 1: sub_d1x<1>(v, mm[0].x, mm[1].y, mm[2].y, mm[3].x); break;
@@ -1506,7 +1528,16 @@ int qfx_tile_check_ops(int T, int block, int flags, const qfx_tile_op* ops, int 
           if (w != 0 && !controls_ok(w))
             return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: bad CX selector controls");
         }
-        const int nc = need[ty] << ((ty != SUB_CX && (cd & kD1Flip)) ? 1 : 0);
+        int nc = need[ty] << ((ty != SUB_CX && (cd & kD1Flip)) ? 1 : 0);
+        if (ty == SUB_D1 || ty == SUB_D1R || ty == SUB_D1X) {
+          // selectors: none, or m = w & 3 codes as SUB_DIAGV's, one set of
+          // coefficients per selector value
+          const unsigned w = (unsigned)sub[3], m = w & 3u;
+          bool good = (w >> (2 + 7 * m)) == 0;
+          for (unsigned j = 0; good && j < m; ++j) good = selector_ok((w >> (2 + 7 * j)) & 127u);
+          if (!good) return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: bad 2x2 selectors");
+          nc <<= m;
+        }
         if (nc && cz + nc > ncoef)
           return qfx::set_error(QFX_EINVAL, "qfx_tile_check_ops: sub-op coefficients out of range");
       }

@@ -588,7 +588,8 @@ def d1_variant(m):
     return SUB_D1
 
 
-def _encode_phase(R, members, pos_index, T, coefs, subs, flips=True, store_cx=True):
+def _encode_phase(R, members, pos_index, T, coefs, subs, flips=True, store_cx=True,
+                  fold_diag=True):
     """qfx_tile_op of a register phase over tile bits R plus its sub-ops.
 
     Logical bit A of the phase is tile bit regs[A]; register i starts out
@@ -795,6 +796,10 @@ def _encode_phase(R, members, pos_index, T, coefs, subs, flips=True, store_cx=Tr
     # A 2x2 matrix on register bit j moves forward to the next one on j
     # when nothing between reads or writes bit j, and the two become one
     # record (their product).
+    def deps(tab):                          # register bits a table depends on
+        return sum(1 << j for j in range(REG_BITS)
+                   if not np.array_equal(tab, tab[:, regs16 ^ (1 << j)]))
+
     def touches(entry, j):
         typ, code, data = entry
         if typ in (SUB_DIAG, SUB_DIAGV):
@@ -811,6 +816,8 @@ def _encode_phase(R, members, pos_index, T, coefs, subs, flips=True, store_cx=Tr
                 j = out[k][1]
                 for k2 in range(k + 1, len(out)):
                     if out[k2][0] == SUB_D1 and out[k2][1] == j:
+                        if isinstance(out[k2][2], tuple) or isinstance(out[k][2], tuple):
+                            break
                         out[k2] = [SUB_D1, j, out[k2][2] @ out[k][2]]
                         del out[k]
                         changed = True
@@ -861,6 +868,45 @@ def _encode_phase(R, members, pos_index, T, coefs, subs, flips=True, store_cx=Tr
     while merge_matrices() | merge_diagonals():
         pass
 
+    # A diagonal that reads at most register bit j folds into the 2x2 matrix
+    # on j that it reaches past records leaving bit j alone: per selector
+    # value v the record then holds m diag(d_v) (diagonal first) or
+    # diag(d_v) m; a D1 entry with selectors carries (sel, matrices).
+    def fold_diagonals():
+        k = 0
+        while k < len(out):
+            typ, sel, tab = out[k]
+            dm = deps(tab) if typ in (SUB_DIAG, SUB_DIAGV) else None
+            if dm is None or dm & (dm - 1):
+                k += 1
+                continue
+            j = dm.bit_length() - 1 if dm else None
+            for step in (1, -1):
+                k2 = k + step
+                while 0 <= k2 < len(out):
+                    t2, c2, d2 = out[k2]
+                    if t2 == SUB_D1 and (j is None or c2 == j) and not isinstance(d2, tuple):
+                        break
+                    if t2 in (SUB_DIAG, SUB_DIAGV) or (j is not None and touches(out[k2], j)):
+                        k2 = -1
+                        break
+                    k2 += step
+                if 0 <= k2 < len(out):
+                    break
+            else:
+                k += 1
+                continue
+            bit = out[k2][1]
+            m = out[k2][2]
+            mats = []
+            for v in range(tab.shape[0]):
+                d = np.diag([tab[v][0], tab[v][1 << bit]])
+                mats.append(m @ d if step == 1 else d @ m)
+            out[k2] = [SUB_D1, bit, (list(sel), np.asarray(mats)) if sel else mats[0]]
+            del out[k]
+    if fold_diag:
+        fold_diagonals()
+
     # A CNOT with its control and target in registers that commutes with
     # every sub-op after it in the phase moves to the phase's end, where
     # the affine store map absorbs it (slot r then stores where E(r) does):
@@ -872,6 +918,8 @@ def _encode_phase(R, members, pos_index, T, coefs, subs, flips=True, store_cx=Tr
         if typ == SUB_CX:
             return (code & 3) != c and not (code >> (2 + t)) & 1
         m = data                            # 2x2 on register bit `code`
+        if isinstance(m, tuple):            # one matrix per selector value
+            return code not in (c, t)
         if code == t:                       # X-symmetric: RX-type, identity
             return m[0, 0] == m[1, 1] and m[0, 1] == m[1, 0]
         if code == c:
@@ -918,10 +966,6 @@ def _encode_phase(R, members, pos_index, T, coefs, subs, flips=True, store_cx=Tr
             load_inv = load_inv[e]
         out = [x for k, x in enumerate(out) if k not in deferred and k not in front]
         deferred = list(deferred.values())
-
-    def deps(tab):                          # register bits a table depends on
-        return sum(1 << j for j in range(REG_BITS)
-                   if not np.array_equal(tab, tab[:, regs16 ^ (1 << j)]))
 
     def flip_of(code, data):
         """(register bit, [(selector code, value)]) of a CX entry that can
@@ -998,7 +1042,9 @@ def _encode_phase(R, members, pos_index, T, coefs, subs, flips=True, store_cx=Tr
         for t, _ in active:
             flipped |= 1 << t
         if typ == SUB_D1:
-            typ = d1_variant(data)
+            ms = data[1] if isinstance(data, tuple) else [data]
+            kinds_ = {d1_variant(x) for x in ms}
+            typ = kinds_.pop() if len(kinds_) == 1 else SUB_D1
         if typ in (SUB_DIAG, SUB_DIAGV):
             # the flip-mask bits a table reads: slot r takes the entry of
             # slot r ^ (flips & fmask) on each lane
@@ -1033,12 +1079,20 @@ def _encode_phase(R, members, pos_index, T, coefs, subs, flips=True, store_cx=Tr
             cm = (code >> 2) & 15
             rec = [typ, code | (CX_FLIP if flipped & cm else 0), 0, data or 0]
         else:
-            m = np.asarray(data, dtype=np.complex128)
-            rec = [typ, code, len(coefs), 0]
-            coefs.extend(m.reshape(-1))
-            if (flipped >> code) & 1:       # X m X for lanes holding the pair reversed
+            # one matrix, or one per selector value (a folded diagonal)
+            sel, mats = data if isinstance(data, tuple) else ([], [data])
+            mats = [np.asarray(x, dtype=np.complex128) for x in mats]
+            word = len(sel)
+            for jj, b in enumerate(sel):
+                word |= b << (2 + 7 * jj)
+            rec = [typ, code, len(coefs), word]
+            flag = (flipped >> code) & 1
+            if flag:                        # X m X for lanes holding the pair reversed
                 rec[1] |= D1_FLIP
-                coefs.extend(m[::-1, ::-1].reshape(-1))
+            for x in mats:
+                coefs.extend(x.reshape(-1))
+                if flag:
+                    coefs.extend(x[::-1, ::-1].reshape(-1))
         if k in carrier:
             rec[0] |= flip_word(carrier[k]) << 8
         subs.extend(rec)
@@ -1392,7 +1446,8 @@ class HostPlan:
 
     def __init__(self, gates, qubits, tile_bits=MAX_TILE_BITS, low_bits=3, block=256,
                  fuse_width=4, mode='reg', persistent=True, reg_io=True, reg_in=True,
-                 free_controls=True, virtual_swaps=True, lane_flips=True, store_cx=True):
+                 free_controls=True, virtual_swaps=True, lane_flips=True, store_cx=True,
+                 fold_diag=True):
         qubits = tuple(qubits)
         n = len(qubits)
         self.n = n
@@ -1415,6 +1470,7 @@ class HostPlan:
         self.free_controls = free_controls
         self.lane_flips = lane_flips
         self.store_cx = store_cx
+        self.fold_diag = fold_diag
         self.gates = list(gates)
         bits0 = [tuple(n - 1 - qubits.index(q) for q in g.qubits) for g in gates]
         kinds0 = [g.kind for g in gates]
@@ -1513,7 +1569,8 @@ class HostPlan:
                 for item in plan_phases(ops, free_diagonals=free is not None):
                     if item[0] == 'phase':
                         for o in self._encode_fitting(item[1], item[2], pos_index, T,
-                                                      coefs, srcs, lane_flips, store_cx):
+                                                      coefs, srcs, lane_flips, store_cx,
+                                                      fold_diag):
                             ops_host.append(o)
                             self.n_phases += 1
                     else:
@@ -1579,21 +1636,22 @@ class HostPlan:
                                                  src_arr, len(srcs)), 'qfx_tile_check_ops')
 
     @staticmethod
-    def _encode_fitting(R, members, pos_index, T, coefs, srcs, flips=True, store_cx=True):
+    def _encode_fitting(R, members, pos_index, T, coefs, srcs, flips=True, store_cx=True,
+                        fold_diag=True):
         """Register phase(s) for ``members`` on bits R, each staging at most
         MAX_STAGED coefficients: a phase over the limit is cut into two
         consecutive phases of its first and second half (each leaves the
         tile in logical order, so they compose)."""
         c0, s0 = len(coefs), len(srcs)
-        o = _encode_phase(R, members, pos_index, T, coefs, srcs, flips, store_cx)
+        o = _encode_phase(R, members, pos_index, T, coefs, srcs, flips, store_cx, fold_diag)
         if len(coefs) - c0 <= MAX_STAGED or len(members) == 1:
             return [o]
         del coefs[c0:], srcs[s0:]
         h = len(members) // 2
         return (HostPlan._encode_fitting(R, members[:h], pos_index, T, coefs, srcs, flips,
-                                         store_cx)
+                                         store_cx, fold_diag)
                 + HostPlan._encode_fitting(R, members[h:], pos_index, T, coefs, srcs, flips,
-                                           store_cx))
+                                           store_cx, fold_diag))
 
     def _add_tile_passes(self, k, ops_host, start, coefs, c_lo, srcs, block, flags):
         """Record tile pass k (ops_host[start:]) with coefficient indices

@@ -205,6 +205,9 @@ def _phase_unitary(o, coefs, srcs, sel_val):
         else:
             bits = list(d2[code])
         k = len(bits)
+        if k == 1 and mask:                 # one matrix per selector value
+            v = sum(sel_val[(mask >> (2 + 7 * j)) & 127] << j for j in range(mask & 3))
+            c0 += v * (8 if code & fusion.D1_FLIP else 4)
         if k == 1 and code & fusion.D1_FLIP and (fm >> bits[0]) & 1:
             c0 += 4                         # X m X: the lane holds the pair reversed
         m = coefs[c0:c0 + 4 ** k].reshape(2 ** k, 2 ** k)
@@ -251,6 +254,9 @@ def _emulate_ops(ket, n, pos, ops, coefs, srcs):
                 typ &= fusion.REC_TYPE
                 if typ == fusion.SUB_DIAGV:
                     sel |= {(code >> (2 + 7 * j)) & 127 for j in range(code & 3)}
+                if typ in (fusion.SUB_D1, fusion.SUB_D1R, fusion.SUB_D1X):
+                    w = srcs[o.src + 4 * q + 3]
+                    sel |= {(w >> (2 + 7 * j)) & 127 for j in range(w & 3)}
                 if typ == fusion.SUB_CX:
                     w = srcs[o.src + 4 * q + 3]
                     sel |= {(w >> (2 + 7 * j)) & 127 for j in range(w & 3)}
@@ -872,18 +878,22 @@ def test_diagonals_merge_across_other_records():
             ('RY', (0.5,), (6,))]
     gates = [qf.CZ(9, 8), qf.CNOT(0, 7), qf.PHASE(0.4, 9), qf.RY(0.5, 6)]
     ref = O.run_circuit(desc, qubits=tuple(range(n)))
-    plan = fusion.HostPlan(gates, tuple(range(n)), tile_bits=8, low_bits=3, block=256,
-                           fuse_width=1, mode='reg', store_cx=False)
-    recs = [plan.srcs[o.src + 4 * q] & fusion.REC_TYPE for o in plan.ops_host
-            if o.kind == fusion.OP_RPHASE for q in range(o.coef)]
-    assert sum(t in (fusion.SUB_DIAG, fusion.SUB_DIAGV) for t in recs) == 1
-    ket = O.zero_state(n)
-    for kind, payload in plan.passes:
-        pos, start, nops, block, flags, c_lo, ncoef, spos = payload
-        ket = _emulate_ops(ket, n, pos, plan.ops_host[start:start + nops],
-                           plan.coefs[c_lo:c_lo + ncoef], plan.srcs)
-        ket = apply_store(ket, n, pos, spos)
-    assert np.max(np.abs(ket - ref)) < 1e-12
+    for fold in (False, True):
+        plan = fusion.HostPlan(gates, tuple(range(n)), tile_bits=8, low_bits=3, block=256,
+                               fuse_width=1, mode='reg', store_cx=False, fold_diag=fold)
+        recs = [plan.srcs[o.src + 4 * q] & fusion.REC_TYPE for o in plan.ops_host
+                if o.kind == fusion.OP_RPHASE for q in range(o.coef)]
+        # folded, the merged diagonal (on lane-group bits only) rides in the
+        # RY record as one matrix per selector value
+        assert sum(t in (fusion.SUB_DIAG, fusion.SUB_DIAGV) for t in recs) == (0 if fold else 1)
+        assert len(recs) == (1 if fold else 2)
+        ket = O.zero_state(n)
+        for kind, payload in plan.passes:
+            pos, start, nops, block, flags, c_lo, ncoef, spos = payload
+            ket = _emulate_ops(ket, n, pos, plan.ops_host[start:start + nops],
+                               plan.coefs[c_lo:c_lo + ncoef], plan.srcs)
+            ket = apply_store(ket, n, pos, spos)
+        assert np.max(np.abs(ket - ref)) < 1e-12
 
 
 def test_lane_flip_validator_rules():
@@ -1024,8 +1034,10 @@ def test_plan_flops_closed_form():
     cases = [([qf.H(0)], 6), ([qf.RX(0.3, 1)], 6), ([qf.RZ(0.3, 2)], 6), ([qf.CNOT(0, 5)], 0),
              ([qf.TX(0.3, 1)], 14), ([qf.H(0), qf.RZ(0.2, 3), qf.CNOT(1, 2), qf.RY(0.4, 4)], 18)]
     for gl, per_amp in cases:
+        # (diagonals kept as records of their own: folded into a 2x2 matrix
+        # they turn it into a general one)
         plan = fusion.HostPlan(gl, tuple(range(n)), tile_bits=8, low_bits=3, block=16,
-                               fuse_width=1, mode='reg')
+                               fuse_width=1, mode='reg', fold_diag=False)
         assert plan.flops() == [per_amp * 2 ** n], [g.name for g in gl]
 
 
