@@ -385,8 +385,9 @@ def subop_costs(n=N_QUBITS, reps=5):
                  "rx_carrying_a_flip"):
         row = {}
         for k in (16, 48):
+            # records kept apart (no merging), so that each gate is one record
             plan = fusion.TilePlan(gates(kind, k), tuple(range(n)), tile_bits=12, low_bits=3,
-                                   block=256, fuse_width=1, mode="reg")
+                                   block=256, fuse_width=1, mode="reg", merge_records=False)
             subs = Counter(names[plan.srcs[o.src + 4 * j] & fusion.REC_TYPE] for o in plan.ops_host
                            if o.kind == fusion.OP_RPHASE for j in range(o.coef))
             flips = sum(bool(plan.srcs[o.src + 4 * j] >> 8) for o in plan.ops_host

@@ -589,7 +589,7 @@ def d1_variant(m):
 
 
 def _encode_phase(R, members, pos_index, T, coefs, subs, flips=True, store_cx=True,
-                  fold_diag=True):
+                  merge_records=True):
     """qfx_tile_op of a register phase over tile bits R plus its sub-ops.
 
     Logical bit A of the phase is tile bit regs[A]; register i starts out
@@ -865,7 +865,7 @@ def _encode_phase(R, members, pos_index, T, coefs, subs, flips=True, store_cx=Tr
         return changed
 
     # each merge can open the way for the other kind
-    while merge_matrices() | merge_diagonals():
+    while merge_records and merge_matrices() | merge_diagonals():
         pass
 
     # A diagonal that reads at most register bit j folds into the 2x2 matrix
@@ -904,7 +904,7 @@ def _encode_phase(R, members, pos_index, T, coefs, subs, flips=True, store_cx=Tr
                 mats.append(m @ d if step == 1 else d @ m)
             out[k2] = [SUB_D1, bit, (list(sel), np.asarray(mats)) if sel else mats[0]]
             del out[k]
-    if fold_diag:
+    if merge_records:
         fold_diagonals()
 
     # A CNOT with its control and target in registers that commutes with
@@ -1447,7 +1447,7 @@ class HostPlan:
     def __init__(self, gates, qubits, tile_bits=MAX_TILE_BITS, low_bits=3, block=256,
                  fuse_width=4, mode='reg', persistent=True, reg_io=True, reg_in=True,
                  free_controls=True, virtual_swaps=True, lane_flips=True, store_cx=True,
-                 fold_diag=True):
+                 merge_records=True):
         qubits = tuple(qubits)
         n = len(qubits)
         self.n = n
@@ -1470,7 +1470,7 @@ class HostPlan:
         self.free_controls = free_controls
         self.lane_flips = lane_flips
         self.store_cx = store_cx
-        self.fold_diag = fold_diag
+        self.merge_records = merge_records
         self.gates = list(gates)
         bits0 = [tuple(n - 1 - qubits.index(q) for q in g.qubits) for g in gates]
         kinds0 = [g.kind for g in gates]
@@ -1570,7 +1570,7 @@ class HostPlan:
                     if item[0] == 'phase':
                         for o in self._encode_fitting(item[1], item[2], pos_index, T,
                                                       coefs, srcs, lane_flips, store_cx,
-                                                      fold_diag):
+                                                      merge_records):
                             ops_host.append(o)
                             self.n_phases += 1
                     else:
@@ -1637,21 +1637,21 @@ class HostPlan:
 
     @staticmethod
     def _encode_fitting(R, members, pos_index, T, coefs, srcs, flips=True, store_cx=True,
-                        fold_diag=True):
+                        merge_records=True):
         """Register phase(s) for ``members`` on bits R, each staging at most
         MAX_STAGED coefficients: a phase over the limit is cut into two
         consecutive phases of its first and second half (each leaves the
         tile in logical order, so they compose)."""
         c0, s0 = len(coefs), len(srcs)
-        o = _encode_phase(R, members, pos_index, T, coefs, srcs, flips, store_cx, fold_diag)
+        o = _encode_phase(R, members, pos_index, T, coefs, srcs, flips, store_cx, merge_records)
         if len(coefs) - c0 <= MAX_STAGED or len(members) == 1:
             return [o]
         del coefs[c0:], srcs[s0:]
         h = len(members) // 2
         return (HostPlan._encode_fitting(R, members[:h], pos_index, T, coefs, srcs, flips,
-                                         store_cx, fold_diag)
+                                         store_cx, merge_records)
                 + HostPlan._encode_fitting(R, members[h:], pos_index, T, coefs, srcs, flips,
-                                           store_cx, fold_diag))
+                                           store_cx, merge_records))
 
     def _add_tile_passes(self, k, ops_host, start, coefs, c_lo, srcs, block, flags):
         """Record tile pass k (ops_host[start:]) with coefficient indices

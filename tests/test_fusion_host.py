@@ -871,22 +871,22 @@ def test_diagonals_merge_across_other_records():
     """A diagonal moves forward past records that leave its bits alone (here
     a CNOT controlled outside the tile, whose selector control made the
     planner emit the first diagonal early) to the next diagonal, and the
-    two become one record."""
+    two become one record; without merging they stay apart."""
     import quantumflow_amd as qf
     n = 10
     desc = [('CZ', (), (9, 8)), ('CNOT', (), (0, 7)), ('PHASE', (0.4,), (9,)),
             ('RY', (0.5,), (6,))]
     gates = [qf.CZ(9, 8), qf.CNOT(0, 7), qf.PHASE(0.4, 9), qf.RY(0.5, 6)]
     ref = O.run_circuit(desc, qubits=tuple(range(n)))
-    for fold in (False, True):
+    for merge in (False, True):
         plan = fusion.HostPlan(gates, tuple(range(n)), tile_bits=8, low_bits=3, block=256,
-                               fuse_width=1, mode='reg', store_cx=False, fold_diag=fold)
+                               fuse_width=1, mode='reg', store_cx=False, merge_records=merge)
         recs = [plan.srcs[o.src + 4 * q] & fusion.REC_TYPE for o in plan.ops_host
                 if o.kind == fusion.OP_RPHASE for q in range(o.coef)]
-        # folded, the merged diagonal (on lane-group bits only) rides in the
-        # RY record as one matrix per selector value
-        assert sum(t in (fusion.SUB_DIAG, fusion.SUB_DIAGV) for t in recs) == (0 if fold else 1)
-        assert len(recs) == (1 if fold else 2)
+        # merged, the two diagonals (on lane-group bits only) become one, and
+        # that rides in the RY record as one matrix per selector value
+        assert sum(t in (fusion.SUB_DIAG, fusion.SUB_DIAGV) for t in recs) == (0 if merge else 2)
+        assert len(recs) == (1 if merge else 3)
         ket = O.zero_state(n)
         for kind, payload in plan.passes:
             pos, start, nops, block, flags, c_lo, ncoef, spos = payload
@@ -1037,7 +1037,7 @@ def test_plan_flops_closed_form():
         # (diagonals kept as records of their own: folded into a 2x2 matrix
         # they turn it into a general one)
         plan = fusion.HostPlan(gl, tuple(range(n)), tile_bits=8, low_bits=3, block=16,
-                               fuse_width=1, mode='reg', fold_diag=False)
+                               fuse_width=1, mode='reg', merge_records=False)
         assert plan.flops() == [per_amp * 2 ** n], [g.name for g in gl]
 
 
