@@ -377,16 +377,20 @@ __device__ __forceinline__ unsigned d1_offset(const int4 sub, unsigned fm, uint6
   const bool flag = code & kD1Flip;
   unsigned off = flag ? (((fm >> (code & 3)) & 1u) << 2) : 0u;
   const unsigned w = (unsigned)sub.w;
-  unsigned sel = 0;
+  if (w) {                                  // uniform: most records have none
+    flagged_branch();
+    unsigned sel = 0;
 #pragma unroll
-  for (int j = 0; j < 3; ++j) {
-    if (j < (int)(w & 3u)) {
-      const unsigned p = (w >> (2 + 7 * j)) & 127u;
-      const unsigned b = p < 64u ? (unsigned)((gbase >> p) & 1u) : (egrp >> (p - 64u)) & 1u;
-      sel |= b << j;
+    for (int j = 0; j < 3; ++j) {
+      if (j < (int)(w & 3u)) {
+        const unsigned p = (w >> (2 + 7 * j)) & 127u;
+        const unsigned b = p < 64u ? (unsigned)((gbase >> p) & 1u) : (egrp >> (p - 64u)) & 1u;
+        sel |= b << j;
+      }
     }
+    off += sel << (flag ? 3 : 2);
   }
-  return off + (sel << (flag ? 3 : 2));
+  return off;
 }
 
 // Register-bit mask of a lane flip word (0 where its selectors do not hold).
