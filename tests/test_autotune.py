@@ -30,6 +30,8 @@ def test_circuit_defaults(cache):
     assert cfg['tile_bits'] == 12 and cfg['block'] == 256
     small = autotune.choose(9)
     assert small['tile_bits'] == 9 and (1 << 9) >= small['block']
+    # 16-amplitude contiguous runs once the state outgrows the 256 MiB cache
+    assert [autotune.choose(n)['low_bits'] for n in (9, 16, 24, 25, 28)] == [3, 3, 3, 4, 4]
 
 
 @pytest.mark.gpu
